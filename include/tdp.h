@@ -1,0 +1,79 @@
+/*
+ * tropidist_amd -- C-ABI of the MI355X (gfx950) exact distance evaluators, libtdp.so.
+ *
+ * Plain pointers and sizes only.  Every pointer named d_* or documented as "device" is HIP device memory owned by
+ * the caller (the reference's ownership model: numpy buffers owned by Python, cdist/kernel.py:101-104,
+ * cdist/tree_dp.py:435-436, cdist/sparse_dp.py:523-527); `stream` is a hipStream_t (NULL = the null stream).
+ * Every entry point returns an int status (TDP_OK = 0); tdp_last_error() describes the last failure of the
+ * calling thread.  Nothing here keeps process-global mutable state: calls on different streams and buffers are
+ * independent.
+ *
+ * Each declaration cites the reference interface it replaces (file:line under the reference tree).
+ */
+#ifndef TDP_H
+#define TDP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDP_OK 0
+#define TDP_EINVAL 1          /* invalid argument (sizes, masks, malformed op stream) */
+#define TDP_EHIP 2            /* a HIP runtime call failed */
+#define TDP_ENOMEM 3          /* a device allocation failed (table handles only) */
+#define TDP_EDEADLINE 4       /* stopped at the caller's deadline */
+#define TDP_ECAPACITY 5       /* a fixed-capacity structure (hash table, output buffer) is full */
+
+/* Message of the calling thread's last failure ("" if none).  No reference counterpart (the reference C returns
+ * no error codes, cdist/viterbi.c; Python raises, cdist/kernel.py:18-25). */
+const char *tdp_last_error(void);
+
+/* Version string and the number of compute units of the current device (replaces dp_num_threads,
+ * cdist/viterbi.c:195-201, as the parallelism report). */
+const char *tdp_version(void);
+int tdp_num_cus(int *cus);
+
+/* ------------------------------------------------------------------ dense trellis, one op per call
+ * All act on a uint8 device table d_T of n = 2^b entries, INF = 255, additions saturating at 255. */
+
+/* T[s] <- min(T[s], T[s ^ h] + 1) for all s, in place.  Replaces dp_update (cdist/viterbi.c:72-118). */
+int tdp_update(uint8_t *d_T, uint64_t n, uint64_t h, void *stream);
+
+/* Same with + w (w >= 1), saturating.  Replaces dp_update_w (cdist/viterbi.c:203-219). */
+int tdp_update_w(uint8_t *d_T, uint64_t n, uint64_t h, int w, void *stream);
+
+/* New top bit flipped by the current column: T[n + (s ^ hl)] = T[s] + w; the buffer holds 2n entries.
+ * Replaces dp_expand_update (cdist/viterbi.c:120-138), with a weight. */
+int tdp_expand_update(uint8_t *d_T, uint64_t n, uint64_t hl, int w, void *stream);
+
+/* Keep the states whose bit `pos` equals `target`, dropping that bit: d_dst[k] = d_src[insert(k)], k < n/2.
+ * d_dst and d_src must not overlap.  *n_out = n/2 (n if n < 2^(pos+1), as the reference).  Out-of-place form of
+ * dp_select (cdist/viterbi.c:149-172). */
+int tdp_select(uint8_t *d_dst, const uint8_t *d_src, uint64_t n, int pos, int target, uint64_t *n_out,
+               void *stream);
+
+/* d_T[0..n) = v.  Replaces dp_fill (cdist/viterbi.c:174-179). */
+int tdp_fill(uint8_t *d_T, uint64_t n, uint8_t v, void *stream);
+
+/* d_hist[v] = number of entries equal to v (256 uint64 device counters, overwritten).  Replaces dp_hist
+ * (cdist/viterbi.c:181-193). */
+int tdp_hist(const uint8_t *d_T, uint64_t n, uint64_t *d_hist, void *stream);
+
+/* ------------------------------------------------------------------ dense trellis, whole plan
+ * Op stream: nops triples of uint64 (code, a, b) in host memory: (0, 0, 0) expand, (1, mask, weight) update,
+ * (2, pos, target) select -- the op list of cdist/trellis.py:125-182 with an update weight.  The loop of
+ * run_plan (cdist/kernel.py:94-138): an expand followed by an update whose top bit is the new bit becomes one
+ * expand-update, otherwise the new half is filled with 255.  d_T0 and d_T1 hold 2^peak_bits bytes each (the second
+ * receives selects).  *value = the final entry (255 = cost >= 255, the caller raises OverflowError as
+ * final_value does, cdist/kernel.py:18-25).  Executed one streaming kernel per op on `stream`. */
+int tdp_run_plan(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value,
+                 void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDP_H */

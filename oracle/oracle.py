@@ -1,0 +1,231 @@
+"""ORACLE -- test infrastructure only (tests/, __graft_entry__.smoke(), bench.py's cpu_baseline leg).
+
+CPU restatements of the reference's exact distance evaluators, used to check the HIP product path:
+
+  run_plan_numpy(plan)          numpy executor of a Plan (cdist/kernel.py:72-91), extended to column weights
+  run_plan_c(plan, impl)        the op-stream loop of cdist/kernel.py:94-138 over the C kernels of
+                                impl="oracle" (liboracle.so, this project's scalar restatement) or
+                                impl="ref" (oracle/_ref/libviterbi.so, the reference's viterbi.c compiled as is)
+  brute(h, l, obs_mask, w)      exhaustive minimum over column subsets (tests/test_dp.py:12-23)
+  brute_affine(columns, w)      all target minima (tests/test_tree_dp.py:44-54)
+  tree joins, sparse steps, MITM tables: ctypes bindings of the same libraries
+
+Pinning: tests/test_oracle_*.py check every function here against the reference's compiled C (_ref), brute force,
+the chain closed form and the golden fixtures of tests/golden.  Nothing in tropidist_amd/ imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import itertools
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+INF = 255
+_LIBS = {}
+
+
+def _load(path):
+    if path not in _LIBS:
+        if not os.path.exists(path):
+            raise FileNotFoundError("%s is not built (make -C oracle)" % path)
+        _LIBS[path] = ctypes.CDLL(path)
+    return _LIBS[path]
+
+
+def have_ref():
+    return os.path.exists(os.path.join(HERE, "_ref", "libviterbi.so"))
+
+
+def olib():
+    L = _load(os.path.join(HERE, "liboracle.so"))
+    if not getattr(L, "_typed", False):
+        vp, u64, i64, ci = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int
+        for name in ("or_dp_update",):
+            getattr(L, name).argtypes = [vp, u64, u64]
+        L.or_dp_update_w.argtypes = [vp, u64, u64, ci]
+        L.or_dp_expand_update.argtypes = [vp, u64, u64]
+        L.or_dp_expand_update_w.argtypes = [vp, u64, u64, ci]
+        L.or_dp_select.argtypes = [vp, u64, ci, ci]
+        L.or_dp_select.restype = u64
+        L.or_dp_fill.argtypes = [vp, u64, ctypes.c_uint8]
+        L.or_dp_hist.argtypes = [vp, u64, vp]
+        L.or_run_ops.argtypes = [vp, i64, vp, ci]
+        L.or_run_ops.restype = ci
+        for b in (8, 16, 32, 64):
+            f = getattr(L, "or_tree_join_u%d" % b)
+            f.argtypes = [vp] * 5 + [ctypes.c_uint] * 2 + [u64] * 3
+        L.or_sp_candidates.argtypes = [vp, i64, u64, u64, u64, ci, ci, vp, vp, i64, u64, ci, ci, ci, vp, vp, ci]
+        L.or_sp_candidates.restype = i64
+        L.or_sp_merge.argtypes = [vp, i64, vp, i64, u64, u64, ci, ci, ci, vp, vp, i64, u64, ci, ci, vp, vp,
+                                  ctypes.POINTER(i64)]
+        L.or_sp_merge.restype = i64
+        L.or_sp_join.argtypes = [vp, i64, vp, i64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.or_sp_join.restype = ci
+        L.or_sp_hist.argtypes = [vp, i64, vp]
+        L.or_tab_create.argtypes = [ci, ci]
+        L.or_tab_create.restype = vp
+        L.or_tab_free.argtypes = [vp]
+        L.or_tab_count.argtypes = [vp]
+        L.or_tab_count.restype = i64
+        L.or_tab_build.argtypes = [vp, ci, vp, vp, vp, ci]
+        L.or_tab_lookup.argtypes = [vp, u64, u64, u64, ci]
+        L.or_tab_lookup.restype = ci
+        L._typed = True
+    return L
+
+
+def rlib(name):
+    """The reference's own C kernel compiled into oracle/_ref (name: viterbi, tree_dp, sparse_dp, gmitm), with
+    the argument types of its ctypes bindings (cdist/kernel.py:52-69, tree_dp.py:221-232, sparse_dp.py:484-499,
+    search.py:54-69)."""
+    L = _load(os.path.join(HERE, "_ref", "lib%s.so" % name))
+    if getattr(L, "_typed", False):
+        return L
+    vp, u64, i64, ci = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int
+    if name == "viterbi":
+        L.dp_update.argtypes = [vp, u64, u64]
+        L.dp_update_w.argtypes = [vp, u64, u64, ci]
+        L.dp_expand_update.argtypes = [vp, u64, u64]
+        L.dp_select.argtypes = [vp, u64, ci, ci]
+        L.dp_select.restype = u64
+        L.dp_fill.argtypes = [vp, u64, ctypes.c_uint8]
+        L.dp_hist.argtypes = [vp, u64, vp]
+        L.dp_num_threads.restype = ci
+    elif name == "tree_dp":
+        for b in (8, 16, 32, 64):
+            f = getattr(L, "tree_join_u%d" % b)
+            f.argtypes = [vp] * 5 + [ctypes.c_uint] * 2 + [u64] * 3
+        L.tree_move.argtypes = [vp, vp, ctypes.c_size_t]
+    elif name == "sparse_dp":
+        L.sp_candidates.argtypes = [vp, i64, u64, u64, u64, ci, ci, vp, vp, i64, u64, ci, ci, ci, vp, vp, ci]
+        L.sp_candidates.restype = i64
+        L.sp_merge.argtypes = [vp, i64, vp, i64, u64, u64, ci, ci, ci, vp, vp, i64, u64, ci, ci, vp, vp,
+                               ctypes.POINTER(i64)]
+        L.sp_merge.restype = i64
+        L.sp_join.argtypes = [vp, i64, vp, i64, u64, ctypes.POINTER(u64), ctypes.POINTER(u64)]
+        L.sp_join.restype = ci
+        L.sp_hist.argtypes = [vp, i64, vp]
+    elif name == "gmitm":
+        L.tab_create.restype = vp
+        L.tab_create.argtypes = [ci, ci]
+        L.tab_free.argtypes = [vp]
+        L.tab_count.restype = i64
+        L.tab_count.argtypes = [vp]
+        L.tab_build.argtypes = [vp, ci, vp, vp, vp, ci]
+    L._typed = True
+    return L
+
+
+def ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def final_value(entry):
+    """cdist/kernel.py:18-25: 255 of a feasible plan means a cost >= 255."""
+    v = int(entry)
+    if v == INF:
+        raise OverflowError("the distance is at least %d, beyond the 8-bit entries of the dense DP" % INF)
+    return v
+
+
+def _weight(op):
+    return op[2] if len(op) > 2 else 1
+
+
+def run_plan_numpy(plan):
+    """cdist/kernel.py:72-91 with weighted updates: None if infeasible, OverflowError at 255."""
+    if not plan.feasible:
+        return None
+    T = np.zeros(1, dtype=np.uint8)
+    for op in plan.ops:
+        if op[0] == "expand":
+            T = np.concatenate([T, np.full(T.size, INF, dtype=np.uint8)])
+        elif op[0] == "update":
+            other = T[np.arange(T.size, dtype=np.int64) ^ op[1]].astype(np.int64) + _weight(op)
+            T = np.minimum(T, np.minimum(other, INF)).astype(np.uint8)
+        else:
+            _, pos, target = op
+            T = T[((np.arange(T.size, dtype=np.int64) >> pos) & 1) == target]
+    assert T.size == 1
+    return final_value(T[0])
+
+
+def run_plan_c(plan, impl="oracle"):
+    """The run_plan loop (cdist/kernel.py:94-138) on CPU C kernels: impl "oracle" runs liboracle's or_run_ops over
+    the encoded op stream; impl "ref" drives the reference's own libviterbi.so op by op (unit weights call
+    dp_update / dp_expand_update, weighted updates dp_update_w; a weighted expand-update has no reference kernel and
+    is done as dp_fill + dp_update_w on the doubled table, which computes the same new half)."""
+    if not plan.feasible:
+        return None
+    buf = np.empty(1 << plan.peak_bits, dtype=np.uint8)
+    if impl == "oracle":
+        v = olib().or_run_ops(ptr(plan.encode()), len(plan.ops), ptr(buf), plan.peak_bits)
+        if v < 0:
+            raise ValueError("malformed op stream")
+        return final_value(v)
+    L = rlib("viterbi")
+    base = buf.ctypes.data
+    buf[0] = 0
+    n, i, ops = 1, 0, plan.ops
+    while i < len(ops):
+        op = ops[i]
+        if op[0] == "expand":
+            nxt = ops[i + 1] if i + 1 < len(ops) else None
+            if nxt is not None and nxt[0] == "update" and (nxt[1] >> (n.bit_length() - 1)) == 1:
+                if _weight(nxt) == 1:
+                    L.dp_expand_update(base, n, nxt[1] ^ n)
+                else:
+                    L.dp_fill(base + n, n, INF)
+                    L.dp_update_w(base, 2 * n, nxt[1], _weight(nxt))
+                n <<= 1
+                i += 2
+                continue
+            L.dp_fill(base + n, n, INF)
+            n <<= 1
+        elif op[0] == "update":
+            if _weight(op) == 1:
+                L.dp_update(base, n, op[1])
+            else:
+                L.dp_update_w(base, n, op[1], _weight(op))
+        else:
+            n = L.dp_select(base, n, op[1], op[2])
+        i += 1
+    assert n == 1
+    return final_value(buf[0])
+
+
+def brute(h, l, obs_mask, w=None):
+    """Least total cost of a column set with H e = 0 and odd parity of (L e) & obs_mask; None if none
+    (tests/test_dp.py:12-23, extended to costs: all subsets are scanned)."""
+    n = len(h)
+    best = None
+    for size in range(1, n + 1):
+        for sup in itertools.combinations(range(n), size):
+            s = lg = 0
+            for j in sup:
+                s ^= h[j]
+                lg ^= l[j]
+            if s == 0 and bin(lg & obs_mask).count("1") & 1:
+                c = size if w is None else sum(w[j] for j in sup)
+                if best is None or c < best:
+                    best = c
+        if w is None and best is not None:
+            return best
+    return best
+
+
+def brute_affine(columns, w=None):
+    """target -> least cost of a column set summing to it (tests/test_tree_dp.py:44-54)."""
+    result = {}
+    for bits in itertools.product((0, 1), repeat=len(columns)):
+        t = 0
+        c = 0
+        for j, (b, col) in enumerate(zip(bits, columns)):
+            if b:
+                t ^= col
+                c += 1 if w is None else w[j]
+        if t not in result or c < result[t]:
+            result[t] = c
+    return result

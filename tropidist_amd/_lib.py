@@ -1,0 +1,92 @@
+"""ctypes binding of libtdp.so (include/tdp.h) and the device-memory plumbing shared by the host modules.
+
+The library must be built in-tree (python -m tropidist_amd.build); loading fails loudly when it is missing or when
+no HIP device is visible -- this package has no CPU execution path."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtdp.so")
+
+TDP_OK, TDP_EINVAL, TDP_EHIP, TDP_ENOMEM, TDP_EDEADLINE, TDP_ECAPACITY = range(6)
+
+_LIB = None
+vp, u64, i64, ci, u8 = ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int64, ctypes.c_int, ctypes.c_uint8
+
+# name -> (restype, argtypes); every entry point returns an int status
+SIGNATURES = {
+    "tdp_last_error": (ctypes.c_char_p, []),
+    "tdp_version": (ctypes.c_char_p, []),
+    "tdp_num_cus": (ci, [ctypes.POINTER(ci)]),
+    "tdp_update": (ci, [vp, u64, u64, vp]),
+    "tdp_update_w": (ci, [vp, u64, u64, ci, vp]),
+    "tdp_expand_update": (ci, [vp, u64, u64, ci, vp]),
+    "tdp_select": (ci, [vp, vp, u64, ci, ci, ctypes.POINTER(u64), vp]),
+    "tdp_fill": (ci, [vp, u64, u8, vp]),
+    "tdp_hist": (ci, [vp, u64, vp, vp]),
+    "tdp_run_plan": (ci, [vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
+}
+
+
+class TdpError(RuntimeError):
+    """A libtdp call failed; .code is its status."""
+
+    def __init__(self, message, code):
+        super().__init__(message)
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libtdp.so is not built: run `python -m tropidist_amd.build` (no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc, what=""):
+    if rc != TDP_OK:
+        msg = lib().tdp_last_error().decode(errors="replace")
+        if rc == TDP_EDEADLINE:
+            raise TimeoutError("%s: %s" % (what, msg))
+        if rc == TDP_ENOMEM:
+            raise MemoryError("%s: %s" % (what, msg))
+        raise TdpError("%s failed (status %d): %s" % (what, rc, msg), rc)
+
+
+# ------------------------------------------------------------------------------------------ device plumbing
+_torch = None
+
+
+def torch():
+    """torch with a visible HIP device (device memory and streams only)."""
+    global _torch
+    if _torch is None:
+        import torch as t
+        if not t.cuda.is_available():
+            raise RuntimeError("tropidist_amd needs a HIP device (torch.cuda.is_available() is False)")
+        _torch = t
+    return _torch
+
+
+def device_empty(nbytes, dtype=None):
+    t = torch()
+    return t.empty(max(1, int(nbytes)), dtype=dtype or t.uint8, device="cuda")
+
+
+def stream_handle(stream=None):
+    t = torch()
+    s = stream if stream is not None else t.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def dptr(tensor):
+    return ctypes.c_void_p(tensor.data_ptr())

@@ -1,0 +1,317 @@
+// Dense min-plus trellis DP on gfx950: one streaming kernel per op (update, expand-update, select, fill, hist)
+// and a host-side plan executor that launches them on one stream.
+//
+// Table: uint8 costs, INF = 255, n = 2^b entries in HBM.  Every kernel moves 16-byte chunks (one dwordx4 per lane,
+// 1 KiB per wave, fully coalesced).  An update with mask h pairs s with s ^ h; with hb = top bit of h:
+//   hb >= 4: chunk x (bit hb clear) pairs with chunk (x ^ (hl >> 4)) | half, and inside the pair byte i of one
+//            chunk meets byte i ^ (hl & 15) of the other -- a register byte permutation;
+//   hb < 4:  both members of every pair lie in the same chunk.
+// Semantics follow cdist/viterbi.c:72-219 (saturating min-plus butterflies on old values).
+#include "tdp_common.h"
+
+namespace tdp {
+
+static __thread char g_err[512];
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+constexpr int kBlock = 256;
+
+static inline unsigned grid_for(uint64_t items) {
+    uint64_t g = (items + kBlock - 1) / kBlock;
+    return (unsigned)(g ? g : 1);
+}
+
+// ------------------------------------------------------------------------------------------------- kernels
+// hb >= 4: one thread per chunk pair.  c enumerates the chunks with bit (hb - 4) of the chunk index clear.
+__global__ __launch_bounds__(kBlock) void k_update_hi(uint8_t *__restrict__ T, uint64_t npairs, int hb4,
+                                                      uint64_t hl_chunk, unsigned m, uint32_t w) {
+    uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= npairs) return;
+    uint64_t low = c & ((1ull << hb4) - 1);
+    uint64_t xi = ((c >> hb4) << (hb4 + 1)) | low;
+    uint64_t yi = (xi ^ hl_chunk) | (1ull << hb4);
+    uint4 *P = reinterpret_cast<uint4 *>(T);
+    uint4 x = P[xi], y = P[yi];
+    uint4 xp = xor_bytes16(x, m), yp = xor_bytes16(y, m);
+    P[xi] = min16(x, sat_add16(yp, w));
+    P[yi] = min16(y, sat_add16(xp, w));
+}
+
+// hb < 4: one thread per chunk; the pair partner of byte i is byte i ^ h of the same chunk.
+__global__ __launch_bounds__(kBlock) void k_update_lo(uint8_t *__restrict__ T, uint64_t nchunks, unsigned h,
+                                                      uint32_t w) {
+    uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nchunks) return;
+    uint4 *P = reinterpret_cast<uint4 *>(T);
+    uint4 x = P[c];
+    P[c] = min16(x, sat_add16(xor_bytes16(x, h), w));
+}
+
+// Tables of fewer than 16 entries: one thread, scalar pairs.
+__global__ void k_update_tiny(uint8_t *T, uint32_t n, uint32_t h, uint32_t w) {
+    uint32_t hb = 1u << (31 - __builtin_clz(h));
+    for (uint32_t s = 0; s < n; s++) {
+        if (s & hb) continue;
+        uint32_t t = s ^ h;
+        uint8_t a = T[s], b = T[t];
+        uint8_t a1 = sat_add1(a, w), b1 = sat_add1(b, w);
+        T[s] = a < b1 ? a : b1;
+        T[t] = b < a1 ? b : a1;
+    }
+}
+
+// U[s ^ hl] = T[s] + w, U = T + n: chunk c of T lands on chunk c ^ (hl >> 4) of U, permuted by hl & 15.
+__global__ __launch_bounds__(kBlock) void k_expand(uint8_t *__restrict__ T, uint64_t nchunks, uint64_t hl_chunk,
+                                                   unsigned m, uint32_t w) {
+    uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nchunks) return;
+    const uint4 *S = reinterpret_cast<const uint4 *>(T);
+    uint4 *U = reinterpret_cast<uint4 *>(T) + nchunks;
+    U[c ^ hl_chunk] = sat_add16(xor_bytes16(S[c], m), w);
+}
+
+__global__ void k_expand_tiny(uint8_t *T, uint32_t n, uint32_t hl, uint32_t w) {
+    for (uint32_t s = 0; s < n; s++) T[n + (s ^ hl)] = sat_add1(T[s], w);
+}
+
+// dst chunk c <- src chunk with bit (pos - 4) inserted = target (pos >= 4).
+__global__ __launch_bounds__(kBlock) void k_select_hi(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src,
+                                                      uint64_t nchunks, int pos4, uint32_t target) {
+    uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nchunks) return;
+    uint64_t low = c & ((1ull << pos4) - 1);
+    uint64_t si = ((c >> pos4) << (pos4 + 1)) | ((uint64_t)target << pos4) | low;
+    reinterpret_cast<uint4 *>(dst)[c] = reinterpret_cast<const uint4 *>(src)[si];
+}
+
+// pos < 4: dst chunk c gathers the 16 bytes with bit pos == target out of src chunks 2c and 2c + 1.
+__global__ __launch_bounds__(kBlock) void k_select_lo(uint8_t *__restrict__ dst, const uint8_t *__restrict__ src,
+                                                      uint64_t nchunks, int pos, uint32_t target) {
+    uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c >= nchunks) return;
+    const uint4 *S = reinterpret_cast<const uint4 *>(src);
+    uint4 a = S[2 * c], b = S[2 * c + 1];
+    uint32_t in[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t out[4] = {0, 0, 0, 0};
+    uint32_t lowmask = (1u << pos) - 1;
+#pragma unroll
+    for (uint32_t k = 0; k < 16; k++) {
+        uint32_t s = ((k & ~lowmask) << 1) | (target << pos) | (k & lowmask);   // 0..31
+        uint32_t byte = (in[s >> 2] >> (8 * (s & 3))) & 0xFFu;
+        out[k >> 2] |= byte << (8 * (k & 3));
+    }
+    reinterpret_cast<uint4 *>(dst)[c] = make_uint4(out[0], out[1], out[2], out[3]);
+}
+
+__global__ void k_select_tiny(uint8_t *dst, const uint8_t *src, uint32_t half, int pos, uint32_t target) {
+    uint32_t lowmask = (1u << pos) - 1;
+    for (uint32_t k = 0; k < half; k++) dst[k] = src[((k & ~lowmask) << 1) | (target << pos) | (k & lowmask)];
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(uint8_t *__restrict__ T, uint64_t nchunks, uint32_t v4) {
+    uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (c < nchunks) reinterpret_cast<uint4 *>(T)[c] = make_uint4(v4, v4, v4, v4);
+}
+
+__global__ void k_fill_tiny(uint8_t *T, uint32_t n, uint8_t v) {
+    for (uint32_t i = 0; i < n; i++) T[i] = v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_hist(const uint8_t *__restrict__ T, uint64_t n,
+                                                 unsigned long long *__restrict__ hist) {
+    __shared__ uint32_t bins[256];
+    bins[threadIdx.x] = 0;
+    __syncthreads();
+    uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) atomicAdd(&bins[T[i]], 1u);
+    __syncthreads();
+    if (bins[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)bins[threadIdx.x]);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------------------------------------- launchers
+static int launch_update(uint8_t *T, uint64_t n, uint64_t h, uint32_t w, hipStream_t st) {
+    if (h == 0 || n < 2) return TDP_OK;
+    TDP_REQUIRE(h < n, "update mask 0x%llx outside a table of %llu entries", (unsigned long long)h,
+                (unsigned long long)n);
+    if (w > 255) w = 255;
+    int hb = top_bit(h);
+    if (n < 16) {
+        k_update_tiny<<<1, 1, 0, st>>>(T, (uint32_t)n, (uint32_t)h, w);
+    } else if (hb >= 4) {
+        uint64_t hl = h ^ (1ull << hb);
+        uint64_t npairs = n >> 5;
+        k_update_hi<<<grid_for(npairs), kBlock, 0, st>>>(T, npairs, hb - 4, hl >> 4, (unsigned)(hl & 15), w);
+    } else {
+        uint64_t nchunks = n >> 4;
+        k_update_lo<<<grid_for(nchunks), kBlock, 0, st>>>(T, nchunks, (unsigned)h, w);
+    }
+    TDP_LAUNCH_CHECK();
+    return TDP_OK;
+}
+
+static int launch_expand(uint8_t *T, uint64_t n, uint64_t hl, uint32_t w, hipStream_t st) {
+    TDP_REQUIRE(hl < n, "expand mask 0x%llx outside a table of %llu entries", (unsigned long long)hl,
+                (unsigned long long)n);
+    if (w > 255) w = 255;
+    if (n < 16) {
+        k_expand_tiny<<<1, 1, 0, st>>>(T, (uint32_t)n, (uint32_t)hl, w);
+    } else {
+        uint64_t nchunks = n >> 4;
+        k_expand<<<grid_for(nchunks), kBlock, 0, st>>>(T, nchunks, hl >> 4, (unsigned)(hl & 15), w);
+    }
+    TDP_LAUNCH_CHECK();
+    return TDP_OK;
+}
+
+static int launch_fill(uint8_t *T, uint64_t n, uint8_t v, hipStream_t st) {
+    if (n == 0) return TDP_OK;
+    if (n < 16 || (n & 15)) {
+        TDP_REQUIRE(n < (1ull << 32), "fill of %llu entries must be a multiple of 16", (unsigned long long)n);
+        k_fill_tiny<<<1, 1, 0, st>>>(T, (uint32_t)n, v);
+    } else {
+        k_fill<<<grid_for(n >> 4), kBlock, 0, st>>>(T, n >> 4, 0x01010101u * v);
+    }
+    TDP_LAUNCH_CHECK();
+    return TDP_OK;
+}
+
+static int launch_select(uint8_t *dst, const uint8_t *src, uint64_t n, int pos, int target, uint64_t *n_out,
+                         hipStream_t st) {
+    TDP_REQUIRE(pos >= 0 && pos < 63 && (target == 0 || target == 1), "select pos %d target %d", pos, target);
+    if ((n >> (pos + 1)) == 0) {   // the reference returns n unchanged (cdist/viterbi.c:157-158)
+        *n_out = n;
+        if (dst != src) TDP_HIP_CHECK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, st));
+        return TDP_OK;
+    }
+    uint64_t half = n >> 1;
+    if (half < 16) {
+        k_select_tiny<<<1, 1, 0, st>>>(dst, src, (uint32_t)half, pos, (uint32_t)target);
+    } else if (pos >= 4) {
+        k_select_hi<<<grid_for(half >> 4), kBlock, 0, st>>>(dst, src, half >> 4, pos - 4, (uint32_t)target);
+    } else {
+        k_select_lo<<<grid_for(half >> 4), kBlock, 0, st>>>(dst, src, half >> 4, pos, (uint32_t)target);
+    }
+    TDP_LAUNCH_CHECK();
+    *n_out = half;
+    return TDP_OK;
+}
+
+static bool pow2(uint64_t n) { return n && !(n & (n - 1)); }
+
+}  // namespace tdp
+
+using namespace tdp;
+
+extern "C" {
+
+const char *tdp_last_error(void) { return g_err; }
+
+const char *tdp_version(void) { return "tropidist_amd 0.1 gfx950"; }
+
+int tdp_num_cus(int *cus) {
+    int dev = 0;
+    TDP_HIP_CHECK(hipGetDevice(&dev));
+    TDP_HIP_CHECK(hipDeviceGetAttribute(cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return TDP_OK;
+}
+
+int tdp_update(uint8_t *d_T, uint64_t n, uint64_t h, void *stream) {
+    TDP_REQUIRE(pow2(n), "table size %llu is not a power of two", (unsigned long long)n);
+    return launch_update(d_T, n, h, 1, as_stream(stream));
+}
+
+int tdp_update_w(uint8_t *d_T, uint64_t n, uint64_t h, int w, void *stream) {
+    TDP_REQUIRE(pow2(n), "table size %llu is not a power of two", (unsigned long long)n);
+    TDP_REQUIRE(w >= 1, "weight %d < 1", w);
+    return launch_update(d_T, n, h, (uint32_t)w, as_stream(stream));
+}
+
+int tdp_expand_update(uint8_t *d_T, uint64_t n, uint64_t hl, int w, void *stream) {
+    TDP_REQUIRE(pow2(n), "table size %llu is not a power of two", (unsigned long long)n);
+    TDP_REQUIRE(w >= 1, "weight %d < 1", w);
+    return launch_expand(d_T, n, hl, (uint32_t)w, as_stream(stream));
+}
+
+int tdp_select(uint8_t *d_dst, const uint8_t *d_src, uint64_t n, int pos, int target, uint64_t *n_out,
+               void *stream) {
+    TDP_REQUIRE(pow2(n), "table size %llu is not a power of two", (unsigned long long)n);
+    TDP_REQUIRE(d_dst + n / 2 <= d_src || d_src + n <= d_dst, "select source and destination overlap");
+    return launch_select(d_dst, d_src, n, pos, target, n_out, as_stream(stream));
+}
+
+int tdp_fill(uint8_t *d_T, uint64_t n, uint8_t v, void *stream) { return launch_fill(d_T, n, v, as_stream(stream)); }
+
+int tdp_hist(const uint8_t *d_T, uint64_t n, uint64_t *d_hist, void *stream) {
+    hipStream_t st = as_stream(stream);
+    k_zero_u64<<<1, 256, 0, st>>>(reinterpret_cast<unsigned long long *>(d_hist), 256);
+    TDP_LAUNCH_CHECK();
+    if (n == 0) return TDP_OK;
+    uint64_t blocks = (n + kBlock * 16 - 1) / (kBlock * 16);
+    if (blocks > 4096) blocks = 4096;
+    k_hist<<<(unsigned)blocks, kBlock, 0, st>>>(d_T, n, reinterpret_cast<unsigned long long *>(d_hist));
+    TDP_LAUNCH_CHECK();
+    return TDP_OK;
+}
+
+int tdp_run_plan(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value,
+                 void *stream) {
+    hipStream_t st = as_stream(stream);
+    TDP_REQUIRE(peak_bits >= 0 && peak_bits <= 40, "peak_bits %d out of range", peak_bits);
+    TDP_REQUIRE(nops >= 0 && (nops == 0 || ops), "bad op stream");
+    const uint64_t cap = 1ull << peak_bits;
+    uint8_t *cur = d_T0, *other = d_T1;
+    uint64_t n = 1;
+    int rc = launch_fill(cur, 1, 0, st);
+    if (rc) return rc;
+    for (int64_t i = 0; i < nops; i++) {
+        uint64_t code = ops[3 * i], a = ops[3 * i + 1], b = ops[3 * i + 2];
+        if (code == 0) {
+            TDP_REQUIRE(2 * n <= cap, "op %lld: expand beyond 2^%d entries", (long long)i, peak_bits);
+            const uint64_t *nx = ops + 3 * (i + 1);
+            if (i + 1 < nops && nx[0] == 1 && (nx[1] >> top_bit(n)) == 1) {
+                TDP_REQUIRE(nx[2] >= 1, "op %lld: weight < 1", (long long)(i + 1));
+                rc = launch_expand(cur, n, nx[1] ^ n, (uint32_t)(nx[2] > 255 ? 255 : nx[2]), st);
+                i++;
+            } else {
+                rc = launch_fill(cur + n, n, 255, st);
+            }
+            n <<= 1;
+        } else if (code == 1) {
+            TDP_REQUIRE(a < n, "op %lld: update mask 0x%llx outside 2^%d entries", (long long)i,
+                        (unsigned long long)a, top_bit(n));
+            TDP_REQUIRE(b >= 1, "op %lld: weight < 1", (long long)i);
+            rc = launch_update(cur, n, a, (uint32_t)(b > 255 ? 255 : b), st);
+        } else if (code == 2) {
+            TDP_REQUIRE(n >= 2 && a < (uint64_t)top_bit(n) && b <= 1, "op %lld: select pos %llu on 2^%d entries",
+                        (long long)i, (unsigned long long)a, top_bit(n));
+            uint64_t nn = 0;
+            rc = launch_select(other, cur, n, (int)a, (int)b, &nn, st);
+            uint8_t *t = cur;
+            cur = other;
+            other = t;
+            n = nn;
+        } else {
+            TDP_REQUIRE(false, "op %lld: unknown code %llu", (long long)i, (unsigned long long)code);
+        }
+        if (rc) return rc;
+    }
+    TDP_REQUIRE(n == 1, "op stream ends with %llu entries, not 1", (unsigned long long)n);
+    uint8_t v = 0;
+    TDP_HIP_CHECK(hipMemcpyAsync(&v, cur, 1, hipMemcpyDeviceToHost, st));
+    TDP_HIP_CHECK(hipStreamSynchronize(st));
+    *value = v;
+    return TDP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,49 @@
+"""Executor of a trellis Plan on the GPU (the call surface of the reference's cdist/kernel.py).
+
+    INF, final_value(entry), run_plan(plan, log=None, progress_every=0.0) -> (value | None, stats)
+
+run_plan hands the whole op stream to libtdp's plan executor (tdp_run_plan), which streams every op over the uint8
+table in HBM on one HIP stream; control comes back to Python once per plan, not once per op as in the reference
+(cdist/kernel.py:112-135).  The reference's numpy executor run_plan_numpy is a CPU path and lives in the oracle
+(oracle/oracle.py) as a checker only."""
+from __future__ import annotations
+
+import ctypes
+import time
+
+from . import _lib
+
+INF = 255               # uint8 table entries; additions saturate here
+
+
+def final_value(entry):
+    """Minimum cost from the last table entry of a feasible plan (cdist/kernel.py:18-25): entries are min(true, 255),
+    so 255 means a cost of at least 255 and raises OverflowError; never None."""
+    v = int(entry)
+    if v == INF:
+        raise OverflowError("the distance is at least %d, beyond the 8-bit entries of the dense DP" % INF)
+    return v
+
+
+def run_plan(plan, log=None, progress_every=0.0, stream=None):
+    """Run the plan on the GPU.  Returns (min cost or None if infeasible, stats); OverflowError if the cost is
+    >= 255 (cdist/kernel.py:94-138).  stats: seconds, peak_bits, cus, executor."""
+    if not plan.feasible:
+        return None, {"feasible": False}
+    L = _lib.lib()
+    size = 1 << plan.peak_bits
+    T0 = _lib.device_empty(size)
+    T1 = _lib.device_empty(size)
+    ops = plan.encode()
+    value = ctypes.c_int(-1)
+    t0 = time.time()
+    rc = L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(plan.ops), plan.peak_bits, _lib.dptr(T0),
+                        _lib.dptr(T1), ctypes.byref(value), _lib.stream_handle(stream))
+    _lib.check(rc, "tdp_run_plan")
+    seconds = time.time() - t0
+    if log:
+        log("  plan of %d ops on 2^%d entries: %.4f s" % (len(plan.ops), plan.peak_bits, seconds))
+    cus = ctypes.c_int(0)
+    _lib.check(L.tdp_num_cus(ctypes.byref(cus)), "tdp_num_cus")
+    return final_value(value.value), {"seconds": seconds, "peak_bits": plan.peak_bits, "cus": cus.value,
+                                      "executor": "per-op"}

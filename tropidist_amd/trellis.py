@@ -1,0 +1,208 @@
+"""Linear rank decomposition (syndrome trellis) of a distance query, and its DP plan.
+
+Restates the planner of the reference (cdist/trellis.py:16-193): for a column order, rows of the check matrix are
+brought into minimal span form (distinct starts, distinct ends), the target-1 row is shortened against them, and
+the plan walks the columns, opening a table bit for each row that starts, applying one min-plus butterfly per
+column over the bits of the rows it touches, and closing each row that ends by keeping the half of the states whose
+bit equals the row's target.
+
+Extension over the reference: every column may carry an integer weight (>= 1).  A unit column emits
+("update", mask) exactly as the reference does; a column of weight w != 1 emits ("update", mask, w).  Plan.work
+and Plan.peak_bits do not depend on the weights.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+OP_EXPAND, OP_UPDATE, OP_SELECT = 0, 1, 2
+
+
+def lowbit(x):
+    """Index of the lowest set bit (cdist/trellis.py:16-17); -1 for 0."""
+    return (x & -x).bit_length() - 1
+
+
+def rank_profile(cols, order):
+    """r[i] = GF(2) rank of the first i columns of the order (cdist/trellis.py:20-36)."""
+    pivots = {}
+    out = [0]
+    for j in order:
+        v = cols[j]
+        while v:
+            top = v.bit_length() - 1
+            p = pivots.get(top)
+            if p is None:
+                pivots[top] = v
+                break
+            v ^= p
+        out.append(len(pivots))
+    return out
+
+
+def cut_profile(cols, order):
+    """Cut ranks lambda_i = rank(past) + rank(future) - rank(all), i = 0..N (cdist/trellis.py:39-44)."""
+    fwd = rank_profile(cols, order)
+    bwd = rank_profile(cols, list(reversed(order)))[::-1]
+    return [a + b - fwd[-1] for a, b in zip(fwd, bwd)]
+
+
+def rows_from_columns(cols, order, nrows):
+    """Row r as a bitmask over column positions of the order (cdist/trellis.py:47-57)."""
+    rows = [0] * nrows
+    for pos, j in enumerate(order):
+        v = cols[j]
+        while v:
+            low = v & -v
+            rows[low.bit_length() - 1] |= 1 << pos
+            v ^= low
+    return rows
+
+
+def minimal_span_form(rows):
+    """Row reduction to distinct starts and distinct ends; dependent rows vanish (cdist/trellis.py:60-97).
+
+    Pass 1 inserts rows keyed by their start; on a clash the row that ends earlier keeps the slot and the sum moves
+    on.  Pass 2 walks rows by decreasing start and keys them by their end; on a clash the row starting earlier
+    absorbs the other (its start stays, its end moves left)."""
+    starts = {}
+    for r in rows:
+        while r:
+            s = lowbit(r)
+            held = starts.get(s)
+            if held is None:
+                starts[s] = r
+                break
+            if r.bit_length() < held.bit_length():
+                starts[s], r = r, held
+            r ^= starts[s]
+    ends = {}
+    for r in sorted(starts.values(), key=lowbit, reverse=True):
+        while True:
+            e = r.bit_length()
+            held = ends.get(e)
+            if held is None:
+                ends[e] = r
+                break
+            if lowbit(held) > lowbit(r):
+                r ^= held
+            else:
+                ends[e], r = r, held ^ r
+    out = list(ends.values())
+    assert len({lowbit(r) for r in out}) == len(out) and len({r.bit_length() for r in out}) == len(out)
+    return out
+
+
+def reduce_target_row(lrow, hrows):
+    """Shorten the target-1 row by constraint rows: clear clashing starts, then clashing ends of rows that start no
+    earlier, re-clearing starts after each step (cdist/trellis.py:100-122).  0 means the query is infeasible."""
+    by_start = {lowbit(r): r for r in hrows}
+    by_end = {r.bit_length(): r for r in hrows}
+
+    def clear_start(x):
+        while x:
+            q = by_start.get(lowbit(x))
+            if q is None:
+                break
+            x ^= q
+        return x
+
+    lrow = clear_start(lrow)
+    while lrow:
+        q = by_end.get(lrow.bit_length())
+        if q is None or lowbit(q) < lowbit(lrow):
+            break
+        lrow = clear_start(lrow ^ q)
+    return lrow
+
+
+@dataclass
+class Plan:
+    """Op stream of the dense DP: ("expand",) | ("update", mask[, weight]) | ("select", pos, target)
+    (cdist/trellis.py:125-144)."""
+    ops: list
+    peak_bits: int
+    widths: list
+    ncols: int
+    feasible: bool = True
+    columns: list = field(default_factory=list)     # column position (in the order) of each update op
+
+    def work(self):
+        """Table entries touched by updates: sum over updates of the table size at that update."""
+        tot, bits = 0, 0
+        for op in self.ops:
+            if op[0] == "expand":
+                bits += 1
+            elif op[0] == "select":
+                bits -= 1
+            else:
+                tot += 1 << bits
+        return tot
+
+    def encode(self):
+        """The op stream as an (nops, 3) uint64 array of (code, a, b): expand (0, 0, 0), update (1, mask, weight),
+        select (2, pos, target) -- the format of the C-ABI plan executors."""
+        out = np.zeros((len(self.ops), 3), dtype=np.uint64)
+        for i, op in enumerate(self.ops):
+            if op[0] == "expand":
+                out[i, 0] = OP_EXPAND
+            elif op[0] == "update":
+                out[i] = (OP_UPDATE, op[1], op[2] if len(op) > 2 else 1)
+            else:
+                out[i] = (OP_SELECT, op[1], op[2])
+        return out
+
+    def max_weight(self):
+        return max((op[2] if len(op) > 2 else 1 for op in self.ops if op[0] == "update"), default=1)
+
+
+def make_plan(hrows, lrow, ncols, weights=None):
+    """DP schedule for constraint rows (target 0) and one target-1 row (cdist/trellis.py:147-182); weights[j] is the
+    cost of the column at position j (default 1)."""
+    if lrow == 0:
+        return Plan([], 0, [], ncols, feasible=False)
+    rows = list(hrows) + [lrow]
+    target = [0] * len(hrows) + [1]
+    opening, closing = {}, {}
+    for idx, r in enumerate(rows):
+        opening.setdefault(lowbit(r), []).append(idx)
+        closing.setdefault(r.bit_length() - 1, []).append(idx)
+    active, ops, widths, columns, peak = [], [], [], [], 0
+    for j in range(ncols):
+        for idx in opening.get(j, ()):
+            active.append(idx)
+            ops.append(("expand",))
+        peak = max(peak, len(active))
+        mask = 0
+        for pos, idx in enumerate(active):
+            if (rows[idx] >> j) & 1:
+                mask |= 1 << pos
+        if mask:                       # a column touching no active row is a free zero-syndrome fault
+            w = 1 if weights is None else int(weights[j])
+            ops.append(("update", mask) if w == 1 else ("update", mask, w))
+            columns.append(j)
+        for pos in sorted((active.index(idx) for idx in closing.get(j, ())), reverse=True):
+            ops.append(("select", pos, target[active[pos]]))
+            active.pop(pos)
+        widths.append(len(active))
+    assert not active
+    return Plan(ops, peak, widths, ncols, columns=columns)
+
+
+def target_row(l, obs_mask, order):
+    """Bitmask over positions of the columns whose logical label has odd parity with obs_mask."""
+    lrow = 0
+    for pos, j in enumerate(order):
+        if bin(l[j] & obs_mask).count("1") & 1:
+            lrow |= 1 << pos
+    return lrow
+
+
+def build_plan(h, l, obs_mask, order, nrows, weights=None):
+    """Plan for H e = 0, (obs_mask . L) e = 1 in the given order (cdist/trellis.py:185-193); weights are per
+    column of the problem (not per position)."""
+    hrows = minimal_span_form(rows_from_columns(h, order, nrows))
+    lrow = reduce_target_row(target_row(l, obs_mask, order), hrows)
+    pw = None if weights is None else [weights[j] for j in order]
+    return make_plan(hrows, lrow, len(order), pw)
