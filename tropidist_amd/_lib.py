@@ -39,10 +39,13 @@ class TdpError(RuntimeError):
 
 
 def lib():
+    """Load libtdp.so.  torch is imported first: its HIP runtime (SONAME libamdhip64.so.7) is then the one the
+    library binds to, so the process holds a single HIP runtime and device memory is shared with torch tensors."""
     global _LIB
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("libtdp.so is not built: run `python -m tropidist_amd.build` (no CPU fallback)")
+        import torch  # noqa: F401  (loads the HIP runtime libtdp.so links against)
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

@@ -17,6 +17,8 @@ ARCH = os.environ.get("TDP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=" + ARCH,
+    # code objects v5: loadable by the HIP runtime torch ships as well as by the system one
+    "-mcode-object-version=5",
     # packed FP32 instructions stay off in every kernel of this library
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
