@@ -72,6 +72,45 @@ int tdp_hist(const uint8_t *d_T, uint64_t n, uint64_t *d_hist, void *stream);
 int tdp_run_plan(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value,
                  void *stream);
 
+/* Witness of a plan: the plan's value and the op indices of the updates on one minimum path, by a checkpointed
+ * traceback from the final state 0 (replaces the numpy replay of dp_witness, cdist/auto.py:126-182, which keeps
+ * every table).  d_work: caller workspace of tdp_witness_workspace_bytes(nops, peak_bits) bytes; used_ops: host
+ * array of nops entries.  *value = 255 leaves *nused = 0 (the caller raises OverflowError). */
+int tdp_witness_workspace_bytes(int64_t nops, int peak_bits, uint64_t *bytes);
+int tdp_run_plan_witness(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_work, uint64_t work_bytes,
+                         int *value, int64_t *used_ops, int64_t *nused, void *stream);
+
+/* ------------------------------------------------------------------ tree-realization DP joins
+ * Tables of unsigned integers of itemsize 1, 2, 4 or 8 bytes in device memory; lm / rm: host arrays of the
+ * `generators` uint64 generator maps.  For the Gray-code counters i in [start, stop): g = i ^ (i >> 1) selects
+ * generators; out[g & (2^parent_bits - 1)] = min(out, L[XOR lm] + R[XOR rm]) when both costs are below `infinity`
+ * and the sum does not exceed it.  [0, 2^generators) runs as one atomic-free launch.  Replaces
+ * tree_join_u8/u16/u32/u64 (cdist/tree_dp.c:13-44). */
+int tdp_tree_join(int itemsize, const void *d_left, const void *d_right, void *d_out, const uint64_t *lm,
+                  const uint64_t *rm, unsigned parent_bits, unsigned generators, uint64_t start, uint64_t stop,
+                  uint64_t infinity, void *stream);
+int tdp_tree_join_u8(const void *d_left, const void *d_right, void *d_out, const uint64_t *lm, const uint64_t *rm,
+                     unsigned parent_bits, unsigned generators, uint64_t start, uint64_t stop, uint64_t infinity,
+                     void *stream);
+int tdp_tree_join_u16(const void *d_left, const void *d_right, void *d_out, const uint64_t *lm, const uint64_t *rm,
+                      unsigned parent_bits, unsigned generators, uint64_t start, uint64_t stop, uint64_t infinity,
+                      void *stream);
+int tdp_tree_join_u32(const void *d_left, const void *d_right, void *d_out, const uint64_t *lm, const uint64_t *rm,
+                      unsigned parent_bits, unsigned generators, uint64_t start, uint64_t stop, uint64_t infinity,
+                      void *stream);
+int tdp_tree_join_u64(const void *d_left, const void *d_right, void *d_out, const uint64_t *lm, const uint64_t *rm,
+                      unsigned parent_bits, unsigned generators, uint64_t start, uint64_t stop, uint64_t infinity,
+                      void *stream);
+
+/* Postorder arena compaction: move `bytes` toward a lower address, source and destination may overlap.
+ * Replaces tree_move (cdist/tree_dp.c:46-51). */
+int tdp_tree_move(void *d_dst, const void *d_src, size_t bytes, void *stream);
+
+/* Leaf and fill initialisation of arena tables (the numpy fills of cdist/tree_dp.py:446-454): out[0] = v0,
+ * out[1] = v1 (when size > 1), the rest = fill; tdp_tree_fill sets all `size` items to `value`. */
+int tdp_tree_leaf(int itemsize, void *d_out, uint64_t size, uint64_t v0, uint64_t v1, uint64_t fill, void *stream);
+int tdp_tree_fill(int itemsize, void *d_out, uint64_t size, uint64_t value, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -229,3 +229,61 @@ def brute_affine(columns, w=None):
         if t not in result or c < result[t]:
             result[t] = c
     return result
+
+
+# ---------------------------------------------------------------------------------------------- tree DP
+def tree_join(item, left, right, out, lm, rm, parent_bits, generators, start, stop, infinity, impl="oracle"):
+    """One Gray-counter range of a join on numpy arrays (cdist/tree_dp.c:13-39): impl "oracle" (liboracle) or
+    "ref" (the reference's libtree_dp.so)."""
+    if impl == "oracle":
+        f = getattr(olib(), "or_tree_join_u%d" % (8 * item))
+    else:
+        f = getattr(rlib("tree_dp"), "tree_join_u%d" % (8 * item))
+    f(left.ctypes.data, right.ctypes.data, out.ctypes.data, lm.ctypes.data, rm.ctypes.data, parent_bits,
+      generators, start, stop, infinity)
+
+
+def tree_run(program, target=0, impl="oracle", chunk=1 << 20):
+    """CompiledTreeDP.run restated on the CPU (cdist/tree_dp.py:401-505): the same postorder arena schedule with
+    numpy tables and the C joins of `impl`.  Uses only the compiled host data of the product's CompiledTreeDP."""
+    e0 = program.solver.solve(target)
+    if e0 is None:
+        return None
+    if not program.nodes:
+        return 0
+    item, inf = program.itemsize, program.infinity
+    arena = np.empty(program.peak_live_entries, dtype=program.dtype)
+    maxj = max((j.join_dimension for j in program.joins.values()), default=0)
+    maps = np.zeros((2, max(1, maxj)), dtype=np.uint64)
+    stack, live = [], 0
+    for index in program.order:
+        node = program.nodes[index]
+        size = 1 << len(program.interfaces[index])
+        out = arena[live:live + size]
+        out.fill(inf)
+        if node.leaf is not None:
+            bit = (e0 >> node.leaf) & 1
+            out[0] = bit if program.columns[node.leaf] else 0
+            if size == 2:
+                out[1] = bit ^ 1
+            stack.append((index, live, size))
+            live += size
+            continue
+        second, first = stack.pop(), stack.pop()
+        left, right = (first, second) if first[0] == node.left else (second, first)
+        join = program.joins[index]
+        gens = join.parent_maps + join.free_maps
+        for b, g in enumerate(gens):
+            maps[0, b] = g & ((1 << join.left_rank) - 1)
+            maps[1, b] = g >> join.left_rank
+        lt = arena[left[1]:left[1] + left[2]]
+        rt = arena[right[1]:right[1] + right[2]]
+        for begin in range(0, join.work, chunk):
+            tree_join(item, lt, rt, out, maps[0], maps[1], join.parent_rank, join.join_dimension, begin,
+                      min(begin + chunk, join.work), inf, impl)
+        dest = first[1]
+        arena[dest:dest + size] = out.copy()
+        live = dest + size
+        stack.append((index, dest, size))
+    v = int(arena[0])
+    return None if v == inf else v

@@ -1,0 +1,91 @@
+"""GPU parity of the tree-realization DP: joins, leaves and arena compaction on the device against brute force,
+the trellis DP and the oracle's CPU run of the same compiled program."""
+import random
+from itertools import product
+from unittest.mock import patch
+
+import pytest
+
+import oracle
+from test_oracle_tree import balanced, comb, random_tree
+from tropidist_amd import kernel, problem, trellis, tree_dp
+from tropidist_amd.tree_dp import CompiledTreeDP, tree_min_distance_for_mask
+
+pytestmark = pytest.mark.gpu
+
+
+def test_all_small_matrices_all_targets():
+    print("case tree 2728 queries", flush=True)
+    solved = 0
+    for ncols in range(5):
+        tree = balanced(list(range(ncols)))
+        for columns in product(range(4), repeat=ncols):
+            ref = oracle.brute_affine(columns)
+            program = CompiledTreeDP(columns, tree)
+            for target in range(8):
+                got, stats = program.run(target)
+                assert got == ref.get(target), (columns, target)
+                if stats["feasible"] and ncols:
+                    assert stats["allocated_dense_bytes"] == stats["peak_dense_bytes"]
+                    assert stats["observed_peak_live_table_bytes"] == stats["arena_bytes"]
+                    assert stats["join_work_done"] == stats["join_work"]
+                solved += 1
+    assert solved == 2728
+
+
+def test_random_logical_queries_against_trellis():
+    print("case tree random vs trellis", flush=True)
+    rng = random.Random(20261021)
+    for trial in range(160):
+        ncols, nrows, nobs = rng.randrange(1, 13), rng.randrange(0, 7), rng.randrange(1, 4)
+        h = [rng.randrange(1 << nrows) for _ in range(ncols)]
+        l = [rng.randrange(1 << nobs) for _ in range(ncols)]
+        order = list(range(ncols))
+        rng.shuffle(order)
+        mask = rng.randrange(1 << nobs)
+        want = oracle.brute(h, l, mask)
+        assert kernel.run_plan(trellis.build_plan(h, l, mask, order, nrows))[0] == want
+        tree = (comb(order), balanced(order), random_tree(order, rng))[trial % 3]
+        assert tree_min_distance_for_mask(h, l, mask, tree)[0] == want, trial
+
+
+def test_large_distance_selects_uint16():
+    print("case tree uint16 chain", flush=True)
+    for ncols in (254, 255, 300):
+        h = [1] + [(1 << (j - 1)) | (1 << j) for j in range(1, ncols - 1)] + [1 << (ncols - 2)]
+        d, stats = tree_min_distance_for_mask(h, [1] + [0] * (ncols - 1), 1, balanced(list(range(ncols))))
+        assert d == ncols and stats["dtype"] == ("uint8" if ncols == 254 else "uint16")
+
+
+def test_chunked_joins_on_gpu():
+    print("case tree chunks of 3", flush=True)
+    columns = [1, 2, 4, 7, 3, 5, 6, 1]
+    with patch.object(tree_dp, "_CHUNK_PAIRS", 3):
+        program = CompiledTreeDP(columns, ((0, (1, (2, 3))), ((4, 5), (6, 7))))
+        for target, want in oracle.brute_affine(columns).items():
+            assert program.run(target)[0] == want
+
+
+def test_overlapping_compaction_and_deep_comb():
+    print("case tree compaction / comb", flush=True)
+    program = CompiledTreeDP([1 << j for j in range(8)] * 2, balanced(list(range(16))))
+    for target in (0, 1, 3, 17, 127, 128, 255):
+        d, stats = program.run(target)
+        assert d == bin(target).count("1") and stats["observed_peak_live_table_bytes"] == 544
+    assert CompiledTreeDP([0] * 1300, comb(list(range(1300)))).run()[0] == 0
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_banded_balanced_tree_equals_oracle(seed):
+    """Interface ranks up to ~14 with wide joins: the GPU run equals the oracle's CPU run and the trellis DP."""
+    print("case tree banded seed", seed, flush=True)
+    p = problem.banded(R=40, N=160, kmin=2, kmax=4, W=12, nobs=1, rho=0.2, seed=seed)
+    order = list(range(p.ncols))
+    plan = trellis.build_plan(p.h, p.l, 1, order, p.nrows)
+    nrows = p.nrows
+    aug = [h | ((l & 1) << nrows) for h, l in zip(p.h, p.l)]
+    program = CompiledTreeDP(aug, balanced(order))
+    prof = program.preflight(max_memory_bytes=None, max_join_work=None)
+    got, _ = program.run(1 << nrows, max_memory_bytes=None, max_join_work=None)
+    assert got == oracle.tree_run(program, 1 << nrows) == kernel.run_plan(plan)[0]
+    assert prof["max_join_dimension"] >= 8

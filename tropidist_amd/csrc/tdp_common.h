@@ -97,4 +97,11 @@ __device__ __forceinline__ uint8_t sat_add1(uint8_t a, uint32_t w) {
     return (uint8_t)(s > 255u ? 255u : s);
 }
 
+// Launchers of the per-op dense kernels (tdp_dense.hip), shared by the plan executors.
+int launch_update(uint8_t *T, uint64_t n, uint64_t h, uint32_t w, hipStream_t st);
+int launch_expand(uint8_t *T, uint64_t n, uint64_t hl, uint32_t w, hipStream_t st);
+int launch_fill(uint8_t *T, uint64_t n, uint8_t v, hipStream_t st);
+int launch_select(uint8_t *dst, const uint8_t *src, uint64_t n, int pos, int target, uint64_t *n_out,
+                  hipStream_t st);
+
 }  // namespace tdp

@@ -140,7 +140,7 @@ __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
 }
 
 // ------------------------------------------------------------------------------------------------- launchers
-static int launch_update(uint8_t *T, uint64_t n, uint64_t h, uint32_t w, hipStream_t st) {
+int launch_update(uint8_t *T, uint64_t n, uint64_t h, uint32_t w, hipStream_t st) {
     if (h == 0 || n < 2) return TDP_OK;
     TDP_REQUIRE(h < n, "update mask 0x%llx outside a table of %llu entries", (unsigned long long)h,
                 (unsigned long long)n);
@@ -160,7 +160,7 @@ static int launch_update(uint8_t *T, uint64_t n, uint64_t h, uint32_t w, hipStre
     return TDP_OK;
 }
 
-static int launch_expand(uint8_t *T, uint64_t n, uint64_t hl, uint32_t w, hipStream_t st) {
+int launch_expand(uint8_t *T, uint64_t n, uint64_t hl, uint32_t w, hipStream_t st) {
     TDP_REQUIRE(hl < n, "expand mask 0x%llx outside a table of %llu entries", (unsigned long long)hl,
                 (unsigned long long)n);
     if (w > 255) w = 255;
@@ -174,7 +174,7 @@ static int launch_expand(uint8_t *T, uint64_t n, uint64_t hl, uint32_t w, hipStr
     return TDP_OK;
 }
 
-static int launch_fill(uint8_t *T, uint64_t n, uint8_t v, hipStream_t st) {
+int launch_fill(uint8_t *T, uint64_t n, uint8_t v, hipStream_t st) {
     if (n == 0) return TDP_OK;
     if (n < 16 || (n & 15)) {
         TDP_REQUIRE(n < (1ull << 32), "fill of %llu entries must be a multiple of 16", (unsigned long long)n);
@@ -186,7 +186,7 @@ static int launch_fill(uint8_t *T, uint64_t n, uint8_t v, hipStream_t st) {
     return TDP_OK;
 }
 
-static int launch_select(uint8_t *dst, const uint8_t *src, uint64_t n, int pos, int target, uint64_t *n_out,
+int launch_select(uint8_t *dst, const uint8_t *src, uint64_t n, int pos, int target, uint64_t *n_out,
                          hipStream_t st) {
     TDP_REQUIRE(pos >= 0 && pos < 63 && (target == 0 || target == 1), "select pos %d target %d", pos, target);
     if ((n >> (pos + 1)) == 0) {   // the reference returns n unchanged (cdist/viterbi.c:157-158)
