@@ -111,6 +111,65 @@ int tdp_tree_move(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int tdp_tree_leaf(int itemsize, void *d_out, uint64_t size, uint64_t v0, uint64_t v1, uint64_t fill, void *stream);
 int tdp_tree_fill(int itemsize, void *d_out, uint64_t size, uint64_t value, void *stream);
 
+/* ------------------------------------------------------------------ sparse (weight-truncated) trellis DP
+ * A device table is n states of 16 bytes {lo, hv}: key = (hv & (2^56 - 1)) << 64 | lo (up to 120 row slots),
+ * value = hv >> 56 (the reference's packed key | value << 56 widened from 56 to 120 key bits,
+ * cdist/sparse_dp.c:3-4), with distinct keys in any order, and a hash index d_slots of uint32 entry numbers
+ * (capacity a power of two >= 2n).  128-bit masks are passed as host arrays {lo, hi}. */
+
+/* Build the index of a table.  No reference counterpart (the reference keeps tables sorted, cdist/sparse_dp.c:51-114). */
+int tdp_sp_index(const void *d_ents, uint64_t n, uint32_t *d_slots, uint64_t capacity, void *stream);
+
+/* One column step T'[s] = min(T[s], T[s ^ upd] + 1) with the closing-row filter, cap, keep_below and lookahead
+ * bound (d_cls_mask: ncls {lo, hi} masks, d_cls_w: ncls weights, device).  Writes the new table to d_out (capacity
+ * out_cap >= 2n) and, if d_ch is not NULL, the new or improved states to d_ch (capacity ch_cap >= 2n); device
+ * counters: [0] table size, [1] changed size, [2] overflow flag.  Replaces sp_candidates + sp_merge
+ * (cdist/sparse_dp.c:116-202). */
+int tdp_sp_step(const void *d_old, uint64_t n, const uint32_t *d_slots, uint64_t capacity, const uint64_t *upd,
+                const uint64_t *sel_mask, const uint64_t *sel_bits, int cap, int keep_below, int ncls,
+                const uint64_t *d_cls_mask, const int64_t *d_cls_w, int64_t extra, const uint64_t *tmask, int K,
+                int use_h, void *d_out, uint64_t out_cap, void *d_ch, uint64_t ch_cap,
+                unsigned long long *d_counters, void *stream);
+
+/* d_vals[i] = value of key {d_keys[2i], d_keys[2i+1]} (255 if absent).  The lookup of NumpyTables/CTables.lookup
+ * (cdist/sparse_dp.py:329-336, 557-565), batched. */
+int tdp_sp_lookup(const void *d_ents, const uint32_t *d_slots, uint64_t capacity, const uint64_t *d_keys, uint64_t m,
+                  uint8_t *d_vals, void *stream);
+
+/* States with value < below into d_out; *d_count = their number.  Replaces CTables.truncate
+ * (cdist/sparse_dp.py:533-535). */
+int tdp_sp_truncate(const void *d_ents, uint64_t n, int below, void *d_out, unsigned long long *d_count,
+                    void *stream);
+
+/* Least value(f) + value(b) over f in d_f and b in the indexed backward table with key(b) = key(f) ^ tmask:
+ * d_best = {total, key hi, key lo, forward value} of the least total with the least forward key, total = 2^64-1
+ * if none.  Replaces sp_join (cdist/sparse_dp.c:204-236). */
+int tdp_sp_join(const void *d_f, uint64_t nf, const void *d_bents, const uint32_t *d_bslots, uint64_t bcapacity,
+                const uint64_t *tmask, unsigned long long *d_best, void *stream);
+
+/* d_hist[v] = number of states of value v (256 device counters).  Replaces sp_hist (cdist/sparse_dp.c:238-242). */
+int tdp_sp_hist(const void *d_ents, uint64_t n, uint64_t *d_hist, void *stream);
+
+/* ------------------------------------------------------------------ meet-in-the-middle lookup tables
+ * Open addressing over 2^log2slots device slots; all subsets of size <= k of n columns keyed by the 128-bit
+ * XOR-linear syndrome hash (zh, zl) with their logical label lg.  The table owns its device memory behind the
+ * handle (as the reference's Table, cdist/gmitm.c:36-84). */
+
+/* NULL on failure (the caller raises MemoryError, cdist/search.py:127-129).  Replaces tab_create (gmitm.c:61-76). */
+void *tdp_tab_create(int log2slots, int wide);
+/* Replaces tab_free (gmitm.c:78-82). */
+void tdp_tab_free(void *table);
+/* Number of distinct keys (compact: fingerprint and 16-bit label; wide: fingerprint and label), -1 on error.
+ * Replaces tab_count (gmitm.c:84). */
+int64_t tdp_tab_count(void *table);
+/* Insert the empty set and every subset of size 1..k (k <= 8) of the n columns given by host arrays zh, zl, lg
+ * (gmitm.c:165-186), on the device. */
+int tdp_tab_build(void *table, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg, int k);
+/* For m device queries: d_out[q] = least cost <= d_R[q] of an entry with the query's fingerprint and a different
+ * label, -1 if none.  Batched form of tab_lookup (gmitm.c:136-163, static in the reference). */
+int tdp_tab_lookup_batch(void *table, const uint64_t *d_zh, const uint64_t *d_zl, const uint64_t *d_lg,
+                         const int *d_R, uint64_t m, int *d_out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

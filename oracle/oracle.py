@@ -287,3 +287,159 @@ def tree_run(program, target=0, impl="oracle", chunk=1 << 20):
         stack.append((index, dest, size))
     v = int(arena[0])
     return None if v == inf else v
+
+
+# ---------------------------------------------------------------------------------------------- sparse tables
+KEYMASK56 = (1 << 56) - 1
+
+
+class OracleTables:
+    """CPU sparse tables in the reference's C format (one sorted uint64 per state, key | value << 56, up to 56
+    slots; cdist/sparse_dp.py:506-605) over liboracle's or_sp_candidates / or_sp_merge / or_sp_join (impl
+    "oracle") or the reference's own libsparse_dp.so (impl "ref").  Implements the backend interface the product's
+    Sweep and query use, so the same host logic can run on the CPU as a checker."""
+    name = "oracle"
+    impl = "oracle"
+
+    def __init__(self, tr):
+        if tr.nslots > 56:
+            raise ValueError("the C tables hold at most 56 row slots (this trellis needs %d)" % tr.nslots)
+        self.tr = tr
+        self.nbits = max(1, tr.nslots)
+        if self.impl == "oracle":
+            L = olib()
+            self.cand, self.merge, self.joinf = L.or_sp_candidates, L.or_sp_merge, L.or_sp_join
+        else:
+            L = rlib("sparse_dp")
+            self.cand, self.merge, self.joinf = L.sp_candidates, L.sp_merge, L.sp_join
+        self._bounds = {}
+
+    @staticmethod
+    def start():
+        return np.zeros(1, dtype=np.uint64)
+
+    @staticmethod
+    def size(tab):
+        return tab.size
+
+    @staticmethod
+    def nbytes(tab):
+        return tab.nbytes
+
+    @staticmethod
+    def arrays(tab):
+        return tab & np.uint64(KEYMASK56), (tab >> np.uint64(56)).astype(np.uint8)
+
+    @staticmethod
+    def from_arrays(keys, vals):
+        x = np.asarray(keys, dtype=np.uint64) | (np.asarray(vals).astype(np.uint64) << np.uint64(56))
+        return x[np.argsort(x & np.uint64(KEYMASK56))]
+
+    @staticmethod
+    def lookup(tab, s):
+        keys = tab & np.uint64(KEYMASK56)
+        i = int(np.searchsorted(keys, np.uint64(s)))
+        if i < keys.size and int(keys[i]) == s:
+            return int(tab[i] >> np.uint64(56))
+        return INF
+
+    def lookup_many(self, tab, keys):
+        return np.array([self.lookup(tab, int(k)) for k in keys], dtype=np.uint8)
+
+    def bound(self, forward):
+        if forward not in self._bounds:
+            self._bounds[forward] = [(np.array(m, dtype=np.uint64), np.array(w, dtype=np.int64), len(m), e)
+                                     for m, w, e in self.tr.bound_data(forward)]
+        return self._bounds[forward]
+
+    def step(self, tab, upd, sel_mask, sel_bits, cap, K, tmask, hd, keep_below=None, want_changed=False):
+        n = tab.size
+        if hd is None:
+            ncls, mp, wp, extra, use_h = 0, None, None, 0, 0
+        else:
+            m_arr, w_arr, ncls, extra = hd
+            mp, wp, use_h = ptr(m_arr), ptr(w_arr), 1
+        cand = np.empty(max(1, n), dtype=np.uint64)
+        tmp = np.empty(max(1, n), dtype=np.uint64)
+        m = self.cand(ptr(tab), n, upd, sel_mask, sel_bits, cap, ncls, mp, wp, extra, tmask, K, use_h, self.nbits,
+                      ptr(cand), ptr(tmp), 1)
+        out = np.empty(max(1, n + m), dtype=np.uint64)
+        ch = np.empty(max(1, m), dtype=np.uint64) if want_changed else None
+        nch = ctypes.c_int64(0)
+        no = self.merge(ptr(tab), n, ptr(cand), m, sel_mask, sel_bits, cap, keep_below or 0, ncls, mp, wp, extra,
+                        tmask, K, use_h, ptr(out), ptr(ch), ctypes.byref(nch))
+        return out[:no].copy(), (ch[:nch.value].copy() if want_changed else None)
+
+    @staticmethod
+    def changed_size(ch):
+        return ch.size
+
+    @staticmethod
+    def persistent(tab):
+        return tab.copy()
+
+    @staticmethod
+    def truncate(tab, below):
+        return tab[(tab >> np.uint64(56)) < np.uint64(below)]
+
+    def join(self, ch, btab, tmask):
+        if not ch.size:
+            return None
+        fb, bb = ctypes.c_uint64(0), ctypes.c_uint64(0)
+        tot = self.joinf(ptr(ch), ch.size, ptr(btab), btab.size, tmask, ctypes.byref(fb), ctypes.byref(bb))
+        if tot < 0:
+            return None
+        return int(tot), fb.value & KEYMASK56, fb.value >> 56, bb.value & KEYMASK56
+
+    @staticmethod
+    def hist(tab):
+        return np.bincount((tab >> np.uint64(56)).astype(np.int64), minlength=256)
+
+
+class RefTables(OracleTables):
+    """OracleTables over the reference's compiled sparse_dp.c."""
+    name = "ref"
+    impl = "ref"
+
+
+# ---------------------------------------------------------------------------------------------- MITM tables
+def tab_build(zh, zl, lg, k, log2slots, wide, impl="oracle"):
+    """Build a table with liboracle (impl "oracle") or the reference's libgmitm.so (impl "ref"); returns
+    (handle, count, free function)."""
+    zh, zl, lg = (np.ascontiguousarray(a, dtype=np.uint64) for a in (zh, zl, lg))
+    if impl == "oracle":
+        L = olib()
+        h = L.or_tab_create(log2slots, wide)
+        L.or_tab_build(h, zh.size, ptr(zh), ptr(zl), ptr(lg), k)
+        return h, L.or_tab_count(h), L.or_tab_free
+    L = rlib("gmitm")
+    h = L.tab_create(log2slots, wide)
+    L.tab_build(h, zh.size, ptr(zh), ptr(zl), ptr(lg), k)
+    return h, L.tab_count(h), L.tab_free
+
+
+def tab_lookup(handle, zh, zl, lg, R):
+    return olib().or_tab_lookup(handle, int(zh), int(zl), int(lg), int(R))
+
+
+def brute_tab_lookup(zh, zl, lg, k, qzh, qzl, qlg, R, wide):
+    """Least size <= min(k, R) of a column subset whose syndrome key matches the query's (compact: bits 16-62 of
+    zl; wide: all of zl) with a different label (compact: low 16 bits), among subsets that share the query's
+    probe chain start is not required -- the exhaustive set of matching entries; -1 if none."""
+    fp = 0x7FFFFFFFFFFF0000
+    best = -1
+    n = len(zh)
+    for size in range(0, min(k, R) + 1):
+        for sup in itertools.combinations(range(n), size):
+            a = b = g = 0
+            for j in sup:
+                a ^= int(zh[j])
+                b ^= int(zl[j])
+                g ^= int(lg[j])
+            if wide:
+                ok = b == int(qzl) and g != int(qlg)
+            else:
+                ok = (b & fp) == (int(qzl) & fp) and (g & 0xFFFF) != (int(qlg) & 0xFFFF)
+            if ok and a == int(qzh):
+                return size
+    return best

@@ -37,6 +37,18 @@ SIGNATURES = {
     "tdp_tree_move": (ci, [vp, vp, ctypes.c_size_t, vp]),
     "tdp_tree_leaf": (ci, [ci, vp, u64, u64, u64, u64, vp]),
     "tdp_tree_fill": (ci, [ci, vp, u64, u64, vp]),
+    "tdp_sp_index": (ci, [vp, u64, vp, u64, vp]),
+    "tdp_sp_step": (ci, [vp, u64, vp, u64, vp, vp, vp, ci, ci, ci, vp, vp, i64, vp, ci, ci, vp, u64, vp, u64, vp,
+                         vp]),
+    "tdp_sp_lookup": (ci, [vp, vp, u64, vp, u64, vp, vp]),
+    "tdp_sp_truncate": (ci, [vp, u64, ci, vp, vp, vp]),
+    "tdp_sp_join": (ci, [vp, u64, vp, vp, u64, vp, vp, vp]),
+    "tdp_sp_hist": (ci, [vp, u64, vp, vp]),
+    "tdp_tab_create": (vp, [ci, ci]),
+    "tdp_tab_free": (None, [vp]),
+    "tdp_tab_count": (i64, [vp]),
+    "tdp_tab_build": (ci, [vp, ci, vp, vp, vp, ci]),
+    "tdp_tab_lookup_batch": (ci, [vp, vp, vp, vp, vp, u64, vp, vp]),
 }
 
 

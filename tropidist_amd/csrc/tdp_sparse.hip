@@ -1,0 +1,352 @@
+// Weight-truncated sparse trellis DP on gfx950 (the step / join / lookup semantics of cdist/sparse_dp.c:116-242 and
+// cdist/sparse_dp.py:338-445, on device-resident tables).
+//
+// Table: n entries of 16 bytes, {lo, hv}: key = (hv & 2^56-1) << 64 | lo (up to 120 row slots), value = hv >> 56,
+// plus an open-addressing index of uint32 entry numbers (power-of-two capacity >= 2n, linear probing, 0xFFFFFFFF =
+// empty).  Keys in a table are distinct; the entry order is arbitrary (results never depend on it).
+//
+// One column step (T'[s] = min(T[s], T[s ^ upd] + 1), then the closing-row filter, the cap and the lookahead) is one
+// pass over the old entries.  Each old entry (k, v) looks up its partner k ^ upd once:
+//   key k      (if its closing bits match): old value v against the candidate partner + 1 (partner below the cap);
+//   key k^upd  (only when the partner is absent, v below the cap, closing bits match): a new state of value v + 1.
+// Every output key arises exactly once (XOR by upd is a bijection and the filter fixes the cleared bits), so the
+// outputs need no deduplication: they are appended with one atomic per wave and indexed afterwards.
+#include "tdp_common.h"
+
+namespace tdp {
+
+constexpr int kSpBlock = 256;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint64_t kKeyMask = (1ull << 56) - 1;
+
+struct Ent {
+    uint64_t lo, hv;
+};
+
+struct Key128 {
+    uint64_t lo, hi;
+};
+
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+__device__ __forceinline__ uint64_t hash_key(uint64_t lo, uint64_t hi) { return mix64(lo ^ mix64(hi + 0x9e3779b97f4a7c15ull)); }
+
+// value of key (lo, hi) in the table, 255 if absent
+__device__ __forceinline__ uint32_t sp_find(const Ent *__restrict__ ents, const uint32_t *__restrict__ slots,
+                                            uint64_t capmask, uint64_t lo, uint64_t hi) {
+    uint64_t h = hash_key(lo, hi) & capmask;
+    for (;;) {
+        uint32_t e = slots[h];
+        if (e == kEmpty) return 255u;
+        Ent x = ents[e];
+        if (x.lo == lo && (x.hv & kKeyMask) == hi) return (uint32_t)(x.hv >> 56);
+        h = (h + 1) & capmask;
+    }
+}
+
+// Lookahead bound (cdist/sparse_dp.c:25-44): ceil((extra + sum_c w_c |(key ^ tmask) & mask_c|) / Q).
+struct BoundArgs {
+    const uint64_t *mask;     // 2 words per class (lo, hi)
+    const int64_t *w;
+    int ncls;
+    int on;
+    int64_t extra;
+    uint64_t tm_lo, tm_hi;
+    int K;
+};
+
+__device__ __forceinline__ bool keep_state(const BoundArgs &b, uint64_t lo, uint64_t hi, int v) {
+    if (!b.on) return true;
+    uint64_t ulo = lo ^ b.tm_lo, uhi = hi ^ b.tm_hi;
+    int64_t tot = b.extra;
+    for (int c = 0; c < b.ncls; c++)
+        tot += b.w[c] * (int64_t)(__popcll(ulo & b.mask[2 * c]) + __popcll(uhi & b.mask[2 * c + 1]));
+    const int64_t Q = 720720;
+    return v + (int)((tot + Q - 1) / Q) <= b.K;
+}
+
+// Wave-aggregated append: returns the slot of this lane's item in `out` (only meaningful where pred), one atomic
+// per wave.  Must be reached by every lane of the wave.
+__device__ __forceinline__ uint64_t wave_append(unsigned long long *counter, bool pred) {
+    uint64_t ballot = __ballot(pred);
+    uint32_t lane = __lane_id();
+    uint64_t below = ballot & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    uint32_t rank = (uint32_t)__popcll(below);
+    uint32_t total = (uint32_t)__popcll(ballot);
+    int leader = ballot ? __ffsll((long long)ballot) - 1 : 0;
+    unsigned long long base = 0;
+    if (pred && (int)lane == leader) base = atomicAdd(counter, (unsigned long long)total);
+    base = __shfl(base, leader, 64);
+    return base + rank;
+}
+
+struct StepArgs {
+    uint64_t upd_lo, upd_hi;
+    uint64_t sm_lo, sm_hi, sb_lo, sb_hi;      // closing rows: mask and required bits
+    int cap, kb;                              // candidates from values < cap; table keeps values < kb
+    int want_changed;
+    uint64_t out_cap, ch_cap;
+};
+
+// counters: [0] out count, [1] changed count, [2] overflow flag
+__global__ __launch_bounds__(kSpBlock) void k_sp_step(const Ent *__restrict__ old, uint64_t n,
+                                                      const uint32_t *__restrict__ slots, uint64_t capmask,
+                                                      StepArgs a, BoundArgs b, Ent *__restrict__ out,
+                                                      Ent *__restrict__ ch, unsigned long long *counters) {
+    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    bool live = i < n;
+    bool o1 = false, o2 = false, c1 = false, c2 = false;
+    Ent e1 = {0, 0}, e2 = {0, 0}, f1 = {0, 0}, f2 = {0, 0};
+    if (live) {
+        Ent x = old[i];
+        uint64_t klo = x.lo, khi = x.hv & kKeyMask;
+        int v = (int)(x.hv >> 56);
+        uint64_t qlo = klo ^ a.upd_lo, qhi = khi ^ a.upd_hi;
+        bool has_upd = (a.upd_lo | a.upd_hi) != 0;
+        uint32_t vq = has_upd ? sp_find(old, slots, capmask, qlo, qhi) : 255u;
+        uint64_t clo = ~a.sm_lo, chi = ~a.sm_hi & kKeyMask;
+        // key k
+        if ((klo & a.sm_lo) == a.sb_lo && (khi & a.sm_hi) == a.sb_hi) {
+            uint64_t alo = klo & clo, ahi = khi & chi;
+            bool cand = has_upd && vq != 255u && (int)vq < a.cap && keep_state(b, alo, ahi, (int)vq + 1);
+            int vc = (int)vq + 1;
+            if (cand && vc < v) {
+                Ent c = {alo, ahi | ((uint64_t)vc << 56)};
+                if (a.want_changed) { c1 = true; f1 = c; }
+                if (vc < a.kb) { o1 = true; e1 = c; }
+            } else if (v < a.kb && keep_state(b, alo, ahi, v)) {
+                o1 = true;
+                e1 = Ent{alo, ahi | ((uint64_t)v << 56)};
+            }
+        }
+        // key k ^ upd, a new state when the partner is absent
+        if (has_upd && vq == 255u && v < a.cap && (qlo & a.sm_lo) == a.sb_lo && (qhi & a.sm_hi) == a.sb_hi) {
+            uint64_t alo = qlo & clo, ahi = qhi & chi;
+            if (keep_state(b, alo, ahi, v + 1)) {
+                Ent c = {alo, ahi | ((uint64_t)(v + 1) << 56)};
+                if (a.want_changed) { c2 = true; f2 = c; }
+                if (v + 1 < a.kb) { o2 = true; e2 = c; }
+            }
+        }
+    }
+    uint64_t p;
+    p = wave_append(&counters[0], o1);
+    if (o1) { if (p < a.out_cap) out[p] = e1; else counters[2] = 1; }
+    p = wave_append(&counters[0], o2);
+    if (o2) { if (p < a.out_cap) out[p] = e2; else counters[2] = 1; }
+    p = wave_append(&counters[1], c1);
+    if (c1) { if (p < a.ch_cap) ch[p] = f1; else counters[2] = 1; }
+    p = wave_append(&counters[1], c2);
+    if (c2) { if (p < a.ch_cap) ch[p] = f2; else counters[2] = 1; }
+}
+
+__global__ __launch_bounds__(kSpBlock) void k_fill_slots(uint32_t *slots, uint64_t cap) {
+    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    if (i < cap) slots[i] = kEmpty;
+}
+
+// keys are distinct: an entry takes the first empty slot of its probe sequence
+__global__ __launch_bounds__(kSpBlock) void k_index(const Ent *__restrict__ ents, uint64_t n, uint32_t *slots,
+                                                    uint64_t capmask) {
+    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    if (i >= n) return;
+    Ent x = ents[i];
+    uint64_t h = hash_key(x.lo, x.hv & kKeyMask) & capmask;
+    for (;;) {
+        if (atomicCAS(&slots[h], kEmpty, (uint32_t)i) == kEmpty) return;
+        h = (h + 1) & capmask;
+    }
+}
+
+__global__ __launch_bounds__(kSpBlock) void k_lookup(const Ent *__restrict__ ents, const uint32_t *__restrict__ slots,
+                                                     uint64_t capmask, const uint64_t *__restrict__ q, uint64_t m,
+                                                     uint8_t *__restrict__ vals) {
+    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    if (i < m) vals[i] = (uint8_t)sp_find(ents, slots, capmask, q[2 * i], q[2 * i + 1]);
+}
+
+__global__ __launch_bounds__(kSpBlock) void k_truncate(const Ent *__restrict__ ents, uint64_t n, int below,
+                                                       Ent *__restrict__ out, unsigned long long *count) {
+    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    bool keep = false;
+    Ent x = {0, 0};
+    if (i < n) {
+        x = ents[i];
+        keep = (int)(x.hv >> 56) < below;
+    }
+    uint64_t p = wave_append(count, keep);
+    if (keep) out[p] = x;
+}
+
+// join: best[0] = least total; then among those the least key (hi, then lo), and its forward value
+__global__ __launch_bounds__(kSpBlock) void k_join_total(const Ent *__restrict__ f, uint64_t nf,
+                                                         const Ent *__restrict__ bents,
+                                                         const uint32_t *__restrict__ bslots, uint64_t capmask,
+                                                         uint64_t tm_lo, uint64_t tm_hi, unsigned long long *best) {
+    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    if (i >= nf) return;
+    Ent x = f[i];
+    uint32_t vb = sp_find(bents, bslots, capmask, x.lo ^ tm_lo, (x.hv & kKeyMask) ^ tm_hi);
+    if (vb != 255u) atomicMin(&best[0], (unsigned long long)((x.hv >> 56) + vb));
+}
+
+__global__ __launch_bounds__(kSpBlock) void k_join_pick(const Ent *__restrict__ f, uint64_t nf,
+                                                        const Ent *__restrict__ bents,
+                                                        const uint32_t *__restrict__ bslots, uint64_t capmask,
+                                                        uint64_t tm_lo, uint64_t tm_hi, int stage,
+                                                        unsigned long long *best) {
+    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    if (i >= nf) return;
+    Ent x = f[i];
+    uint64_t hi = x.hv & kKeyMask;
+    if (stage >= 2 && hi != best[1]) return;
+    if (stage >= 3 && x.lo != best[2]) return;
+    uint32_t vb = sp_find(bents, bslots, capmask, x.lo ^ tm_lo, hi ^ tm_hi);
+    if (vb == 255u || (x.hv >> 56) + vb != best[0]) return;
+    if (stage == 1) atomicMin(&best[1], (unsigned long long)hi);
+    else if (stage == 2) atomicMin(&best[2], (unsigned long long)x.lo);
+    else best[3] = x.hv >> 56;
+}
+
+__global__ __launch_bounds__(kSpBlock) void k_sp_hist(const Ent *__restrict__ ents, uint64_t n,
+                                                      unsigned long long *hist) {
+    __shared__ uint32_t bins[256];
+    bins[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kSpBlock)
+        atomicAdd(&bins[ents[i].hv >> 56], 1u);
+    __syncthreads();
+    if (bins[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)bins[threadIdx.x]);
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v, int n) {
+    int i = threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static unsigned sgrid(uint64_t items) {
+    uint64_t g = (items + kSpBlock - 1) / kSpBlock;
+    return (unsigned)(g ? g : 1);
+}
+
+}  // namespace tdp
+
+using namespace tdp;
+
+extern "C" {
+
+int tdp_sp_index(const void *d_ents, uint64_t n, uint32_t *d_slots, uint64_t capacity, void *stream) {
+    TDP_REQUIRE(capacity && !(capacity & (capacity - 1)) && capacity >= 2 * n && n < 0xFFFFFFFFull,
+                "index capacity %llu for %llu entries", (unsigned long long)capacity, (unsigned long long)n);
+    hipStream_t st = as_stream(stream);
+    k_fill_slots<<<sgrid(capacity), kSpBlock, 0, st>>>(d_slots, capacity);
+    TDP_LAUNCH_CHECK();
+    if (n) {
+        k_index<<<sgrid(n), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_ents), n, d_slots, capacity - 1);
+        TDP_LAUNCH_CHECK();
+    }
+    return TDP_OK;
+}
+
+int tdp_sp_step(const void *d_old, uint64_t n, const uint32_t *d_slots, uint64_t capacity, const uint64_t *upd,
+                const uint64_t *sel_mask, const uint64_t *sel_bits, int cap, int keep_below, int ncls,
+                const uint64_t *d_cls_mask, const int64_t *d_cls_w, int64_t extra, const uint64_t *tmask, int K,
+                int use_h, void *d_out, uint64_t out_cap, void *d_ch, uint64_t ch_cap,
+                unsigned long long *d_counters, void *stream) {
+    TDP_REQUIRE(capacity && !(capacity & (capacity - 1)), "index capacity %llu", (unsigned long long)capacity);
+    TDP_REQUIRE(cap >= 0 && cap <= 254, "cap %d outside 0..254", cap);
+    hipStream_t st = as_stream(stream);
+    StepArgs a;
+    a.upd_lo = upd[0];
+    a.upd_hi = upd[1] & kKeyMask;
+    a.sm_lo = sel_mask[0];
+    a.sm_hi = sel_mask[1] & kKeyMask;
+    a.sb_lo = sel_bits[0];
+    a.sb_hi = sel_bits[1] & kKeyMask;
+    a.cap = cap;
+    int kb = keep_below > 0 ? keep_below : (1 << 30);
+    a.kb = cap + 1 < kb ? cap + 1 : kb;
+    a.want_changed = d_ch != nullptr;
+    a.out_cap = out_cap;
+    a.ch_cap = ch_cap;
+    BoundArgs b;
+    b.mask = d_cls_mask;
+    b.w = d_cls_w;
+    b.ncls = use_h ? ncls : 0;
+    b.on = use_h;
+    b.extra = extra;
+    b.tm_lo = tmask[0];
+    b.tm_hi = tmask[1] & kKeyMask;
+    b.K = K;
+    k_set_u64<<<1, 64, 0, st>>>(d_counters, 0ull, 3);
+    TDP_LAUNCH_CHECK();
+    if (n) {
+        k_sp_step<<<sgrid(n), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_old), n, d_slots, capacity - 1, a, b,
+                                                 static_cast<Ent *>(d_out), static_cast<Ent *>(d_ch), d_counters);
+        TDP_LAUNCH_CHECK();
+    }
+    return TDP_OK;
+}
+
+int tdp_sp_lookup(const void *d_ents, const uint32_t *d_slots, uint64_t capacity, const uint64_t *d_keys, uint64_t m,
+                  uint8_t *d_vals, void *stream) {
+    if (!m) return TDP_OK;
+    hipStream_t st = as_stream(stream);
+    k_lookup<<<sgrid(m), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_ents), d_slots, capacity - 1, d_keys, m,
+                                            d_vals);
+    TDP_LAUNCH_CHECK();
+    return TDP_OK;
+}
+
+int tdp_sp_truncate(const void *d_ents, uint64_t n, int below, void *d_out, unsigned long long *d_count,
+                    void *stream) {
+    hipStream_t st = as_stream(stream);
+    k_set_u64<<<1, 64, 0, st>>>(d_count, 0ull, 1);
+    TDP_LAUNCH_CHECK();
+    if (n) {
+        k_truncate<<<sgrid(n), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_ents), n, below,
+                                                  static_cast<Ent *>(d_out), d_count);
+        TDP_LAUNCH_CHECK();
+    }
+    return TDP_OK;
+}
+
+int tdp_sp_join(const void *d_f, uint64_t nf, const void *d_bents, const uint32_t *d_bslots, uint64_t bcapacity,
+                const uint64_t *tmask, unsigned long long *d_best, void *stream) {
+    hipStream_t st = as_stream(stream);
+    k_set_u64<<<1, 64, 0, st>>>(d_best, ~0ull, 4);
+    TDP_LAUNCH_CHECK();
+    if (!nf) return TDP_OK;
+    const Ent *f = static_cast<const Ent *>(d_f);
+    const Ent *be = static_cast<const Ent *>(d_bents);
+    uint64_t tlo = tmask[0], thi = tmask[1] & kKeyMask;
+    k_join_total<<<sgrid(nf), kSpBlock, 0, st>>>(f, nf, be, d_bslots, bcapacity - 1, tlo, thi, d_best);
+    TDP_LAUNCH_CHECK();
+    for (int stage = 1; stage <= 3; stage++) {
+        k_join_pick<<<sgrid(nf), kSpBlock, 0, st>>>(f, nf, be, d_bslots, bcapacity - 1, tlo, thi, stage, d_best);
+        TDP_LAUNCH_CHECK();
+    }
+    return TDP_OK;
+}
+
+int tdp_sp_hist(const void *d_ents, uint64_t n, uint64_t *d_hist, void *stream) {
+    hipStream_t st = as_stream(stream);
+    unsigned long long *h = reinterpret_cast<unsigned long long *>(d_hist);
+    k_set_u64<<<1, 256, 0, st>>>(h, 0ull, 256);
+    TDP_LAUNCH_CHECK();
+    if (n) {
+        uint64_t g = sgrid(n);
+        k_sp_hist<<<(unsigned)(g > 2048 ? 2048 : g), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_ents), n, h);
+        TDP_LAUNCH_CHECK();
+    }
+    return TDP_OK;
+}
+
+}  // extern "C"

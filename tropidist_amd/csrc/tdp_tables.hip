@@ -1,0 +1,362 @@
+// Meet-in-the-middle lookup tables on gfx950: all column subsets of size <= k keyed by syndrome hash (the tables of
+// cdist/gmitm.c:36-186, built and probed on the device).
+//
+// Open addressing over 2^log2slots slots, linear probing from zh & mask; key 0 = empty.
+//   compact (<= 16 observables): key = bit63 | bits 16..62 of zl (fingerprint) | 16-bit logical label;
+//   wide:                        key = bit63 | zl, the full label in lgs[], cost 255 until the label is published.
+// Insert: CAS the key into an empty slot; a slot holding the same key keeps the least cost (byte minimum by CAS
+// on its 32-bit word).  A wide insert that meets a matching slot whose label is not yet published does not wait
+// (lanes of one wave cannot wait on each other): it is written to a retry list and inserted again by the next
+// launch, after the owner has published.
+// Build: one thread per leading index pair (i < j) walks the remaining indices depth first, so every subset of
+// size 2..k is inserted once; singletons and the empty set come from their own launch.
+#include "tdp_common.h"
+
+#include <new>
+#include <vector>
+
+namespace tdp {
+
+constexpr int kTabBlock = 256;
+constexpr uint64_t kFpMask = 0x7FFFFFFFFFFF0000ull;
+constexpr int kMaxK = 8;
+
+struct DevTable {
+    uint64_t mask;
+    uint64_t *key;
+    uint64_t *lgs;              // wide only
+    uint8_t *cost;
+    unsigned long long *count;
+    int k;
+    int wide;
+    uint64_t log2slots;
+};
+
+struct TabView {
+    uint64_t mask;
+    uint64_t *key;
+    uint64_t *lgs;
+    uint8_t *cost;
+    unsigned long long *count;
+};
+
+struct Retry {
+    uint64_t zh, zl, lg;
+    uint32_t cost;
+};
+
+__device__ __forceinline__ void cost_min(uint8_t *cost, uint64_t i, uint8_t v) {
+    uintptr_t a = reinterpret_cast<uintptr_t>(cost + i);
+    uint32_t *word = reinterpret_cast<uint32_t *>(a & ~(uintptr_t)3);
+    unsigned shift = (unsigned)(a & 3) * 8;
+    uint32_t old = __hip_atomic_load(word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        uint32_t cur = (old >> shift) & 0xFFu;
+        if (cur <= v) return;
+        uint32_t next = (old & ~(0xFFu << shift)) | ((uint32_t)v << shift);
+        uint32_t seen = atomicCAS(word, old, next);
+        if (seen == old) return;
+        old = seen;
+    }
+}
+
+__device__ __forceinline__ uint8_t cost_load(const uint8_t *cost, uint64_t i) {
+    uintptr_t a = reinterpret_cast<uintptr_t>(cost + i);
+    const uint32_t *word = reinterpret_cast<const uint32_t *>(a & ~(uintptr_t)3);
+    uint32_t w = __hip_atomic_load(const_cast<uint32_t *>(word), __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return (uint8_t)((w >> ((a & 3) * 8)) & 0xFFu);
+}
+
+// returns false when a wide insert must be retried by a later launch
+__device__ bool tab_insert(const TabView &t, bool wide, uint64_t zh, uint64_t zl, uint64_t lg, uint8_t c) {
+    uint64_t i = zh & t.mask;
+    if (!wide) {
+        uint64_t key = (zl & kFpMask) | (lg & 0xFFFF) | (1ull << 63);
+        for (;;) {
+            uint64_t cur = __hip_atomic_load(&t.key[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (cur == 0) {
+                uint64_t seen = atomicCAS(reinterpret_cast<unsigned long long *>(&t.key[i]), 0ull,
+                                          (unsigned long long)key);
+                if (seen == 0) {
+                    atomicAdd(t.count, 1ull);
+                    cur = key;
+                } else {
+                    cur = seen;
+                }
+            }
+            if (cur == key) {
+                cost_min(t.cost, i, c);
+                return true;
+            }
+            i = (i + 1) & t.mask;
+        }
+    }
+    uint64_t key = zl | (1ull << 63);
+    for (;;) {
+        uint64_t cur = __hip_atomic_load(&t.key[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (cur == 0) {
+            uint64_t seen = atomicCAS(reinterpret_cast<unsigned long long *>(&t.key[i]), 0ull,
+                                      (unsigned long long)key);
+            if (seen == 0) {
+                atomicAdd(t.count, 1ull);
+                __hip_atomic_store(&t.lgs[i], lg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __threadfence();
+                cost_min(t.cost, i, c);
+                return true;
+            }
+            cur = seen;
+        }
+        if (cur == key) {
+            if (cost_load(t.cost, i) == 255) return false;           // owner has not published: retry later
+            __threadfence();
+            if (__hip_atomic_load(&t.lgs[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == lg) {
+                cost_min(t.cost, i, c);
+                return true;
+            }
+        }
+        i = (i + 1) & t.mask;
+    }
+}
+
+__device__ __forceinline__ void insert_or_defer(const TabView &t, bool wide, uint64_t zh, uint64_t zl, uint64_t lg,
+                                                uint8_t c, Retry *retry, unsigned long long *nretry,
+                                                uint64_t retry_cap) {
+    if (!tab_insert(t, wide, zh, zl, lg, c)) {
+        unsigned long long r = atomicAdd(nretry, 1ull);
+        if (r < retry_cap) retry[r] = Retry{zh, zl, lg, c};
+    }
+}
+
+__global__ void k_tab_clear(TabView t, uint64_t n, int wide) {
+    uint64_t i = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
+    if (i >= n) return;
+    t.key[i] = 0;
+    t.cost[i] = 255;
+    if (wide) t.lgs[i] = 0;
+}
+
+// the empty set (cost 0) and the singletons
+__global__ void k_tab_small(TabView t, int wide, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg,
+                            Retry *retry, unsigned long long *nretry, uint64_t retry_cap) {
+    int j = (int)(blockIdx.x * kTabBlock + threadIdx.x) - 1;
+    if (j >= n) return;
+    if (j < 0) insert_or_defer(t, wide, 0, 0, 0, 0, retry, nretry, retry_cap);
+    else insert_or_defer(t, wide, zh[j], zl[j], lg[j], 1, retry, nretry, retry_cap);
+}
+
+// subsets of size 2..k with leading pair (i, j), i < j, remaining indices ascending (depth first)
+__global__ void k_tab_pairs(TabView t, int wide, int n, int k, const uint64_t *zh, const uint64_t *zl,
+                            const uint64_t *lg, Retry *retry, unsigned long long *nretry, uint64_t retry_cap) {
+    uint64_t idx = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
+    uint64_t npairs = (uint64_t)n * (uint64_t)(n - 1) / 2;
+    if (idx >= npairs) return;
+    // unrank idx -> (i, j) with i < j: row i holds n - 1 - i pairs
+    int i = 0;
+    uint64_t rem = idx;
+    {
+        // closed form start, then correct
+        double nn = (double)n;
+        double disc = (2 * nn - 1) * (2 * nn - 1) - 8.0 * (double)idx;
+        int guess = (int)((2 * nn - 1 - sqrt(disc > 0 ? disc : 0)) / 2);
+        if (guess < 0) guess = 0;
+        if (guess > n - 2) guess = n - 2;
+        i = guess;
+        uint64_t before = (uint64_t)i * (uint64_t)(2 * n - i - 1) / 2;
+        while (i > 0 && before > idx) {
+            i--;
+            before = (uint64_t)i * (uint64_t)(2 * n - i - 1) / 2;
+        }
+        while (i < n - 2 && (uint64_t)(i + 1) * (uint64_t)(2 * n - i - 2) / 2 <= idx) {
+            i++;
+            before = (uint64_t)i * (uint64_t)(2 * n - i - 1) / 2;
+        }
+        rem = idx - before;
+    }
+    int j = i + 1 + (int)rem;
+    uint64_t h[kMaxK + 1], l[kMaxK + 1], g[kMaxK + 1];
+    int at[kMaxK + 1];
+    h[2] = zh[i] ^ zh[j];
+    l[2] = zl[i] ^ zl[j];
+    g[2] = lg[i] ^ lg[j];
+    insert_or_defer(t, wide, h[2], l[2], g[2], 2, retry, nretry, retry_cap);
+    if (k <= 2) return;
+    int depth = 2;           // subset size of h[depth]
+    at[depth] = j;           // last index used
+    int next = j + 1;
+    for (;;) {
+        if (next < n) {
+            int d1 = depth + 1;
+            h[d1] = h[depth] ^ zh[next];
+            l[d1] = l[depth] ^ zl[next];
+            g[d1] = g[depth] ^ lg[next];
+            insert_or_defer(t, wide, h[d1], l[d1], g[d1], (uint8_t)d1, retry, nretry, retry_cap);
+            if (d1 < k) {
+                at[d1] = next;
+                depth = d1;
+                next = next + 1;
+            } else {
+                next++;
+            }
+        } else {
+            if (depth == 2) break;
+            next = at[depth] + 1;
+            depth--;
+        }
+    }
+}
+
+__global__ void k_tab_retry(TabView t, const Retry *in, uint64_t m, Retry *out, unsigned long long *nout,
+                            uint64_t cap) {
+    uint64_t i = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
+    if (i >= m) return;
+    Retry r = in[i];
+    insert_or_defer(t, true, r.zh, r.zl, r.lg, (uint8_t)r.cost, out, nout, cap);
+}
+
+// Least cost <= R of an entry with the query's fingerprint and a different label; -1 if none (gmitm.c:136-163).
+__global__ void k_tab_lookup(TabView t, int wide, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg,
+                             const int *R, uint64_t m, int *out) {
+    uint64_t q = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
+    if (q >= m) return;
+    uint64_t i = zh[q] & t.mask;
+    int best = -1, r = R[q];
+    uint64_t fp = zl[q] & kFpMask, wkey = zl[q] | (1ull << 63), label = lg[q];
+    for (;;) {
+        uint64_t cur = t.key[i];
+        if (cur == 0) break;
+        bool match = wide ? (cur == wkey && t.lgs[i] != label)
+                          : ((cur & kFpMask) == fp && (cur & 0xFFFF) != (label & 0xFFFF));
+        if (match) {
+            int c = t.cost[i];
+            if (c <= r && (best < 0 || c < best)) best = c;
+        }
+        i = (i + 1) & t.mask;
+    }
+    out[q] = best;
+}
+
+static TabView view(const DevTable *t) { return TabView{t->mask, t->key, t->lgs, t->cost, t->count}; }
+
+static unsigned tab_grid(uint64_t n) { return (unsigned)((n + kTabBlock - 1) / kTabBlock); }
+
+}  // namespace tdp
+
+using namespace tdp;
+
+extern "C" {
+
+void *tdp_tab_create(int log2slots, int wide) {
+    if (log2slots < 1 || log2slots > 40) {
+        set_error("log2slots %d out of range", log2slots);
+        return nullptr;
+    }
+    DevTable *t = new (std::nothrow) DevTable();
+    if (!t) return nullptr;
+    uint64_t n = 1ull << log2slots;
+    t->mask = n - 1;
+    t->wide = wide;
+    t->log2slots = (uint64_t)log2slots;
+    bool ok = hipMalloc(&t->key, n * 8) == hipSuccess && hipMalloc(&t->cost, n + 4) == hipSuccess &&
+              hipMalloc(&t->count, 8) == hipSuccess && (!wide || hipMalloc(&t->lgs, n * 8) == hipSuccess);
+    if (!ok) {
+        set_error("device allocation of a 2^%d-slot table failed", log2slots);
+        hipFree(t->key);
+        hipFree(t->cost);
+        hipFree(t->count);
+        hipFree(t->lgs);
+        delete t;
+        return nullptr;
+    }
+    k_tab_clear<<<tab_grid(n), kTabBlock>>>(view(t), n, wide);
+    hipMemset(t->count, 0, 8);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        set_error("table initialisation failed");
+        return nullptr;
+    }
+    return t;
+}
+
+void tdp_tab_free(void *p) {
+    DevTable *t = static_cast<DevTable *>(p);
+    if (!t) return;
+    hipFree(t->key);
+    hipFree(t->cost);
+    hipFree(t->count);
+    hipFree(t->lgs);
+    delete t;
+}
+
+int64_t tdp_tab_count(void *p) {
+    DevTable *t = static_cast<DevTable *>(p);
+    unsigned long long c = 0;
+    if (!t || hipMemcpy(&c, t->count, 8, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return (int64_t)c;
+}
+
+int tdp_tab_build(void *p, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg, int k) {
+    DevTable *t = static_cast<DevTable *>(p);
+    TDP_REQUIRE(t && n >= 0 && k >= 0 && k <= kMaxK, "tab_build: n %d k %d (k <= %d)", n, k, kMaxK);
+    t->k = k;
+    uint64_t *d = nullptr;
+    size_t cols = n > 0 ? (size_t)n : 1;
+    TDP_HIP_CHECK(hipMalloc(&d, 3 * cols * 8));
+    uint64_t *dzh = d, *dzl = d + cols, *dlg = d + 2 * cols;
+    if (n) {
+        TDP_HIP_CHECK(hipMemcpy(dzh, zh, n * 8, hipMemcpyHostToDevice));
+        TDP_HIP_CHECK(hipMemcpy(dzl, zl, n * 8, hipMemcpyHostToDevice));
+        TDP_HIP_CHECK(hipMemcpy(dlg, lg, n * 8, hipMemcpyHostToDevice));
+    }
+    const uint64_t retry_cap = 1u << 20;
+    Retry *rbuf = nullptr;
+    unsigned long long *rcnt = nullptr;
+    TDP_HIP_CHECK(hipMalloc(&rbuf, 2 * retry_cap * sizeof(Retry)));
+    TDP_HIP_CHECK(hipMalloc(&rcnt, 2 * sizeof(unsigned long long)));
+    TDP_HIP_CHECK(hipMemset(rcnt, 0, 2 * sizeof(unsigned long long)));
+    TabView v = view(t);
+    int rc = TDP_OK;
+    int small = k >= 1 ? n + 1 : 1;
+    k_tab_small<<<tab_grid(small), kTabBlock>>>(v, t->wide, k >= 1 ? n : 0, dzh, dzl, dlg, rbuf, rcnt, retry_cap);
+    if (k >= 2 && n >= 2) {
+        uint64_t npairs = (uint64_t)n * (uint64_t)(n - 1) / 2;
+        k_tab_pairs<<<tab_grid(npairs), kTabBlock>>>(v, t->wide, n, k, dzh, dzl, dlg, rbuf, rcnt, retry_cap);
+    }
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        set_error("table build launch failed");
+        rc = TDP_EHIP;
+    }
+    // wide tables: inserts that met an unpublished slot, again until none is left
+    int which = 0;
+    for (int round = 0; rc == TDP_OK && round < 1000; round++) {
+        unsigned long long m = 0;
+        if (hipMemcpy(&m, rcnt + which, 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = TDP_EHIP; break; }
+        if (m == 0) break;
+        if (m > retry_cap) {
+            set_error("table build: %llu deferred inserts exceed the retry list", m);
+            rc = TDP_ECAPACITY;
+            break;
+        }
+        int other = which ^ 1;
+        hipMemset(rcnt + other, 0, 8);
+        k_tab_retry<<<tab_grid(m), kTabBlock>>>(v, rbuf + which * retry_cap, m, rbuf + other * retry_cap,
+                                                rcnt + other, retry_cap);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) { rc = TDP_EHIP; break; }
+        which = other;
+    }
+    hipFree(rbuf);
+    hipFree(rcnt);
+    hipFree(d);
+    if (rc == TDP_EHIP) set_error("table build failed on the device");
+    return rc;
+}
+
+int tdp_tab_lookup_batch(void *p, const uint64_t *d_zh, const uint64_t *d_zl, const uint64_t *d_lg, const int *d_R,
+                         uint64_t m, int *d_out, void *stream) {
+    DevTable *t = static_cast<DevTable *>(p);
+    TDP_REQUIRE(t != nullptr, "null table");
+    if (!m) return TDP_OK;
+    k_tab_lookup<<<tab_grid(m), kTabBlock, 0, as_stream(stream)>>>(view(t), t->wide, d_zh, d_zl, d_lg, d_R, m, d_out);
+    TDP_LAUNCH_CHECK();
+    return TDP_OK;
+}
+
+}  // extern "C"
