@@ -170,6 +170,16 @@ int tdp_tab_build(void *table, int n, const uint64_t *zh, const uint64_t *zl, co
 int tdp_tab_lookup_batch(void *table, const uint64_t *d_zh, const uint64_t *d_zl, const uint64_t *d_lg,
                          const int *d_R, uint64_t m, int *d_out, void *stream);
 
+/* ------------------------------------------------------------------ batched dense plans
+ * d_items: nitems device records of tdp_batch_item_bytes() bytes (layout in tropidist_amd/csrc/tdp_batch.hip:
+ * op stream, peak bits, checkpoint interval, table / checkpoint / decision-map / output pointers); each
+ * workgroup of one persistent launch takes the next record from the queue counter d_next and runs that plan alone,
+ * with its witness when asked (checkpointed traceback with one decision bit per state and update).  No
+ * reference counterpart: the reference runs plans one at a time (cdist/auto.py:105-123); this serves the batched
+ * scans and the serving loop. */
+int tdp_batch_run(void *d_items, int nitems, int *d_next, int workgroups, void *stream);
+int tdp_batch_item_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif

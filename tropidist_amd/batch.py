@@ -1,0 +1,184 @@
+"""Batched certification of many dense distance problems in one persistent GPU launch.
+
+    certify_batch(problems, orders=None)  -> one certificate record per problem (the fields of auto.certify)
+
+Pass 1 runs every (problem, observable) plan for its value; pass 2 runs, for each problem, the plan of its
+minimising observable again with the witness traceback.  Each pass is one launch of tdp_batch_run: a queue of plans
+sorted by decreasing work, one workgroup per plan (no launch per op, no host round trip per plan).  Witnesses are
+checked against H and L on the host.
+
+The per-plan checkpoint interval S and the number of resident workgroups come from tropidist_amd.autotune (the
+tuned value for the plan's width bucket, or sqrt(8 * ops) and 2 workgroups per CU by default)."""
+from __future__ import annotations
+
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib, auto
+from .problem import verify_witness
+
+ALIGN = 256
+
+ITEM_DTYPE = np.dtype([("ops", "<u8"), ("nops", "<i8"), ("peak_bits", "<i4"), ("S", "<i4"),
+                       ("tab0", "<u8"), ("tab1", "<u8"), ("ckpt", "<u8"), ("ck_n", "<u8"), ("bits", "<u8"),
+                       ("used", "<u8"), ("result", "<u8"), ("witness", "<i4"), ("pad", "<i4")])
+
+
+def _up(x):
+    return (x + ALIGN - 1) // ALIGN * ALIGN
+
+
+def default_interval(nops):
+    return max(1, int(round(math.sqrt(8 * max(1, nops)))))
+
+
+def plan_bytes(plan, witness, S=None):
+    """Device bytes of one batch item (tables, ops, and for witnesses checkpoints, decision maps, used list)."""
+    cap = 1 << plan.peak_bits
+    nops = len(plan.ops)
+    b = 2 * _up(cap) + _up(24 * max(1, nops)) + 64
+    if witness:
+        S = S or default_interval(nops)
+        nseg = (nops + S - 1) // S
+        b += nseg * _up(cap) + _up(8 * max(1, nseg)) + S * _up(max(4, cap // 8)) + 8 * max(1, nops) + ALIGN
+    return b
+
+
+def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None):
+    """Run plans in one persistent launch.  Returns a list of (value or None, used op indices or None).  value 255
+    means a cost >= 255 (callers apply final_value)."""
+    L = _lib.lib()
+    assert ITEM_DTYPE.itemsize == L.tdp_batch_item_bytes(), "BatchItem layout mismatch"
+    t = _lib.torch()
+    idx = [i for i, p in enumerate(plans) if p.feasible]
+    out = [(None, None)] * len(plans)
+    if not idx:
+        return out
+    S = {i: (intervals[i] if intervals else default_interval(len(plans[i].ops))) for i in idx}
+    sizes = {i: plan_bytes(plans[i], witness, S[i]) for i in idx}
+    arena = _lib.device_empty(sum(sizes.values()) + ALIGN)
+    base = _up(arena.data_ptr())
+    items = np.zeros(len(idx), dtype=ITEM_DTYPE)
+    # queue order: most work first; all op streams first, in one contiguous region copied once
+    order = sorted(idx, key=lambda i: -plans[i].work())
+    ops_blob = [plans[i].encode().reshape(-1) for i in order]
+    ops_off, off = [], 0
+    for a in ops_blob:
+        ops_off.append(off)
+        off += _up(8 * max(3, a.size))
+    host = np.zeros(off, dtype=np.uint8)
+    for o, a in zip(ops_off, ops_blob):
+        host[o:o + a.nbytes] = a.view(np.uint8)
+    r0 = base - arena.data_ptr()
+    arena[r0:r0 + off].copy_(t.from_numpy(host))
+    off += base
+    res_base = off                                   # results of all items, contiguous
+    off += _up(24 * len(order))
+    used_base = off                                  # used-op lists of all items, contiguous
+    used_off = []
+    if witness:
+        for i in order:
+            used_off.append(off - used_base)
+            off += 8 * max(1, len(plans[i].ops))
+        off = _up(off)
+    for slot, i in enumerate(order):
+        p = plans[i]
+        cap = 1 << p.peak_bits
+        nops = len(p.ops)
+        it = items[slot]
+        it["ops"] = base + ops_off[slot]
+        it["nops"], it["peak_bits"], it["S"] = nops, p.peak_bits, S[i]
+        it["tab0"] = off
+        off += _up(cap)
+        it["tab1"] = off
+        off += _up(cap)
+        it["result"] = res_base + 24 * slot
+        it["witness"] = int(witness)
+        if witness:
+            nseg = (nops + S[i] - 1) // S[i]
+            it["ckpt"] = off
+            off += nseg * _up(cap)
+            it["ck_n"] = off
+            off += _up(8 * max(1, nseg))
+            it["bits"] = off
+            off += S[i] * _up(max(4, cap // 8))
+            it["used"] = used_base + used_off[slot]
+    assert off <= arena.data_ptr() + arena.numel()
+    d_items = t.from_numpy(items.view(np.uint8)).cuda()
+    d_next = t.zeros(1, dtype=t.int32, device="cuda")
+    if workgroups is None:
+        cus = ctypes.c_int(0)
+        _lib.check(L.tdp_num_cus(ctypes.byref(cus)), "tdp_num_cus")
+        workgroups = 2 * cus.value
+    workgroups = max(1, min(workgroups, len(idx)))
+    _lib.check(L.tdp_batch_run(d_items.data_ptr(), len(idx), d_next.data_ptr(), workgroups,
+                               _lib.stream_handle(stream)), "tdp_batch_run")
+    t.cuda.synchronize()
+    a0 = res_base - arena.data_ptr()
+    res = arena[a0:a0 + 24 * len(order)].cpu().numpy().view(np.int64).reshape(-1, 3)
+    used_all = None
+    if witness:
+        u0 = used_base - arena.data_ptr()
+        used_all = arena[u0:u0 + used_off[-1] + 8 * max(1, len(plans[order[-1]].ops))].cpu().numpy().view(np.int64)
+    for slot, i in enumerate(order):
+        value, nused, err = (int(x) for x in res[slot])
+        if err:
+            raise RuntimeError("batched plan %d failed (code %d)" % (i, err))
+        used = None
+        if witness and value != 255:
+            u = used_off[slot] // 8
+            used = used_all[u:u + nused].copy()
+        out[i] = (value, used)
+    return out
+
+
+def _columns(plan, order, used):
+    upd, k = {}, 0
+    for idx, op in enumerate(plan.ops):
+        if op[0] == "update":
+            upd[idx] = plan.columns[k]
+            k += 1
+    return sorted(order[upd[int(i)]] for i in used)
+
+
+def certify_batch(problems, orders=None, workgroups=None, intervals=None):
+    """Certificates of many problems: {"distance", "lower_bound", "upper_bound", "witness", "observable",
+    "witness_ok", "per_observable"} per problem, as auto.certify; values >= 255 raise OverflowError."""
+    t0 = time.time()
+    orders = orders or [auto.best_order(p)[0] for p in problems]
+    flat, owner = [], []
+    for pi, (p, o) in enumerate(zip(problems, orders)):
+        for i, plan in auto._plans(p, o):
+            flat.append(plan)
+            owner.append((pi, i))
+    values = run_plans(flat, witness=False, workgroups=workgroups)
+    recs = [{"distance": None, "per_observable": [None] * p.nobs, "lower_bound": None, "upper_bound": None,
+             "witness": None, "observable": None, "witness_ok": False} for p in problems]
+    for (pi, i), (v, _) in zip(owner, values):
+        if v is not None:
+            if v == 255:
+                raise OverflowError("the distance is at least 255, beyond the 8-bit entries of the dense DP")
+            recs[pi]["per_observable"][i] = v
+    wplans, wowner = [], []
+    for pi, rec in enumerate(recs):
+        vals = [v for v in rec["per_observable"] if v is not None]
+        if vals:
+            best = min(vals)
+            obs = rec["per_observable"].index(best)
+            rec.update(distance=best, lower_bound=best, observable=obs)
+            wplans.append(flat[owner.index((pi, obs))])
+            wowner.append(pi)
+    traced = run_plans(wplans, witness=True, workgroups=workgroups,
+                       intervals=None if intervals is None else [intervals(p) for p in wplans])
+    for pi, plan, (v, used) in zip(wowner, wplans, traced):
+        rec = recs[pi]
+        cols = _columns(plan, orders[pi], used)
+        ok = v == rec["distance"] and verify_witness(problems[pi], cols, v, rec["observable"])
+        rec.update(witness=cols, witness_ok=ok, upper_bound=problems[pi].cost(cols))
+    dt = time.time() - t0
+    for rec in recs:
+        rec["batch_seconds"] = dt
+    return recs
