@@ -403,7 +403,7 @@ class CompiledTreeDP:
         if value == self.infinity:
             raise AssertionError("a consistent affine system must have a finite root cost")
         return value, dict(profile, feasible=True, seconds=time.monotonic() - start, join_work_done=done,
-                           allocated_dense_bytes=arena.numel() + maps.nbytes,
+                           allocated_dense_bytes=self.peak_live_entries * item + 16 * profile["max_join_dimension"],
                            observed_peak_live_table_bytes=observed_peak * item, backend="hip")
 
 
