@@ -1,0 +1,56 @@
+"""Serving loop on the GPU: every request's result is the same alone and in the mix (continuous batching,
+overlapping lanes, footprint admission), and a short Poisson stream completes with recorded latencies."""
+import numpy as np
+import pytest
+
+from tropidist_amd import auto, problem, serve, workloads
+from tropidist_amd.problem import verify_witness
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        if i % 7 == 3:
+            p = problem.quasi_cyclic(int(rng.integers(3, 6)), int(rng.integers(3, 6)), seed=seed + i)
+            out.append(("sparse", p, int(rng.integers(3, 7))))
+        else:
+            out.append(("dense", workloads.banded_instance(rng, seed + i, int(rng.integers(6, 15)), bool(i % 2)), None))
+    return out
+
+
+def _alone(kind, p, K):
+    if kind == "dense":
+        r = auto.certify(p)
+        return r["distance"], r["per_observable"]
+    r = serve.sparse_certify(p, K)
+    return r["distance"], None
+
+
+def test_results_alone_equal_results_in_the_mix():
+    print("case serve mix", flush=True)
+    reqs = _mix(42, 9100)
+    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=2, sparse_lanes=2, max_batch=8)
+    tickets = [srv.submit_dense(p) if kind == "dense" else srv.submit_sparse(p, K) for kind, p, K in reqs]
+    recs = [tk.wait(timeout=300) for tk in tickets]
+    srv.close()
+    for (kind, p, K), rec in zip(reqs, recs):
+        d, per = _alone(kind, p, K)
+        assert rec["distance"] == d, (kind, p.name)
+        if kind == "dense":
+            assert rec["per_observable"] == per
+        if d is not None:
+            assert rec["witness_ok"] and verify_witness(p, rec["witness"], d, rec["observable"])
+
+
+def test_poisson_stream_completes_with_latencies():
+    print("case serve poisson", flush=True)
+    reqs = [("dense", (p,)) if kind == "dense" else ("sparse", (p, K)) for kind, p, K in _mix(30, 9300)]
+    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=2, sparse_lanes=1)
+    tickets, wall = serve.run_poisson(srv, reqs, rate=50.0, seed=1)
+    srv.close()
+    lat = [tk.latency for tk in tickets]
+    assert all(x is not None and x > 0 for x in lat) and all(tk.error is None for tk in tickets)
+    assert serve.percentile(lat, 50) <= serve.percentile(lat, 99)

@@ -116,7 +116,7 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
     workgroups = max(1, min(workgroups, len(idx)))
     _lib.check(L.tdp_batch_run(d_items.data_ptr(), len(idx), d_next.data_ptr(), workgroups,
                                _lib.stream_handle(stream)), "tdp_batch_run")
-    t.cuda.synchronize()
+    (stream if stream is not None else t.cuda.current_stream()).synchronize()
     a0 = res_base - arena.data_ptr()
     res = arena[a0:a0 + 24 * len(order)].cpu().numpy().view(np.int64).reshape(-1, 3)
     used_all = None
@@ -144,17 +144,20 @@ def _columns(plan, order, used):
     return sorted(order[upd[int(i)]] for i in used)
 
 
-def certify_batch(problems, orders=None, workgroups=None, intervals=None):
-    """Certificates of many problems: {"distance", "lower_bound", "upper_bound", "witness", "observable",
-    "witness_ok", "per_observable"} per problem, as auto.certify; values >= 255 raise OverflowError."""
-    t0 = time.time()
+def plan_problems(problems, orders=None):
+    """Host planning of a batch: (orders, flat list of plans, owner (problem, observable) of each plan)."""
     orders = orders or [auto.best_order(p)[0] for p in problems]
     flat, owner = [], []
     for pi, (p, o) in enumerate(zip(problems, orders)):
         for i, plan in auto._plans(p, o):
             flat.append(plan)
             owner.append((pi, i))
-    values = run_plans(flat, witness=False, workgroups=workgroups)
+    return orders, flat, owner
+
+
+def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=None, stream=None):
+    """The device part of certify_batch on plans made by plan_problems."""
+    values = run_plans(flat, witness=False, workgroups=workgroups, stream=stream)
     recs = [{"distance": None, "per_observable": [None] * p.nobs, "lower_bound": None, "upper_bound": None,
              "witness": None, "observable": None, "witness_ok": False} for p in problems]
     for (pi, i), (v, _) in zip(owner, values):
@@ -162,6 +165,7 @@ def certify_batch(problems, orders=None, workgroups=None, intervals=None):
             if v == 255:
                 raise OverflowError("the distance is at least 255, beyond the 8-bit entries of the dense DP")
             recs[pi]["per_observable"][i] = v
+    where = {o: k for k, o in enumerate(owner)}
     wplans, wowner = [], []
     for pi, rec in enumerate(recs):
         vals = [v for v in rec["per_observable"] if v is not None]
@@ -169,16 +173,51 @@ def certify_batch(problems, orders=None, workgroups=None, intervals=None):
             best = min(vals)
             obs = rec["per_observable"].index(best)
             rec.update(distance=best, lower_bound=best, observable=obs)
-            wplans.append(flat[owner.index((pi, obs))])
+            wplans.append(flat[where[(pi, obs)]])
             wowner.append(pi)
-    traced = run_plans(wplans, witness=True, workgroups=workgroups,
+    traced = run_plans(wplans, witness=True, workgroups=workgroups, stream=stream,
                        intervals=None if intervals is None else [intervals(p) for p in wplans])
     for pi, plan, (v, used) in zip(wowner, wplans, traced):
         rec = recs[pi]
         cols = _columns(plan, orders[pi], used)
         ok = v == rec["distance"] and verify_witness(problems[pi], cols, v, rec["observable"])
         rec.update(witness=cols, witness_ok=ok, upper_bound=problems[pi].cost(cols))
-    dt = time.time() - t0
+    return recs
+
+
+def certify_planned_one_at_a_time(problems, orders, flat, owner):
+    """The plain-GPU baseline on the same plans: each problem alone, one kernel per op (kernel.run_plan) and the
+    per-op checkpointed witness (auto.witness_plan)."""
+    from . import kernel
+    recs = [{"distance": None, "per_observable": [None] * p.nobs, "witness": None, "witness_ok": False}
+            for p in problems]
+    by_problem = {}
+    for plan, (pi, i) in zip(flat, owner):
+        by_problem.setdefault(pi, []).append((i, plan))
+    for pi, items in sorted(by_problem.items()):
+        rec = recs[pi]
+        for i, plan in items:
+            rec["per_observable"][i] = kernel.run_plan(plan)[0]
+        vals = [v for v in rec["per_observable"] if v is not None]
+        if vals:
+            best = min(vals)
+            obs = rec["per_observable"].index(best)
+            plan = dict(items)[obs]
+            v, cols = auto.witness_plan(plan, orders[pi])
+            rec.update(distance=best, observable=obs, witness=cols,
+                       witness_ok=v == best and verify_witness(problems[pi], cols, v, obs))
+    return recs
+
+
+def certify_batch(problems, orders=None, workgroups=None, intervals=None, stream=None):
+    """Certificates of many problems: {"distance", "lower_bound", "upper_bound", "witness", "observable",
+    "witness_ok", "per_observable"} per problem, as auto.certify; values >= 255 raise OverflowError."""
+    t0 = time.time()
+    orders, flat, owner = plan_problems(problems, orders)
+    t1 = time.time()
+    recs = certify_planned(problems, orders, flat, owner, workgroups=workgroups, intervals=intervals, stream=stream)
+    t2 = time.time()
     for rec in recs:
-        rec["batch_seconds"] = dt
+        rec["planning_seconds"] = t1 - t0
+        rec["batch_seconds"] = t2 - t1
     return recs

@@ -1,0 +1,242 @@
+"""Serving loop for exact distance queries on one GPU: continuous batching with admission by device footprint.
+
+    srv = Server(memory_bytes=..., dense_lanes=2, sparse_lanes=2)
+    ticket = srv.submit_dense(problem, order=None)          # certificate of the dense DP
+    ticket = srv.submit_sparse(problem, K, order=None)      # least weight <= K by the sparse MITM DP, or None
+    ticket.wait() -> record;  srv.close()
+
+Dense requests wait in a FIFO queue.  Each dense lane is a thread with its own HIP stream: when free, it takes
+every waiting request whose plans fit in the free device budget (in arrival order, up to max_batch), runs them as
+one batched launch (batch.certify_planned: one persistent kernel for the values, one for the witnesses) and
+completes them; requests that arrive meanwhile are admitted by the next free lane, so batches of different lanes
+overlap on the device.  Sparse requests run on sparse lanes (threads with their own streams) through
+sparse_dp.query, one request per lane at a time, admitted against the same budget with a reservation of
+`sparse_bytes`.  Host planning of a request happens on its lane before admission.
+
+The result of a request does not depend on what it was batched with: every plan runs alone in its workgroup on
+its own tables (tests/test_gpu_serve.py checks each result alone and in the mix)."""
+from __future__ import annotations
+
+import collections
+import threading
+import time
+
+from . import _lib, auto, batch, sparse_dp
+
+
+class Ticket:
+    __slots__ = ("kind", "problem", "order", "K", "t_submit", "t_start", "t_done", "result", "error", "_event",
+                 "plans", "footprint")
+
+    def __init__(self, kind, problem, order, K):
+        self.kind, self.problem, self.order, self.K = kind, problem, order, K
+        self.t_submit = time.perf_counter()
+        self.t_start = self.t_done = None
+        self.result = self.error = None
+        self._event = threading.Event()
+        self.plans = None
+        self.footprint = 0
+
+    def wait(self, timeout=None):
+        self._event.wait(timeout)
+        if self.error is not None:
+            raise self.error
+        return self.result
+
+    @property
+    def latency(self):
+        return None if self.t_done is None else self.t_done - self.t_submit
+
+
+class Server:
+    def __init__(self, memory_bytes=64 << 30, dense_lanes=2, sparse_lanes=2, max_batch=1024, sparse_bytes=1 << 30):
+        self.budget = memory_bytes
+        self.in_use = 0
+        self.max_batch = max_batch
+        self.sparse_bytes = sparse_bytes
+        self.dense_q = collections.deque()
+        self.sparse_q = collections.deque()
+        self.cv = threading.Condition()
+        self.closed = False
+        self.done = []
+        _lib.lib()
+        t = _lib.torch()
+        self.threads = []
+        for k in range(dense_lanes):
+            th = threading.Thread(target=self._dense_lane, args=(t.cuda.Stream(),), daemon=True)
+            th.start()
+            self.threads.append(th)
+        for k in range(sparse_lanes):
+            th = threading.Thread(target=self._sparse_lane, args=(t.cuda.Stream(),), daemon=True)
+            th.start()
+            self.threads.append(th)
+
+    # ------------------------------------------------------------------ submission
+    def submit_dense(self, problem, order=None):
+        tk = Ticket("dense", problem, order, None)
+        with self.cv:
+            self.dense_q.append(tk)
+            self.cv.notify_all()
+        return tk
+
+    def submit_sparse(self, problem, K, order=None):
+        tk = Ticket("sparse", problem, order, K)
+        with self.cv:
+            self.sparse_q.append(tk)
+            self.cv.notify_all()
+        return tk
+
+    def close(self):
+        with self.cv:
+            self.closed = True
+            self.cv.notify_all()
+        for th in self.threads:
+            th.join()
+
+    # ------------------------------------------------------------------ lanes
+    def _finish(self, tickets, results=None, error=None):
+        now = time.perf_counter()
+        with self.cv:
+            for k, tk in enumerate(tickets):
+                tk.t_done = now
+                if error is not None:
+                    tk.error = error
+                else:
+                    tk.result = results[k]
+                self.in_use -= tk.footprint
+                tk._event.set()
+                self.done.append(tk)
+            self.cv.notify_all()
+
+    @staticmethod
+    def _plan(tk):
+        if tk.plans is None:
+            order = tk.order if tk.order is not None else auto.best_order(tk.problem)[0]
+            tk.order = order
+            orders, flat, owner = batch.plan_problems([tk.problem], [order])
+            tk.plans = (flat, owner)
+            tk.footprint = sum(batch.plan_bytes(p, False) for p in flat if p.feasible) + \
+                max((batch.plan_bytes(p, True) for p in flat if p.feasible), default=0)
+        return tk
+
+    def _dense_lane(self, stream):
+        t = _lib.torch()
+        while True:
+            with self.cv:
+                while not self.dense_q and not self.closed:
+                    self.cv.wait()
+                if not self.dense_q and self.closed:
+                    return
+                claimed = [self.dense_q.popleft() for _ in range(min(self.max_batch, len(self.dense_q)))]
+            for tk in claimed:                    # host planning outside the lock
+                self._plan(tk)
+            while claimed:
+                with self.cv:
+                    # admit the longest prefix that fits the free budget (at least one request when idle)
+                    take, need = [], 0
+                    for tk in claimed:
+                        if self.in_use + need + tk.footprint > self.budget and (take or self.in_use > 0):
+                            break
+                        take.append(tk)
+                        need += tk.footprint
+                    if not take:
+                        self.cv.wait(0.005)
+                        continue
+                    self.in_use += need
+                    now = time.perf_counter()
+                    for tk in take:
+                        tk.t_start = now
+                claimed = claimed[len(take):]
+                probs = [tk.problem for tk in take]
+                orders = [tk.order for tk in take]
+                flat, owner = [], []
+                for k, tk in enumerate(take):
+                    f, o = tk.plans
+                    flat += f
+                    owner += [(k, i) for (_, i) in o]
+                try:
+                    with t.cuda.stream(stream):
+                        recs = batch.certify_planned(probs, orders, flat, owner, stream=stream)
+                    self._finish(take, recs)
+                except Exception as exc:          # delivered to every waiter of the failed batch
+                    self._finish(take, error=exc)
+
+    def _sparse_lane(self, stream):
+        t = _lib.torch()
+        while True:
+            with self.cv:
+                while (not self.sparse_q or self.in_use + self.sparse_bytes > self.budget) and not self.closed:
+                    self.cv.wait(0.01)
+                if not self.sparse_q and self.closed:
+                    return
+                if not self.sparse_q:
+                    continue
+                tk = self.sparse_q.popleft()
+                tk.footprint = self.sparse_bytes
+                self.in_use += tk.footprint
+                tk.t_start = time.perf_counter()
+            try:
+                with t.cuda.stream(stream):
+                    rec = sparse_certify(tk.problem, tk.K, tk.order)
+                self._finish([tk], [rec])
+            except Exception as exc:
+                self._finish([tk], error=exc)
+
+    # ------------------------------------------------------------------ statistics
+    def latencies(self, kind=None):
+        return [tk.latency for tk in self.done if tk.latency is not None and (kind is None or tk.kind == kind)]
+
+
+def sparse_certify(p, K, order=None):
+    """Least weight <= K of a solution over all observables by the sparse MITM DP (auto.sparse_distance's loop,
+    cdist/auto.py:257-284, with the default split): {"distance", "witness", "observable", "witness_ok"}."""
+    from .ordering import greedy_straddle_order
+    from .problem import verify_witness
+    order = order if order is not None else greedy_straddle_order(p.h, p.nrows)
+    best = None
+    for i in range(p.nobs):
+        tr = sparse_dp.Trellis.for_query(p, i, order)
+        if not tr.feasible:
+            continue
+        k = K if best is None else best[0] - 1
+        if k < 1:
+            break
+        r = sparse_dp.query(tr, k, mode="mitm", use_h=False, witness=True, backend="hip")
+        if r["value"] is not None:
+            best = (r["value"], r["witness"], i)
+    if best is None:
+        return {"distance": None, "witness": None, "observable": None, "witness_ok": True, "K": K}
+    v, cols, obs = best
+    return {"distance": v, "witness": cols, "observable": obs, "K": K,
+            "witness_ok": verify_witness(p, cols, v, obs)}
+
+
+def percentile(xs, q):
+    if not xs:
+        return None
+    xs = sorted(xs)
+    i = min(len(xs) - 1, max(0, int(round(q / 100.0 * (len(xs) - 1)))))
+    return xs[i]
+
+
+def run_poisson(server, requests, rate, seed=0):
+    """Submit requests at Poisson arrival times of the given rate (per second); wait for all; returns
+    (tickets, wall seconds)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=len(requests))
+    tickets = []
+    t0 = time.perf_counter()
+    at = t0
+    for (kind, args), gap in zip(requests, gaps):
+        at += gap
+        now = time.perf_counter()
+        if at > now:
+            time.sleep(at - now)
+        if kind == "dense":
+            tickets.append(server.submit_dense(*args))
+        else:
+            tickets.append(server.submit_sparse(*args))
+    for tk in tickets:
+        tk._event.wait()
+    return tickets, time.perf_counter() - t0
