@@ -180,6 +180,16 @@ int tdp_tab_lookup_batch(void *table, const uint64_t *d_zh, const uint64_t *d_zl
 int tdp_batch_run(void *d_items, int nitems, int *d_next, int workgroups, void *stream);
 int tdp_batch_item_bytes(void);
 
+/* ------------------------------------------------------------------ host planner
+ * Dense DP plans of nq queries sharing one check matrix, in C++ (the steps of cdist/trellis.py:47-193).  h_words:
+ * ncols columns of ceil(nrows / 64) uint64 words; order: the column order; targets: nq x ncols bytes, 1 where the
+ * column flips the query's logical parity; weights: per column (NULL = unit).  Per query q: ops + 3 * q * ops_cap
+ * receives the op stream (code, a, b), columns + q * ops_cap the column position of each update, and nops[q],
+ * peak[q], work[q] (sum over updates of the table size), feasible[q].  ops_cap >= ncols + 2 * (nrows + 1). */
+int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t *order, const uint8_t *targets,
+                   int nq, const int32_t *weights, uint64_t *ops, int64_t ops_cap, int64_t *nops, int32_t *peak,
+                   double *work, int32_t *columns, int32_t *feasible);
+
 #ifdef __cplusplus
 }
 #endif

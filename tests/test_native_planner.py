@@ -1,0 +1,36 @@
+"""The C++ planner (tdp_plan_build) emits the same op streams, widths and work as the Python restatement of the
+reference's planner, on random problems (with costs) and on every recorded fixture."""
+import random
+
+from helpers import dense_fixtures, random_problem
+from tropidist_amd import trellis, workloads
+
+
+def _same(a, b):
+    assert a.feasible == b.feasible
+    if a.feasible:
+        assert a.ops == b.ops and a.peak_bits == b.peak_bits and a.work() == b.work() and a.columns == b.columns
+        assert (a.encode() == b.encode()).all()
+
+
+def test_random_problems():
+    rng = random.Random(1)
+    for t in range(2000):
+        m, k, h, l = random_problem(rng)
+        order = list(range(len(h)))
+        rng.shuffle(order)
+        w = [rng.randint(1, 3) for _ in h] if t % 2 else None
+        masks = [rng.randrange(1, 1 << k) for _ in range(2)]
+        native = trellis.build_plans_native(h, l, masks, order, m, weights=w)
+        for om, b in zip(masks, native):
+            _same(trellis.build_plan(h, l, om, order, m, weights=w), b)
+
+
+def test_recorded_fixtures_and_banded():
+    for r in dense_fixtures():
+        masks = [1 << p["observable"] for p in r["plans"]]
+        for om, b in zip(masks, trellis.build_plans_native(r["h"], r["l"], masks, r["order"], r["nrows"])):
+            _same(trellis.build_plan(r["h"], r["l"], om, r["order"], r["nrows"]), b)
+    p = workloads.single_cpu()
+    order = list(range(p.ncols))
+    _same(trellis.build_plan(p.h, p.l, 1, order, p.nrows), trellis.build_plans_native(p.h, p.l, [1], order, p.nrows)[0])

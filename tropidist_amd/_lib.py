@@ -51,6 +51,7 @@ SIGNATURES = {
     "tdp_tab_lookup_batch": (ci, [vp, vp, vp, vp, vp, u64, vp, vp]),
     "tdp_batch_run": (ci, [vp, ci, vp, ci, vp]),
     "tdp_batch_item_bytes": (ci, []),
+    "tdp_plan_build": (ci, [vp, ci, ci, vp, vp, ci, vp, vp, i64, vp, vp, vp, vp, vp]),
 }
 
 

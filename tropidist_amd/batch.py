@@ -145,11 +145,14 @@ def _columns(plan, order, used):
 
 
 def plan_problems(problems, orders=None):
-    """Host planning of a batch: (orders, flat list of plans, owner (problem, observable) of each plan)."""
+    """Host planning of a batch with the C++ planner: (orders, flat list of plans, owner (problem, observable) of
+    each plan)."""
+    from .trellis import build_plans_native
     orders = orders or [auto.best_order(p)[0] for p in problems]
     flat, owner = [], []
     for pi, (p, o) in enumerate(zip(problems, orders)):
-        for i, plan in auto._plans(p, o):
+        plans = build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], o, p.nrows, p.w)
+        for i, plan in enumerate(plans):
             flat.append(plan)
             owner.append((pi, i))
     return orders, flat, owner

@@ -1,0 +1,231 @@
+// Host planner of the dense trellis DP in C++ (the steps of tropidist_amd/trellis.py, which restates
+// cdist/trellis.py:47-193): rows over column positions, minimal span form, target-row reduction, op stream.
+// Rows are bitsets over positions with their nonzero word span tracked, so the XORs of the span-form passes
+// touch only the words a row occupies (rows of local codes are short intervals).
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/tdp.h"
+
+namespace tdp {
+void set_error(const char *fmt, ...);
+}
+
+namespace {
+
+struct Row {
+    std::vector<uint64_t> w;
+    int lo = 0, hi = 0;          // nonzero words lie in [lo, hi)
+
+    bool zero() const { return lo >= hi; }
+    void trim() {
+        while (lo < hi && w[lo] == 0) lo++;
+        while (hi > lo && w[hi - 1] == 0) hi--;
+        if (lo >= hi) lo = hi = 0;
+    }
+    int lowbit() const { return zero() ? -1 : lo * 64 + __builtin_ctzll(w[lo]); }
+    int bitlen() const { return zero() ? 0 : (hi - 1) * 64 + 64 - __builtin_clzll(w[hi - 1]); }
+    bool test(int pos) const { return (w[pos >> 6] >> (pos & 63)) & 1; }
+    void xor_with(const Row &o) {
+        if (o.zero()) return;
+        int a = zero() ? o.lo : std::min(lo, o.lo), b = zero() ? o.hi : std::max(hi, o.hi);
+        for (int k = o.lo; k < o.hi; k++) w[k] ^= o.w[k];
+        lo = a;
+        hi = b;
+        trim();
+    }
+};
+
+// pass 1 keyed by start (the row ending earlier keeps the key), pass 2 keyed by end (the row starting earlier
+// absorbs the other); dependent rows vanish
+std::vector<Row> minimal_span_form(std::vector<Row> rows, int npos) {
+    std::vector<int> by_start(npos + 1, -1);
+    std::vector<Row> store;
+    store.reserve(rows.size());
+    for (Row &r : rows) {
+        while (!r.zero()) {
+            int s = r.lowbit();
+            if (by_start[s] < 0) {
+                by_start[s] = (int)store.size();
+                store.push_back(r);
+                break;
+            }
+            Row &held = store[by_start[s]];
+            if (r.bitlen() < held.bitlen()) std::swap(r, held);
+            r.xor_with(held);
+        }
+    }
+    std::vector<Row *> stack;
+    for (Row &r : store) stack.push_back(&r);
+    std::sort(stack.begin(), stack.end(), [](Row *a, Row *b) { return a->lowbit() > b->lowbit(); });
+    std::vector<int> by_end(npos + 2, -1);
+    std::vector<Row> out;
+    out.reserve(store.size());
+    for (Row *rp : stack) {
+        Row r = *rp;
+        while (true) {
+            int e = r.bitlen();
+            if (by_end[e] < 0) {
+                by_end[e] = (int)out.size();
+                out.push_back(r);
+                break;
+            }
+            Row &held = out[by_end[e]];
+            if (held.lowbit() > r.lowbit()) {
+                r.xor_with(held);
+            } else {
+                Row next = held;
+                next.xor_with(r);
+                held = r;
+                r = next;
+            }
+        }
+    }
+    // the reference keeps the order in which ends were first filled; out already is that order
+    return out;
+}
+
+Row reduce_target_row(Row lrow, const std::vector<Row> &hrows, int npos) {
+    std::vector<int> by_start(npos + 1, -1), by_end(npos + 2, -1);
+    for (size_t i = 0; i < hrows.size(); i++) {
+        by_start[hrows[i].lowbit()] = (int)i;
+        by_end[hrows[i].bitlen()] = (int)i;
+    }
+    auto clear_start = [&](Row &x) {
+        while (!x.zero()) {
+            int q = by_start[x.lowbit()];
+            if (q < 0) break;
+            x.xor_with(hrows[q]);
+        }
+    };
+    clear_start(lrow);
+    while (!lrow.zero()) {
+        int q = by_end[lrow.bitlen()];
+        if (q < 0 || hrows[q].lowbit() < lrow.lowbit()) break;
+        lrow.xor_with(hrows[q]);
+        clear_start(lrow);
+    }
+    return lrow;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t *order, const uint8_t *targets,
+                   int nq, const int32_t *weights, uint64_t *ops, int64_t ops_cap, int64_t *nops, int32_t *peak,
+                   double *work, int32_t *columns, int32_t *feasible) {
+    if (ncols < 0 || nrows < 0 || nq < 0 || ops_cap < 0) {
+        tdp::set_error("plan: bad sizes");
+        return TDP_EINVAL;
+    }
+    int rw = (nrows + 63) / 64, pw = (ncols + 63) / 64;
+    // rows over positions
+    std::vector<Row> rows(nrows);
+    for (Row &r : rows) r.w.assign(pw > 0 ? pw : 1, 0);
+    for (int pos = 0; pos < ncols; pos++) {
+        int j = order[pos];
+        if (j < 0 || j >= ncols) {
+            tdp::set_error("plan: order entry %d out of range", j);
+            return TDP_EINVAL;
+        }
+        const uint64_t *c = h_words + (int64_t)j * rw;
+        for (int k = 0; k < rw; k++)
+            for (uint64_t x = c[k]; x; x &= x - 1) {
+                int r = k * 64 + __builtin_ctzll(x);
+                if (r < nrows) rows[r].w[pos >> 6] |= 1ull << (pos & 63);
+            }
+    }
+    for (Row &r : rows) {
+        r.lo = 0;
+        r.hi = pw;
+        r.trim();
+    }
+    std::vector<Row> hrows = minimal_span_form(rows, ncols);
+    for (int q = 0; q < nq; q++) {
+        uint64_t *qops = ops + (int64_t)q * ops_cap * 3;
+        int32_t *qcols = columns + (int64_t)q * ops_cap;
+        Row lrow;
+        lrow.w.assign(pw > 0 ? pw : 1, 0);
+        for (int pos = 0; pos < ncols; pos++)
+            if (targets[(int64_t)q * ncols + order[pos]]) lrow.w[pos >> 6] |= 1ull << (pos & 63);
+        lrow.lo = 0;
+        lrow.hi = pw;
+        lrow.trim();
+        lrow = reduce_target_row(lrow, hrows, ncols);
+        if (lrow.zero()) {
+            feasible[q] = 0;
+            nops[q] = 0;
+            peak[q] = 0;
+            work[q] = 0;
+            continue;
+        }
+        feasible[q] = 1;
+        std::vector<const Row *> all;
+        for (const Row &r : hrows) all.push_back(&r);
+        all.push_back(&lrow);
+        std::vector<int> target(all.size(), 0);
+        target.back() = 1;
+        std::vector<std::vector<int>> opening(ncols), closing(ncols);
+        for (size_t i = 0; i < all.size(); i++) {
+            opening[all[i]->lowbit()].push_back((int)i);
+            closing[all[i]->bitlen() - 1].push_back((int)i);
+        }
+        std::vector<int> active;
+        int64_t n = 0, k = 0;
+        int bits = 0, pk = 0;
+        double wk = 0;
+        bool overflow = false;
+        auto emit = [&](uint64_t code, uint64_t a, uint64_t b) {
+            if (n >= ops_cap) { overflow = true; return; }
+            qops[3 * n] = code;
+            qops[3 * n + 1] = a;
+            qops[3 * n + 2] = b;
+            n++;
+        };
+        for (int j = 0; j < ncols && !overflow; j++) {
+            for (int idx : opening[j]) {
+                active.push_back(idx);
+                emit(0, 0, 0);
+                bits++;
+            }
+            pk = std::max(pk, (int)active.size());
+            uint64_t mask = 0;
+            for (size_t p = 0; p < active.size(); p++)
+                if (all[active[p]]->test(j)) mask |= 1ull << p;
+            if (mask) {
+                if (active.size() > 63) {
+                    tdp::set_error("plan: more than 63 active rows");
+                    return TDP_EINVAL;
+                }
+                qcols[k++] = j;
+                emit(1, mask, weights ? (uint64_t)weights[order[j]] : 1);
+                wk += (double)(1ull << bits);
+            }
+            std::vector<int> ending;
+            for (int idx : closing[j]) {
+                for (size_t p = 0; p < active.size(); p++)
+                    if (active[p] == idx) ending.push_back((int)p);
+            }
+            std::sort(ending.rbegin(), ending.rend());
+            for (int p : ending) {
+                emit(2, (uint64_t)p, (uint64_t)target[active[p]]);
+                active.erase(active.begin() + p);
+                bits--;
+            }
+        }
+        if (overflow) {
+            tdp::set_error("plan: op buffer of %lld entries too small", (long long)ops_cap);
+            return TDP_ECAPACITY;
+        }
+        nops[q] = n;
+        peak[q] = pk;
+        work[q] = wk;
+    }
+    return TDP_OK;
+}
+
+}  // extern "C"

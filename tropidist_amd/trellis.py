@@ -206,3 +206,83 @@ def build_plan(h, l, obs_mask, order, nrows, weights=None):
     lrow = reduce_target_row(target_row(l, obs_mask, order), hrows)
     pw = None if weights is None else [weights[j] for j in order]
     return make_plan(hrows, lrow, len(order), pw)
+
+
+class NativePlan(Plan):
+    """A Plan made by the C++ planner: the op stream is kept encoded and decoded only when .ops is read."""
+
+    def __init__(self, encoded, peak_bits, ncols, work, columns, feasible=True):
+        self._encoded = encoded
+        self._work = int(work)
+        Plan.__init__(self, None, peak_bits, [], ncols, feasible, list(columns))
+
+    @property
+    def ops(self):
+        out = []
+        for code, a, b in self._encoded.tolist():
+            if code == OP_EXPAND:
+                out.append(("expand",))
+            elif code == OP_UPDATE:
+                out.append(("update", a) if b == 1 else ("update", a, b))
+            else:
+                out.append(("select", a, b))
+        return out
+
+    @ops.setter
+    def ops(self, value):
+        pass
+
+    def encode(self):
+        return self._encoded
+
+    def work(self):
+        return self._work
+
+    def max_weight(self):
+        return int(self._encoded[self._encoded[:, 0] == OP_UPDATE, 2].max(initial=1))
+
+
+def _words(x, nw):
+    return np.frombuffer(int(x).to_bytes(8 * nw, "little"), dtype=np.uint64)
+
+
+def build_plans_native(h, l, obs_masks, order, nrows, weights=None):
+    """Plans of several queries (one per obs_mask) by the C++ planner of libtdp (tdp_plan_build); identical op
+    streams to make_plan.  Plans with more than 63 active rows are refused (ValueError)."""
+    import ctypes
+    from . import _lib
+    L = _lib.lib()
+    ncols, nq = len(order), len(obs_masks)
+    rw = max(1, (nrows + 63) // 64)
+    hw = np.zeros((max(1, len(h)), rw), dtype=np.uint64)
+    for j, c in enumerate(h):
+        if c:
+            hw[j] = _words(c, rw)
+    targets = np.zeros((max(1, nq), max(1, len(h))), dtype=np.uint8)
+    for q, m in enumerate(obs_masks):
+        for j, x in enumerate(l):
+            targets[q, j] = bin(x & m).count("1") & 1
+    order_a = np.asarray(order, dtype=np.int32)
+    w = None if weights is None else np.asarray(weights, dtype=np.int32)
+    cap = ncols + 2 * (nrows + 1) + 4
+    ops = np.zeros((max(1, nq), cap, 3), dtype=np.uint64)
+    cols = np.zeros((max(1, nq), cap), dtype=np.int32)
+    nops = np.zeros(max(1, nq), dtype=np.int64)
+    peak = np.zeros(max(1, nq), dtype=np.int32)
+    work = np.zeros(max(1, nq), dtype=np.float64)
+    feas = np.zeros(max(1, nq), dtype=np.int32)
+    p_ = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    rc = L.tdp_plan_build(p_(hw), ncols, nrows, p_(order_a), p_(targets), nq, None if w is None else p_(w),
+                          p_(ops), cap, p_(nops), p_(peak), p_(work), p_(cols), p_(feas))
+    if rc == _lib.TDP_EINVAL:
+        raise ValueError(L.tdp_last_error().decode())
+    _lib.check(rc, "tdp_plan_build")
+    out = []
+    for q in range(nq):
+        if not feas[q]:
+            out.append(Plan([], 0, [], ncols, feasible=False))
+            continue
+        n = int(nops[q])
+        nupd = int((ops[q, :n, 0] == OP_UPDATE).sum())
+        out.append(NativePlan(ops[q, :n].copy(), int(peak[q]), ncols, work[q], cols[q, :nupd].tolist()))
+    return out
