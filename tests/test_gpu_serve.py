@@ -14,7 +14,7 @@ def _mix(n, seed):
     out = []
     for i in range(n):
         if i % 7 == 3:
-            p = problem.quasi_cyclic(int(rng.integers(3, 6)), int(rng.integers(3, 6)), seed=seed + i)
+            p = problem.quasi_cyclic(int(rng.choice([3, 6, 7])), int(rng.choice([3, 6, 7])), seed=seed + i)
             out.append(("sparse", p, int(rng.integers(3, 7))))
         else:
             out.append(("dense", workloads.banded_instance(rng, seed + i, int(rng.integers(6, 15)), bool(i % 2)), None))

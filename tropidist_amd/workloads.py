@@ -55,8 +55,12 @@ def serving_mix(n=10000, seed=5000, dense_lo=(10, 20), dense_hi=(21, 25), qc=(6,
         u = rng.random()
         s = seed + i + 1
         if u < sparse_frac:
-            reqs.append(("sparse", {"l": int(rng.integers(qc[0], qc[1] + 1)), "m": int(rng.integers(qc[0], qc[1] + 1)),
-                                    "seed": s, "K": int(rng.integers(4, 11))}))
+            # torus sizes with a side divisible by 3: the two-block codes of the other sizes rarely encode qubits
+            while True:
+                l, m = int(rng.integers(qc[0], qc[1] + 1)), int(rng.integers(qc[0], qc[1] + 1))
+                if l % 3 == 0 or m % 3 == 0:
+                    break
+            reqs.append(("sparse", {"l": l, "m": m, "seed": s, "K": int(rng.integers(4, 11))}))
         elif u < sparse_frac + wide_frac:
             reqs.append(("dense", {"width": int(rng.integers(dense_hi[0], dense_hi[1] + 1)), "seed": s,
                                    "weighted": bool(rng.random() < 0.5)}))
