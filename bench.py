@@ -1,17 +1,25 @@
 #!/usr/bin/env python
-"""Benchmark: dense Rank DP states/s on one MI355X for the single large dense instance, with the roofline of the
-dominant kernel and the reference's CPU kernels timed on this host beside it.
+"""Benchmark of the exact distance evaluators on one MI355X.  Prints one JSON line.
 
-Workload (one "step" = one exact distance of the instance: every observable's plan, host planning excluded):
-banded random detector error model, seed 2000: 600 detectors, 6000 fault columns each flipping 2-6 detectors inside
-a 25-row sliding window, 4 logical rows of density 0.05, integer weights uniform in 1..3, uint8 saturating costs.
-The natural column order has rank width 25 and 2^26-entry tables (64 MiB).
+value = dense Rank DP states/s (sum of Plan.work per second) on the single large dense instance: banded random
+detector error model, seed 2000: 600 detectors, 6000 fault columns each flipping 2-6 detectors inside a 25-row
+sliding window, 4 logical rows of density 0.05, integer costs uniform in 1..3, uint8 saturating tables; the natural
+column order has rank width 25 and 2^26-entry tables.  One step = the exact distance of the instance (all four
+observables' plans; host planning is timed separately).
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu]
+Beside it (each can be skipped):
+  batched   512 banded scans (200-1200 detectors, 5-15 faults per detector, width 10-22, half with costs 1-3):
+            certificates (distance + checked witness) per second from one persistent launch per pass, against the
+            same plans certified one at a time with per-op kernels on a sample ("plain GPU");
+  sparse    the two-block quasi-cyclic code l = m = 12 (288 columns), meet-in-the-middle sweep at threshold K;
+  serving   a Poisson stream of requests from the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse
+            MITM on quasi-cyclic codes), p50 / p99 latency and certificates per second;
+  cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
 
-Prints one JSON line."""
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--skip-batched] [--skip-sparse]
+                    [--skip-serving] [--serving-requests 600] [--batch-instances 512] [--sparse-K 8]
+"""
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -23,40 +31,51 @@ sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
 
 
-def workload():
-    from tropidist_amd import problem, trellis
-    p = problem.banded(R=600, N=6000, kmin=2, kmax=6, W=25, nobs=4, rho=0.05, weights=(1, 3), seed=2000)
-    order = list(range(p.ncols))
-    plans = [trellis.build_plan(p.h, p.l, 1 << i, order, p.nrows, weights=p.w) for i in range(p.nobs)]
-    return p, [pl for pl in plans if pl.feasible]
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--roofline-launches", type=int, default=60)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (the sharding design is written up, "
-              "not built); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
+def _gen_scan(i):
+    from tropidist_amd import workloads
     import numpy as np
-    from tropidist_amd import _lib, kernel
-    torch = _lib.torch()
-    L = _lib.lib()
+    rng = np.random.default_rng(4000 + i)
+    return workloads.banded_instance(rng, 4000 + i + 1, int(rng.integers(10, 23)), weighted=bool(i % 2))
 
+
+def _gen_request(req):
+    from tropidist_amd import problem, workloads
+    import numpy as np
+    kind, a = req
+    if kind == "sparse":
+        return kind, problem.quasi_cyclic(a["l"], a["m"], seed=a["seed"]), a["K"]
+    rng = np.random.default_rng(a["seed"])
+    return kind, workloads.banded_instance(rng, a["seed"], a["width"], a["weighted"]), None
+
+
+def generate(args):
+    """Synthetic inputs of every section, made in worker processes before the GPU is touched."""
+    import multiprocessing as mp
+    from tropidist_amd import workloads
+    out = {}
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(min(16, os.cpu_count() or 4)) as pool:
+        if not args.skip_batched:
+            out["scan"] = pool.map(_gen_scan, range(args.batch_instances), chunksize=8)
+        if not args.skip_serving:
+            out["requests"] = pool.map(_gen_request, workloads.serving_mix(args.serving_requests,
+                                                                            K=(4, args.serving_max_K)), chunksize=8)
+    return out
+
+
+def bench_dense(args, L, torch):
+    import ctypes
+    from tropidist_amd import _lib, kernel, trellis, workloads
     t0 = time.time()
-    p, plans = workload()
+    p = workloads.single_large()
+    order = list(range(p.ncols))
+    plans = [pl for pl in trellis.build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], order, p.nrows, p.w)
+             if pl.feasible]
     plan_seconds = time.time() - t0
     peak = max(pl.peak_bits for pl in plans)
     total_work = sum(pl.work() for pl in plans)
     encoded = [pl.encode() for pl in plans]
-    T0 = _lib.device_empty(1 << peak)
-    T1 = _lib.device_empty(1 << peak)
+    T0, T1 = _lib.device_empty(1 << peak), _lib.device_empty(1 << peak)
     stream = torch.cuda.current_stream()
     sh = _lib.stream_handle(stream)
 
@@ -64,8 +83,8 @@ def main():
         vals = []
         for pl, ops in zip(plans, encoded):
             v = ctypes.c_int(-1)
-            _lib.check(L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(pl.ops), pl.peak_bits,
-                                      _lib.dptr(T0), _lib.dptr(T1), ctypes.byref(v), sh), "tdp_run_plan")
+            _lib.check(L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(ops), pl.peak_bits, _lib.dptr(T0),
+                                      _lib.dptr(T1), ctypes.byref(v), sh), "tdp_run_plan")
             vals.append(kernel.final_value(v.value))
         return vals
 
@@ -77,12 +96,11 @@ def main():
         values = step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
-    states_per_s = args.steps * total_work / elapsed
 
-    # Roofline of the dominant kernel (the streaming butterfly update at the peak table size): average launch
-    # duration from HIP events on the stream the kernel runs on; algorithmic bytes = read + write of every entry.
+    # roofline of the dominant kernel: the streaming butterfly update at the peak table size, average launch
+    # duration from HIP events on its stream; algorithmic bytes = read + write of every entry
     n = 1 << peak
-    masks = [op[1] for op in plans[0].ops if op[0] == "update" and op[1] < n and op[1] >= (n >> 1)]
+    masks = [int(m) for c, m, _ in encoded[0].tolist() if c == 1 and (n >> 1) <= m < n]
     masks = (masks * (args.roofline_launches // max(1, len(masks)) + 1))[:args.roofline_launches]
     _lib.check(L.tdp_fill(_lib.dptr(T0), n, 7, sh), "fill")
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -92,31 +110,133 @@ def main():
     ev1.record(stream)
     torch.cuda.synchronize()
     avg_launch = ev0.elapsed_time(ev1) / 1e3 / len(masks)
-    alg_bytes = 2 * n
-    achieved = alg_bytes / avg_launch
-
-    out = {
-        "metric": "Rank-DP states/s (dense trellis, sum of Plan.work / s)",
-        "value": states_per_s,
-        "unit": "states/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic (seeded banded detector error model)",
+    achieved = 2 * n / avg_launch
+    return p, plans, values, {
+        "value": args.steps * total_work / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
         "config": {"workload": "single large dense Rank DP: banded R=600 N=6000 k=2-6 W=25, 4 logical rows "
-                               "rho=0.05, weights 1-3, uint8 saturating, natural order",
+                               "rho=0.05, costs 1-3, uint8 saturating, natural order, seed 2000",
                    "columns": p.ncols, "observables": len(plans), "peak_bits": peak, "work_per_step": total_work,
                    "distances": values, "distance": min(values), "host_planning_seconds": round(plan_seconds, 3),
-                   "executor": "per-op streaming kernels"},
+                   "executor": "per-op streaming kernels (tdp_run_plan)"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "k_update_hi", "launch_seconds": avg_launch, "alg_bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "tdp::k_update_hi",
+                     "launch_seconds": avg_launch, "alg_bytes_per_launch": 2 * n, "table_entries": n},
     }
+
+
+def bench_batched(args, scan):
+    from tropidist_amd import batch
+    t0 = time.time()
+    orders = [list(range(p.ncols)) for p in scan]
+    orders, flat, owner = batch.plan_problems(scan, orders)
+    plan_s = time.time() - t0
+    # warm the allocator and the code objects on a few problems
+    k = 8
+    sub = [i for i, (pi, _) in enumerate(owner) if pi < k]
+    batch.certify_planned(scan[:k], orders[:k], [flat[i] for i in sub], [owner[i] for i in sub])
+    import torch
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    recs = batch.certify_planned(scan, orders, flat, owner)
+    torch.cuda.synchronize()
+    batched_s = time.perf_counter() - t1
+    ok = all(r["witness_ok"] for r in recs if r["distance"] is not None)
+    # plain GPU: the first sample problems one at a time with per-op kernels
+    ns = min(args.plain_sample, len(scan))
+    sub = [i for i, (pi, _) in enumerate(owner) if pi < ns]
+    t2 = time.perf_counter()
+    plain = batch.certify_planned_one_at_a_time(scan[:ns], orders[:ns], [flat[i] for i in sub],
+                                                [owner[i] for i in sub])
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t2
+    same = all(a["per_observable"] == b["per_observable"] and b["witness_ok"] for a, b in zip(recs[:ns], plain))
+    widths = sorted(set(pl.peak_bits for pl in flat if pl.feasible))
+    return {"workload": "%d banded scans, 200-1200 detectors, 5-15 faults/detector, width 10-22, 2 logical rows, "
+                        "half with costs 1-3" % len(scan),
+            "certs_per_s": len(scan) / batched_s, "batched_seconds": batched_s,
+            "plain_gpu_certs_per_s": ns / plain_s, "plain_sample": ns, "plain_seconds": plain_s,
+            "speedup_vs_plain": (len(scan) / batched_s) / (ns / plain_s), "witnesses_ok": ok,
+            "sample_identical_to_plain": same, "states": int(sum(pl.work() for pl in flat if pl.feasible)),
+            "states_per_s": sum(pl.work() for pl in flat if pl.feasible) / batched_s,
+            "host_planning_seconds": round(plan_s, 3), "peak_bits_range": [widths[0], widths[-1]],
+            "latency_s": batched_s}
+
+
+def bench_sparse(args):
+    from tropidist_amd import ordering, sparse_dp, workloads
+    p = workloads.sparse_qc()
+    order = ordering.greedy_straddle_order(p.h, p.nrows)
+    tr = sparse_dp.Trellis.for_query(p, 0, order)
+    sparse_dp.query(tr, 2, mode="mitm", use_h=False, backend="hip")            # warm-up
+    t0 = time.perf_counter()
+    r = sparse_dp.query(tr, args.sparse_K, mode="mitm", use_h=False, backend="hip")
+    dt = time.perf_counter() - t0
+    steps = sum(r["forward_counts"]) + sum(r["backward_counts"])
+    return {"workload": "two-block QC code l=m=12 (288 columns), seed 3000, observable 0, greedy order, "
+                        "MITM split K//2+1", "K": args.sparse_K, "value": r["value"], "width": max(tr.width),
+            "slots": tr.nslots, "state_steps": int(steps), "state_steps_per_s": steps / dt, "seconds": dt,
+            "peak_states": r["peak_states"], "witness_cols": len(r["witness"] or [])}
+
+
+def bench_serving(args, requests):
+    from tropidist_amd import serve
+    srv = serve.Server(memory_bytes=64 << 30, dense_lanes=3, sparse_lanes=4)
+    reqs = [("dense", (p,)) if kind == "dense" else ("sparse", (p, K)) for kind, p, K in requests]
+    tickets, wall = serve.run_poisson(srv, reqs, rate=args.serving_rate, seed=7)
+    srv.close()
+    lat = [tk.latency for tk in tickets]
+    errors = sum(tk.error is not None for tk in tickets)
+    ok = all(tk.result["witness_ok"] for tk in tickets if tk.error is None and tk.result["distance"] is not None)
+    dl = [tk.latency for tk in tickets if tk.kind == "dense"]
+    sl = [tk.latency for tk in tickets if tk.kind == "sparse"]
+    return {"workload": "Poisson stream of %d requests at %.0f/s: 70%% dense width 10-20, 20%% width 21-25, 10%% "
+                        "sparse MITM K<=%d on QC codes l,m 6-15" % (len(reqs), args.serving_rate, args.serving_max_K),
+            "certs_per_s": len(reqs) / wall, "wall_seconds": wall, "p50_s": serve.percentile(lat, 50),
+            "p99_s": serve.percentile(lat, 99), "dense_p50_s": serve.percentile(dl, 50),
+            "dense_p99_s": serve.percentile(dl, 99), "sparse_p50_s": serve.percentile(sl, 50),
+            "sparse_p99_s": serve.percentile(sl, 99), "errors": errors, "witnesses_ok": ok}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--roofline-launches", type=int, default=60)
+    ap.add_argument("--skip-batched", action="store_true")
+    ap.add_argument("--skip-sparse", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true")
+    ap.add_argument("--batch-instances", type=int, default=512)
+    ap.add_argument("--plain-sample", type=int, default=24)
+    ap.add_argument("--sparse-K", type=int, default=8)
+    ap.add_argument("--serving-requests", type=int, default=600)
+    ap.add_argument("--serving-rate", type=float, default=100.0)
+    ap.add_argument("--serving-max-K", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the sharding design is written up, "
+              "not built); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    inputs = generate(args)
+    from tropidist_amd import _lib
+    torch = _lib.torch()
+    L = _lib.lib()
+    p, plans, values, dense = bench_dense(args, L, torch)
+    out = {
+        "metric": "Rank-DP states/s (dense trellis, sum of Plan.work / s)",
+        "value": dense["value"], "unit": "states/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dense["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8", "data": "synthetic (seeded banded detector error models, quasi-cyclic codes)",
+        "config": dense["config"], "roofline": dense["roofline"],
+    }
+    if not args.skip_batched:
+        out["batched"] = bench_batched(args, inputs["scan"])
+    if not args.skip_sparse:
+        out["sparse"] = bench_sparse(args)
+    if not args.skip_serving:
+        out["serving"] = bench_serving(args, inputs["requests"])
     if not args.no_cpu:
         import oracle
         if oracle.have_ref():
@@ -127,7 +247,7 @@ def main():
             cpu_s = time.perf_counter() - c0
             out["cpu_baseline"] = {"value": plans[0].work() / cpu_s, "unit": "states/s", "cores": threads,
                                    "kind": "reference",
-                                   "sample": "observable 0 of the same instance (work %d states), the reference's "
+                                   "sample": "observable 0 of the same instance (work %d states): the reference's "
                                              "viterbi.c (OpenMP) driven op by op" % plans[0].work(),
                                    "seconds": cpu_s, "distance": cv, "matches_gpu": cv == values[0]}
     print(json.dumps(out))

@@ -82,15 +82,17 @@ def banded(R, N, kmin, kmax, W, nobs, rho, weights=None, seed=0, name=None):
     probability rho; weights: None (unit) or (lo, hi) for costs ~ U{lo..hi}.  Zero and duplicate columns are merged
     (least cost kept).  The natural column order has cut rank <= W."""
     rng = np.random.default_rng(seed)
-    h, l, w = [], [], []
-    for j in range(N):
-        s = int(round(j * (R - W) / max(1, N - 1)))
-        k = int(rng.integers(kmin, kmax + 1))
-        rows = rng.choice(W, size=min(k, W), replace=False) + s
-        h.append(sum(1 << int(r) for r in rows))
-        l.append(sum(1 << i for i in range(nobs) if rng.random() < rho))
-        w.append(1 if weights is None else int(rng.integers(weights[0], weights[1] + 1)))
-    hh, ll, ww = dedupe(h, l, w)
+    starts = np.rint(np.arange(N) * (R - W) / max(1, N - 1)).astype(np.int64)
+    k = rng.integers(kmin, kmax + 1, size=N)
+    picks = np.argsort(rng.random((N, W)), axis=1)           # a random permutation of the window per column
+    chosen = np.zeros((N, W), dtype=bool)
+    np.put_along_axis(chosen, picks, np.arange(W)[None, :] < np.minimum(k, W)[:, None], axis=1)
+    lbits = rng.random((N, nobs)) < rho
+    wts = rng.integers(weights[0], weights[1] + 1, size=N) if weights is not None else np.ones(N, dtype=np.int64)
+    packed = np.packbits(chosen, axis=1, bitorder="little")
+    h = [int.from_bytes(packed[j].tobytes(), "little") << int(starts[j]) for j in range(N)]
+    l = (lbits.astype(np.int64) << np.arange(nobs)).sum(axis=1).tolist() if nobs else [0] * N
+    hh, ll, ww = dedupe(h, l, wts.tolist())
     if weights is None:
         ww = None
     return Problem(name or "banded_R%d_N%d_W%d_s%d" % (R, N, W, seed), R, nobs, hh, ll, w=ww)

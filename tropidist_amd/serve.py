@@ -187,9 +187,10 @@ class Server:
         return [tk.latency for tk in self.done if tk.latency is not None and (kind is None or tk.kind == kind)]
 
 
-def sparse_certify(p, K, order=None):
+def sparse_certify(p, K, order=None, max_states=30_000_000):
     """Least weight <= K of a solution over all observables by the sparse MITM DP (auto.sparse_distance's loop,
-    cdist/auto.py:257-284, with the default split): {"distance", "witness", "observable", "witness_ok"}."""
+    cdist/auto.py:257-284, with the default split): {"distance", "witness", "observable", "witness_ok"}.  A table
+    beyond max_states raises sparse_dp.BudgetExceeded (the request is answered with that error)."""
     from .ordering import greedy_straddle_order
     from .problem import verify_witness
     order = order if order is not None else greedy_straddle_order(p.h, p.nrows)
@@ -201,7 +202,7 @@ def sparse_certify(p, K, order=None):
         k = K if best is None else best[0] - 1
         if k < 1:
             break
-        r = sparse_dp.query(tr, k, mode="mitm", use_h=False, witness=True, backend="hip")
+        r = sparse_dp.query(tr, k, mode="mitm", use_h=False, witness=True, backend="hip", max_states=max_states)
         if r["value"] is not None:
             best = (r["value"], r["witness"], i)
     if best is None:

@@ -47,7 +47,7 @@ def batch_scan(n=512, wmin=10, wmax=22, seed=4000):
 
 
 def serving_mix(n=10000, seed=5000, dense_lo=(10, 20), dense_hi=(21, 25), qc=(6, 15), sparse_frac=0.10,
-                wide_frac=0.20):
+                wide_frac=0.20, K=(4, 10)):
     """Request descriptors (kind, parameters) -- problems are built lazily by the serving loop."""
     rng = np.random.default_rng(seed)
     reqs = []
@@ -60,7 +60,7 @@ def serving_mix(n=10000, seed=5000, dense_lo=(10, 20), dense_hi=(21, 25), qc=(6,
                 l, m = int(rng.integers(qc[0], qc[1] + 1)), int(rng.integers(qc[0], qc[1] + 1))
                 if l % 3 == 0 or m % 3 == 0:
                     break
-            reqs.append(("sparse", {"l": l, "m": m, "seed": s, "K": int(rng.integers(4, 11))}))
+            reqs.append(("sparse", {"l": l, "m": m, "seed": s, "K": int(rng.integers(K[0], K[1] + 1))}))
         elif u < sparse_frac + wide_frac:
             reqs.append(("dense", {"width": int(rng.integers(dense_hi[0], dense_hi[1] + 1)), "seed": s,
                                    "weighted": bool(rng.random() < 0.5)}))
