@@ -171,13 +171,15 @@ int tdp_tab_lookup_batch(void *table, const uint64_t *d_zh, const uint64_t *d_zl
                          const int *d_R, uint64_t m, int *d_out, void *stream);
 
 /* ------------------------------------------------------------------ batched dense plans
- * d_items: nitems device records of tdp_batch_item_bytes() bytes (layout in tropidist_amd/csrc/tdp_batch.hip:
- * op stream, peak bits, checkpoint interval, table / checkpoint / decision-map / output pointers); each
- * workgroup of one persistent launch takes the next record from the queue counter d_next and runs that plan alone,
- * with its witness when asked (checkpointed traceback with one decision bit per state and update).  No
- * reference counterpart: the reference runs plans one at a time (cdist/auto.py:105-123); this serves the batched
- * scans and the serving loop. */
-int tdp_batch_run(void *d_items, int nitems, int *d_next, int workgroups, void *stream);
+ * d_items: device records of tdp_batch_item_bytes() bytes (layout in tropidist_amd/csrc/tdp_batch.hip: op stream,
+ * peak bits, checkpoint interval, table / checkpoint / decision-map / result / barrier pointers, group size);
+ * d_tickets: ntickets device pairs {item, member}, the `group` members of an item consecutive.  Each workgroup of
+ * one persistent launch takes the next ticket from the zeroed counter d_next and runs its share of that plan, the
+ * members of a plan meeting at a group barrier between ops; with the witness when asked (checkpointed traceback,
+ * one decision bit per state and update).  Result records and barrier words must be zeroed by the caller;
+ * `workgroups` (resident at once) must be >= the largest group.  No reference counterpart: the reference runs
+ * plans one at a time (cdist/auto.py:105-123); this serves the batched scans and the serving loop. */
+int tdp_batch_run(void *d_items, const void *d_tickets, int ntickets, int *d_next, int workgroups, void *stream);
 int tdp_batch_item_bytes(void);
 
 /* ------------------------------------------------------------------ host planner

@@ -11,8 +11,10 @@ The per-plan checkpoint interval S and the number of resident workgroups come fr
 tuned value for the plan's width bucket, or sqrt(8 * ops) and 2 workgroups per CU by default)."""
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import math
+import threading
 import time
 
 import numpy as np
@@ -24,7 +26,18 @@ ALIGN = 256
 
 ITEM_DTYPE = np.dtype([("ops", "<u8"), ("nops", "<i8"), ("peak_bits", "<i4"), ("S", "<i4"),
                        ("tab0", "<u8"), ("tab1", "<u8"), ("ckpt", "<u8"), ("ck_n", "<u8"), ("bits", "<u8"),
-                       ("used", "<u8"), ("result", "<u8"), ("witness", "<i4"), ("pad", "<i4")])
+                       ("used", "<u8"), ("result", "<u8"), ("barrier", "<u8"), ("witness", "<i4"),
+                       ("group", "<i4")])
+TICKET_DTYPE = np.dtype([("item", "<i4"), ("member", "<i4")])
+# Launches whose plans span several workgroups wait at group barriers, which needs every member resident; one such
+# launch runs at a time in the process (launches of single-workgroup plans never wait and run concurrently).
+_GROUP_LOCK = threading.Lock()
+RESULT_WORDS = 6
+
+
+def default_group(peak_bits):
+    """Workgroups sharing one plan: one up to 2^16 entries, doubling per bit above, at most 32."""
+    return int(min(32, 1 << max(0, peak_bits - 16)))
 
 
 def _up(x):
@@ -39,7 +52,7 @@ def plan_bytes(plan, witness, S=None):
     """Device bytes of one batch item (tables, ops, and for witnesses checkpoints, decision maps, used list)."""
     cap = 1 << plan.peak_bits
     nops = len(plan.ops)
-    b = 2 * _up(cap) + _up(24 * max(1, nops)) + 64
+    b = 2 * _up(cap) + _up(24 * max(1, nops))
     if witness:
         S = S or default_interval(nops)
         nseg = (nops + S - 1) // S
@@ -47,9 +60,9 @@ def plan_bytes(plan, witness, S=None):
     return b
 
 
-def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None):
+def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None, groups=None):
     """Run plans in one persistent launch.  Returns a list of (value or None, used op indices or None).  value 255
-    means a cost >= 255 (callers apply final_value)."""
+    means a cost >= 255 (callers apply final_value).  groups(peak_bits) -> workgroups per plan (default_group)."""
     L = _lib.lib()
     assert ITEM_DTYPE.itemsize == L.tdp_batch_item_bytes(), "BatchItem layout mismatch"
     t = _lib.torch()
@@ -75,8 +88,9 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
     r0 = base - arena.data_ptr()
     arena[r0:r0 + off].copy_(t.from_numpy(host))
     off += base
-    res_base = off                                   # results of all items, contiguous
-    off += _up(24 * len(order))
+    zeroed = t.zeros(len(order) * (8 * RESULT_WORDS + 8), dtype=t.uint8, device="cuda")
+    res_base = zeroed.data_ptr()                     # results, then barrier words, of all items (zeroed)
+    bar_base = res_base + 8 * RESULT_WORDS * len(order)
     used_base = off                                  # used-op lists of all items, contiguous
     used_off = []
     if witness:
@@ -95,7 +109,8 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
         off += _up(cap)
         it["tab1"] = off
         off += _up(cap)
-        it["result"] = res_base + 24 * slot
+        it["result"] = res_base + 8 * RESULT_WORDS * slot
+        it["barrier"] = bar_base + 8 * slot
         it["witness"] = int(witness)
         if witness:
             nseg = (nops + S[i] - 1) // S[i]
@@ -107,24 +122,33 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
             off += S[i] * _up(max(4, cap // 8))
             it["used"] = used_base + used_off[slot]
     assert off <= arena.data_ptr() + arena.numel()
-    d_items = t.from_numpy(items.view(np.uint8)).cuda()
-    d_next = t.zeros(1, dtype=t.int32, device="cuda")
     if workgroups is None:
         cus = ctypes.c_int(0)
         _lib.check(L.tdp_num_cus(ctypes.byref(cus)), "tdp_num_cus")
-        workgroups = 2 * cus.value
-    workgroups = max(1, min(workgroups, len(idx)))
-    _lib.check(L.tdp_batch_run(d_items.data_ptr(), len(idx), d_next.data_ptr(), workgroups,
-                               _lib.stream_handle(stream)), "tdp_batch_run")
-    (stream if stream is not None else t.cuda.current_stream()).synchronize()
-    a0 = res_base - arena.data_ptr()
-    res = arena[a0:a0 + 24 * len(order)].cpu().numpy().view(np.int64).reshape(-1, 3)
+        workgroups = cus.value                       # one 1024-thread workgroup per CU, all resident
+    gfun = groups or default_group
+    tickets = []
+    for slot, i in enumerate(order):
+        g = max(1, min(int(gfun(plans[i].peak_bits)), workgroups))
+        items[slot]["group"] = g
+        tickets += [(slot, m) for m in range(g)]
+    tick = np.array(tickets, dtype=TICKET_DTYPE)
+    workgroups = max(1, min(workgroups, len(tick)))
+    d_items = t.from_numpy(items.view(np.uint8)).cuda()
+    d_tick = t.from_numpy(tick.view(np.uint8)).cuda()
+    d_next = t.zeros(1, dtype=t.int32, device="cuda")
+    grouped = len(tick) > len(order)
+    with (_GROUP_LOCK if grouped else contextlib.nullcontext()):
+        _lib.check(L.tdp_batch_run(d_items.data_ptr(), d_tick.data_ptr(), len(tick), d_next.data_ptr(), workgroups,
+                                   _lib.stream_handle(stream)), "tdp_batch_run")
+        (stream if stream is not None else t.cuda.current_stream()).synchronize()
+    res = zeroed[:8 * RESULT_WORDS * len(order)].cpu().numpy().view(np.int64).reshape(-1, RESULT_WORDS)
     used_all = None
     if witness:
         u0 = used_base - arena.data_ptr()
         used_all = arena[u0:u0 + used_off[-1] + 8 * max(1, len(plans[order[-1]].ops))].cpu().numpy().view(np.int64)
     for slot, i in enumerate(order):
-        value, nused, err = (int(x) for x in res[slot])
+        value, nused, err = (int(x) for x in res[slot][:3])
         if err:
             raise RuntimeError("batched plan %d failed (code %d)" % (i, err))
         used = None
