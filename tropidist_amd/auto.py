@@ -98,25 +98,20 @@ def witness_plan(plan, order, max_bytes=None, stream=None):
     workspace would exceed max_bytes.  OverflowError at 255."""
     L = _lib.lib()
     nbytes = ctypes.c_uint64(0)
-    _lib.check(L.tdp_witness_workspace_bytes(len(plan.ops), plan.peak_bits, ctypes.byref(nbytes)), "workspace")
+    ops = plan.encode()
+    _lib.check(L.tdp_witness_workspace_bytes(len(ops), plan.peak_bits, ctypes.byref(nbytes)), "workspace")
     if max_bytes is not None and nbytes.value > max_bytes:
         return None
     work = _lib.device_empty(nbytes.value)
-    ops = plan.encode()
-    used = np.zeros(max(1, len(plan.ops)), dtype=np.int64)
+    used = np.zeros(max(1, len(ops)), dtype=np.int64)
     value, nused = ctypes.c_int(-1), ctypes.c_int64(0)
-    rc = L.tdp_run_plan_witness(ops.ctypes.data_as(ctypes.c_void_p), len(plan.ops), plan.peak_bits,
+    rc = L.tdp_run_plan_witness(ops.ctypes.data_as(ctypes.c_void_p), len(ops), plan.peak_bits,
                                 _lib.dptr(work), nbytes.value, ctypes.byref(value),
                                 used.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nused), _lib.stream_handle(stream))
     _lib.check(rc, "tdp_run_plan_witness")
     v = kernel.final_value(value.value)
-    update_col = {}
-    k = 0
-    for idx, op in enumerate(plan.ops):
-        if op[0] == "update":
-            update_col[idx] = plan.columns[k]
-            k += 1
-    cols = sorted(order[update_col[int(i)]] for i in used[:nused.value])
+    ordinal = np.cumsum(ops[:, 0] == 1) - 1
+    cols = sorted(order[plan.columns[int(ordinal[int(i)])]] for i in used[:nused.value])
     return v, cols
 
 

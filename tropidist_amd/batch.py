@@ -51,7 +51,7 @@ def default_interval(nops):
 def plan_bytes(plan, witness, S=None):
     """Device bytes of one batch item (tables, ops, and for witnesses checkpoints, decision maps, used list)."""
     cap = 1 << plan.peak_bits
-    nops = len(plan.ops)
+    nops = plan.nops
     b = 2 * _up(cap) + _up(24 * max(1, nops))
     if witness:
         S = S or default_interval(nops)
@@ -70,7 +70,7 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
     out = [(None, None)] * len(plans)
     if not idx:
         return out
-    S = {i: (intervals[i] if intervals else default_interval(len(plans[i].ops))) for i in idx}
+    S = {i: (intervals[i] if intervals else default_interval(plans[i].nops)) for i in idx}
     sizes = {i: plan_bytes(plans[i], witness, S[i]) for i in idx}
     arena = _lib.device_empty(sum(sizes.values()) + ALIGN)
     base = _up(arena.data_ptr())
@@ -96,12 +96,12 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
     if witness:
         for i in order:
             used_off.append(off - used_base)
-            off += 8 * max(1, len(plans[i].ops))
+            off += 8 * max(1, plans[i].nops)
         off = _up(off)
     for slot, i in enumerate(order):
         p = plans[i]
         cap = 1 << p.peak_bits
-        nops = len(p.ops)
+        nops = p.nops
         it = items[slot]
         it["ops"] = base + ops_off[slot]
         it["nops"], it["peak_bits"], it["S"] = nops, p.peak_bits, S[i]
@@ -146,7 +146,7 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
     used_all = None
     if witness:
         u0 = used_base - arena.data_ptr()
-        used_all = arena[u0:u0 + used_off[-1] + 8 * max(1, len(plans[order[-1]].ops))].cpu().numpy().view(np.int64)
+        used_all = arena[u0:u0 + used_off[-1] + 8 * max(1, plans[order[-1]].nops)].cpu().numpy().view(np.int64)
     for slot, i in enumerate(order):
         value, nused, err = (int(x) for x in res[slot][:3])
         if err:
@@ -160,12 +160,10 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
 
 
 def _columns(plan, order, used):
-    upd, k = {}, 0
-    for idx, op in enumerate(plan.ops):
-        if op[0] == "update":
-            upd[idx] = plan.columns[k]
-            k += 1
-    return sorted(order[upd[int(i)]] for i in used)
+    """Problem columns of the used update ops (op indices -> update ordinal -> column position -> column)."""
+    codes = plan.encode()[:, 0]
+    ordinal = np.cumsum(codes == 1) - 1
+    return sorted(order[plan.columns[int(ordinal[int(i)])]] for i in used)
 
 
 def plan_problems(problems, orders=None):

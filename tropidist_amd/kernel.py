@@ -37,12 +37,12 @@ def run_plan(plan, log=None, progress_every=0.0, stream=None):
     ops = plan.encode()
     value = ctypes.c_int(-1)
     t0 = time.time()
-    rc = L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(plan.ops), plan.peak_bits, _lib.dptr(T0),
+    rc = L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(ops), plan.peak_bits, _lib.dptr(T0),
                         _lib.dptr(T1), ctypes.byref(value), _lib.stream_handle(stream))
     _lib.check(rc, "tdp_run_plan")
     seconds = time.time() - t0
     if log:
-        log("  plan of %d ops on 2^%d entries: %.4f s" % (len(plan.ops), plan.peak_bits, seconds))
+        log("  plan of %d ops on 2^%d entries: %.4f s" % (len(ops), plan.peak_bits, seconds))
     cus = ctypes.c_int(0)
     _lib.check(L.tdp_num_cus(ctypes.byref(cus)), "tdp_num_cus")
     return final_value(value.value), {"seconds": seconds, "peak_bits": plan.peak_bits, "cus": cus.value,

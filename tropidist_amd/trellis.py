@@ -156,6 +156,10 @@ class Plan:
     def max_weight(self):
         return max((op[2] if len(op) > 2 else 1 for op in self.ops if op[0] == "update"), default=1)
 
+    @property
+    def nops(self):
+        return len(self.ops)
+
 
 def make_plan(hrows, lrow, ncols, weights=None):
     """DP schedule for constraint rows (target 0) and one target-1 row (cdist/trellis.py:147-182); weights[j] is the
@@ -214,19 +218,26 @@ class NativePlan(Plan):
     def __init__(self, encoded, peak_bits, ncols, work, columns, feasible=True):
         self._encoded = encoded
         self._work = int(work)
+        self._decoded = None
         Plan.__init__(self, None, peak_bits, [], ncols, feasible, list(columns))
 
     @property
+    def nops(self):
+        return len(self._encoded)
+
+    @property
     def ops(self):
-        out = []
-        for code, a, b in self._encoded.tolist():
-            if code == OP_EXPAND:
-                out.append(("expand",))
-            elif code == OP_UPDATE:
-                out.append(("update", a) if b == 1 else ("update", a, b))
-            else:
-                out.append(("select", a, b))
-        return out
+        if self._decoded is None:
+            out = []
+            for code, a, b in self._encoded.tolist():
+                if code == OP_EXPAND:
+                    out.append(("expand",))
+                elif code == OP_UPDATE:
+                    out.append(("update", a) if b == 1 else ("update", a, b))
+                else:
+                    out.append(("select", a, b))
+            self._decoded = out
+        return self._decoded
 
     @ops.setter
     def ops(self, value):
