@@ -36,8 +36,9 @@ RESULT_WORDS = 6
 
 
 def default_group(peak_bits):
-    """Workgroups sharing one plan: one up to 2^16 entries, doubling per bit above, at most 32."""
-    return int(min(32, 1 << max(0, peak_bits - 16)))
+    """Workgroups sharing one plan: the autotuner's choice for the width bucket (tropidist_amd.autotune)."""
+    from . import autotune
+    return autotune.group_for(peak_bits)
 
 
 def _up(x):
@@ -70,7 +71,11 @@ def run_plans(plans, witness=False, intervals=None, workgroups=None, stream=None
     out = [(None, None)] * len(plans)
     if not idx:
         return out
-    S = {i: (intervals[i] if intervals else default_interval(plans[i].nops)) for i in idx}
+    if intervals is None:
+        from . import autotune
+        S = {i: autotune.interval_for(plans[i]) for i in idx}
+    else:
+        S = {i: intervals[i] for i in idx}
     sizes = {i: plan_bytes(plans[i], witness, S[i]) for i in idx}
     arena = _lib.device_empty(sum(sizes.values()) + ALIGN)
     base = _up(arena.data_ptr())
