@@ -72,6 +72,17 @@ int tdp_hist(const uint8_t *d_T, uint64_t n, uint64_t *d_hist, void *stream);
 int tdp_run_plan(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value,
                  void *stream);
 
+/* The same plan by fused passes: one kernel applies up to five update columns (with the expands and selects of
+ * their rows) to register-resident cubes of the table, so a pass reads 2^b_in and writes 2^b_out bytes for all
+ * of them.  segments: nseg triples of int64 (0, first, last) = ops [first, last) by the per-op kernels, or (1, k, 0)
+ * = pass descriptor k of `passes` (npasses records of tdp_pass_args_bytes() bytes, compiled on the host from the
+ * op stream).  The result equals tdp_run_plan's on the same op stream (the loop of run_plan,
+ * cdist/kernel.py:94-138, over the ops of cdist/viterbi.c).  TDP_EINVAL if a descriptor addresses outside its
+ * tables or does not match the table size the segments before it leave. */
+int tdp_pass_args_bytes(void);
+int tdp_run_program(const uint64_t *ops, int64_t nops, const int64_t *segments, int64_t nseg, const void *passes,
+                    int64_t npasses, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value, void *stream);
+
 /* Witness of a plan: the plan's value and the op indices of the updates on one minimum path, by a checkpointed
  * traceback from the final state 0 (replaces the numpy replay of dp_witness, cdist/auto.py:126-182, which keeps
  * every table).  d_work: caller workspace of tdp_witness_workspace_bytes(nops, peak_bits) bytes; used_ops: host

@@ -1,0 +1,126 @@
+"""GPU parity of the fused-pass executor (tdp_run_program): the value of every plan equals the oracle's, bit for bit,
+for every pass shape the compiler makes (1-5 directions, in-dword updates, dword and byte stores, passes from 2^2
+entries up), and equals the per-op executor at the full 2^25-entry workload."""
+import ctypes
+import random
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import dense_fixtures, random_problem
+from tropidist_amd import _lib, fused, kernel, problem, trellis
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dmax,min_bits", [(1, 2), (2, 3), (3, 2), (5, 2), (5, 4)])
+def test_random_plans(dmax, min_bits):
+    print("case fused random dmax", dmax, "min_bits", min_bits, flush=True)
+    rng = random.Random(40 + dmax + 7 * min_bits)
+    checked = 0
+    for t in range(120):
+        m = rng.randint(4, 12)
+        n = rng.randint(6, 30)
+        h = [rng.randrange(1, 1 << m) for _ in range(n)]
+        l = [rng.randrange(0, 2) for _ in range(n)]
+        w = [rng.randint(1, 3) for _ in range(n)] if t % 2 else None
+        order = list(range(n))
+        rng.shuffle(order)
+        plan = trellis.build_plan(h, l, 1, order, m, weights=w)
+        if not plan.feasible:
+            continue
+        prog = fused.compile_program(plan, dmax=dmax, min_bits=min_bits)
+        assert fused.run_program(plan, prog) == oracle.run_plan_numpy(plan), t
+        checked += 1
+    assert checked > 60
+
+
+def test_reference_random_problems_against_brute_force():
+    """The generator of the reference's tests/test_dp.py:26-33, seed 1: fused GPU value == brute force."""
+    rng = random.Random(1)
+    checked = 0
+    for t in range(300):
+        m, k, h, l = random_problem(rng)
+        order = list(range(len(h)))
+        rng.shuffle(order)
+        obs_mask = rng.randrange(1, 1 << k)
+        plan = trellis.build_plan(h, l, obs_mask, order, m)
+        if not plan.feasible:
+            continue
+        prog = fused.compile_program(plan, min_bits=2)
+        assert fused.run_program(plan, prog) == oracle.brute(h, l, obs_mask), t
+        checked += 1
+    assert checked > 100
+
+
+def test_recorded_distances():
+    """Distances of the reference's recorded audit runs, tables up to 2^22 entries, default compilation."""
+    checked = 0
+    for r in dense_fixtures():
+        if r["distances"] is None or r["peak_bits"] > 22:
+            continue
+        for p, d in zip(r["plans"], r["distances"]):
+            print("case fused fixture", r["name"], p["observable"], flush=True)
+            plan = trellis.build_plan(r["h"], r["l"], 1 << p["observable"], r["order"], r["nrows"])
+            assert fused.run_program(plan) == d, (r["name"], p["observable"])
+            checked += 1
+    assert checked >= 10
+
+
+@pytest.mark.parametrize("W,weights", [(12, None), (16, (1, 3)), (20, (1, 2))])
+def test_banded_against_oracle(W, weights):
+    print("case fused banded W", W, flush=True)
+    p = problem.banded(R=70, N=400, kmin=2, kmax=6, W=W, nobs=1, rho=0.1, weights=weights, seed=W)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows, weights=p.w)
+    assert plan.feasible
+    want = oracle.run_plan_c(plan, "oracle")
+    assert fused.run_program(plan) == want
+    assert kernel.run_plan(plan)[0] == want
+
+
+def test_full_size_equals_per_op():
+    """2^25-entry banded plan (the headline workload's size): fused and per-op executors agree."""
+    print("case fused full size W 25", flush=True)
+    p = problem.banded(R=120, N=600, kmin=2, kmax=6, W=25, nobs=1, rho=0.1, seed=1)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+    assert plan.feasible and plan.peak_bits >= 24
+    prog, _ = fused.program_for(plan)
+    assert max(q.b_in for q in prog.passes) >= 24
+    assert fused.run_program(plan) == kernel.run_plan(plan)[0]
+
+
+def test_weighted_overflow():
+    """Every column of weight 200: any logical costs at least 400, saturates at 255 and raises OverflowError, as
+    the per-op path does."""
+    p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=5)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows, weights=[200] * p.ncols)
+    with pytest.raises(OverflowError):
+        oracle.run_plan_numpy(plan)
+    prog = fused.compile_program(plan, min_bits=2)
+    assert prog.passes and fused.simulate(prog, plan) == 255
+    with pytest.raises(OverflowError):
+        fused.run_program(plan, prog)
+
+
+def test_bad_descriptor_rejected_on_host():
+    """A descriptor whose addresses leave its table is refused before any pass kernel is launched."""
+    p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=5)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+    prog = fused.compile_program(plan, min_bits=4)
+    segs, arr = fused.pack(prog)
+    arr = arr.copy()
+    assert arr[0]["nfree"] >= 1
+    arr[0]["nruns"][0] = 1
+    arr[0]["run_f0"][0, 0], arr[0]["run_len"][0, 0], arr[0]["run_sh"][0, 0] = 0, 1, 38
+    L = _lib.lib()
+    ops = plan.encode()
+    T0 = _lib.device_empty(1 << plan.peak_bits)
+    T1 = _lib.device_empty(1 << plan.peak_bits)
+    v = ctypes.c_int(-1)
+    rc = L.tdp_run_program(ops.ctypes.data_as(ctypes.c_void_p), len(ops), segs.ctypes.data_as(ctypes.c_void_p),
+                           len(segs), arr.ctypes.data_as(ctypes.c_void_p), len(arr), plan.peak_bits,
+                           _lib.dptr(T0), _lib.dptr(T1), ctypes.byref(v), _lib.stream_handle())
+    assert rc == 1    # TDP_EINVAL
+    assert b"outside" in L.tdp_last_error()
+    _lib.torch().cuda.synchronize()

@@ -27,6 +27,8 @@ SIGNATURES = {
     "tdp_fill": (ci, [vp, u64, u8, vp]),
     "tdp_hist": (ci, [vp, u64, vp, vp]),
     "tdp_run_plan": (ci, [vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
+    "tdp_pass_args_bytes": (ci, []),
+    "tdp_run_program": (ci, [vp, i64, vp, i64, vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
     "tdp_witness_workspace_bytes": (ci, [i64, ci, ctypes.POINTER(u64)]),
     "tdp_run_plan_witness": (ci, [vp, i64, ci, vp, u64, ctypes.POINTER(ci), vp, ctypes.POINTER(i64), vp]),
     "tdp_tree_join": (ci, [ci, vp, vp, vp, vp, vp, ctypes.c_uint, ctypes.c_uint, u64, u64, u64, vp]),

@@ -103,5 +103,7 @@ int launch_expand(uint8_t *T, uint64_t n, uint64_t hl, uint32_t w, hipStream_t s
 int launch_fill(uint8_t *T, uint64_t n, uint8_t v, hipStream_t st);
 int launch_select(uint8_t *dst, const uint8_t *src, uint64_t n, int pos, int target, uint64_t *n_out,
                   hipStream_t st);
+int run_ops_range(const uint64_t *ops, int64_t first, int64_t last, uint8_t *&cur, uint8_t *&other, uint64_t &n,
+                  int peak_bits, hipStream_t st);
 
 }  // namespace tdp

@@ -213,6 +213,47 @@ static bool pow2(uint64_t n) { return n && !(n & (n - 1)); }
 
 using namespace tdp;
 
+// Ops [first, last) of an encoded stream on the table cur (n entries, at most 2^peak_bits), selects writing to
+// other and swapping the two; an expand directly followed by an update of its new row is one kernel.
+int tdp::run_ops_range(const uint64_t *ops, int64_t first, int64_t last, uint8_t *&cur, uint8_t *&other,
+                       uint64_t &n, int peak_bits, hipStream_t st) {
+    const uint64_t cap = 1ull << peak_bits;
+    int rc = TDP_OK;
+    for (int64_t i = first; i < last; i++) {
+        uint64_t code = ops[3 * i], a = ops[3 * i + 1], b = ops[3 * i + 2];
+        if (code == 0) {
+            TDP_REQUIRE(2 * n <= cap, "op %lld: expand beyond 2^%d entries", (long long)i, peak_bits);
+            const uint64_t *nx = ops + 3 * (i + 1);
+            if (i + 1 < last && nx[0] == 1 && (nx[1] >> top_bit(n)) == 1) {
+                TDP_REQUIRE(nx[2] >= 1, "op %lld: weight < 1", (long long)(i + 1));
+                rc = launch_expand(cur, n, nx[1] ^ n, (uint32_t)(nx[2] > 255 ? 255 : nx[2]), st);
+                i++;
+            } else {
+                rc = launch_fill(cur + n, n, 255, st);
+            }
+            n <<= 1;
+        } else if (code == 1) {
+            TDP_REQUIRE(a < n, "op %lld: update mask 0x%llx outside 2^%d entries", (long long)i,
+                        (unsigned long long)a, top_bit(n));
+            TDP_REQUIRE(b >= 1, "op %lld: weight < 1", (long long)i);
+            rc = launch_update(cur, n, a, (uint32_t)(b > 255 ? 255 : b), st);
+        } else if (code == 2) {
+            TDP_REQUIRE(n >= 2 && a < (uint64_t)top_bit(n) && b <= 1, "op %lld: select pos %llu on 2^%d entries",
+                        (long long)i, (unsigned long long)a, top_bit(n));
+            uint64_t nn = 0;
+            rc = launch_select(other, cur, n, (int)a, (int)b, &nn, st);
+            uint8_t *t = cur;
+            cur = other;
+            other = t;
+            n = nn;
+        } else {
+            TDP_REQUIRE(false, "op %lld: unknown code %llu", (long long)i, (unsigned long long)code);
+        }
+        if (rc) return rc;
+    }
+    return TDP_OK;
+}
+
 extern "C" {
 
 const char *tdp_last_error(void) { return g_err; }
@@ -269,43 +310,12 @@ int tdp_run_plan(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_T0
     hipStream_t st = as_stream(stream);
     TDP_REQUIRE(peak_bits >= 0 && peak_bits <= 40, "peak_bits %d out of range", peak_bits);
     TDP_REQUIRE(nops >= 0 && (nops == 0 || ops), "bad op stream");
-    const uint64_t cap = 1ull << peak_bits;
     uint8_t *cur = d_T0, *other = d_T1;
     uint64_t n = 1;
     int rc = launch_fill(cur, 1, 0, st);
     if (rc) return rc;
-    for (int64_t i = 0; i < nops; i++) {
-        uint64_t code = ops[3 * i], a = ops[3 * i + 1], b = ops[3 * i + 2];
-        if (code == 0) {
-            TDP_REQUIRE(2 * n <= cap, "op %lld: expand beyond 2^%d entries", (long long)i, peak_bits);
-            const uint64_t *nx = ops + 3 * (i + 1);
-            if (i + 1 < nops && nx[0] == 1 && (nx[1] >> top_bit(n)) == 1) {
-                TDP_REQUIRE(nx[2] >= 1, "op %lld: weight < 1", (long long)(i + 1));
-                rc = launch_expand(cur, n, nx[1] ^ n, (uint32_t)(nx[2] > 255 ? 255 : nx[2]), st);
-                i++;
-            } else {
-                rc = launch_fill(cur + n, n, 255, st);
-            }
-            n <<= 1;
-        } else if (code == 1) {
-            TDP_REQUIRE(a < n, "op %lld: update mask 0x%llx outside 2^%d entries", (long long)i,
-                        (unsigned long long)a, top_bit(n));
-            TDP_REQUIRE(b >= 1, "op %lld: weight < 1", (long long)i);
-            rc = launch_update(cur, n, a, (uint32_t)(b > 255 ? 255 : b), st);
-        } else if (code == 2) {
-            TDP_REQUIRE(n >= 2 && a < (uint64_t)top_bit(n) && b <= 1, "op %lld: select pos %llu on 2^%d entries",
-                        (long long)i, (unsigned long long)a, top_bit(n));
-            uint64_t nn = 0;
-            rc = launch_select(other, cur, n, (int)a, (int)b, &nn, st);
-            uint8_t *t = cur;
-            cur = other;
-            other = t;
-            n = nn;
-        } else {
-            TDP_REQUIRE(false, "op %lld: unknown code %llu", (long long)i, (unsigned long long)code);
-        }
-        if (rc) return rc;
-    }
+    rc = run_ops_range(ops, 0, nops, cur, other, n, peak_bits, st);
+    if (rc) return rc;
     TDP_REQUIRE(n == 1, "op stream ends with %llu entries, not 1", (unsigned long long)n);
     uint8_t v = 0;
     TDP_HIP_CHECK(hipMemcpyAsync(&v, cur, 1, hipMemcpyDeviceToHost, st));
