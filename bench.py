@@ -63,28 +63,75 @@ def generate(args):
     return out
 
 
-def bench_dense(args, L, torch):
-    import ctypes
-    from tropidist_amd import _lib, kernel, trellis, workloads
+def _time_launches(torch, stream, launch, count):
+    """Average duration of `count` back-to-back launches on `stream`, by HIP events recorded on that stream."""
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    launch()
+    ev0.record(stream)
+    for _ in range(count):
+        launch()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / 1e3 / count
+
+
+def dense_setup(L, torch):
+    """The headline instance, its plans and fused programs, and device tables (shared with the profiling mode)."""
+    from tropidist_amd import _lib, fused, trellis, workloads
     t0 = time.time()
     p = workloads.single_large()
     order = list(range(p.ncols))
     plans = [pl for pl in trellis.build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], order, p.nrows, p.w)
              if pl.feasible]
     plan_seconds = time.time() - t0
+    t0 = time.time()
+    progs = [fused.program_for(pl) for pl in plans]
+    compile_seconds = time.time() - t0
     peak = max(pl.peak_bits for pl in plans)
+    T0, T1 = _lib.device_empty(1 << peak), _lib.device_empty(1 << peak)
+    return p, plans, progs, peak, T0, T1, plan_seconds, compile_seconds
+
+
+def dominant_pass(progs):
+    """The pass with the most bytes moved in the first program (the widest table), and its packed descriptor."""
+    from tropidist_amd import fused
+    prog, (segs, arr) = progs[0]
+    k = max(range(len(prog.passes)), key=lambda i: prog.passes[i].bytes_moved)
+    return prog.passes[k], arr[k:k + 1].copy()
+
+
+def measured_traffic(kname, ps):
+    """HBM bytes per launch of the dominant pass from the committed counter summary, when it was measured on the
+    same kernel and pass geometry; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "pass_d5_banded_w25.json")
+    try:
+        with open(path) as f:
+            prof = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if kname + "(" not in prof.get("kernel", "") or prof.get("alg_bytes_per_launch") != ps.bytes_moved:
+        return None
+    return prof["traffic_bytes_per_launch"]
+
+
+def bench_dense(args, L, torch):
+    import ctypes
+    import numpy as np
+    from tropidist_amd import _lib, kernel
+    p, plans, progs, peak, T0, T1, plan_seconds, compile_seconds = dense_setup(L, torch)
     total_work = sum(pl.work() for pl in plans)
     encoded = [pl.encode() for pl in plans]
-    T0, T1 = _lib.device_empty(1 << peak), _lib.device_empty(1 << peak)
     stream = torch.cuda.current_stream()
     sh = _lib.stream_handle(stream)
 
     def step():
         vals = []
-        for pl, ops in zip(plans, encoded):
+        for pl, ops, (_, (segs, arr)) in zip(plans, encoded, progs):
             v = ctypes.c_int(-1)
-            _lib.check(L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(ops), pl.peak_bits, _lib.dptr(T0),
-                                      _lib.dptr(T1), ctypes.byref(v), sh), "tdp_run_plan")
+            _lib.check(L.tdp_run_program(ops.ctypes.data_as(ctypes.c_void_p), len(ops),
+                                         segs.ctypes.data_as(ctypes.c_void_p), len(segs),
+                                         arr.ctypes.data_as(ctypes.c_void_p), len(arr), pl.peak_bits,
+                                         _lib.dptr(T0), _lib.dptr(T1), ctypes.byref(v), sh), "tdp_run_program")
             vals.append(kernel.final_value(v.value))
         return vals
 
@@ -97,30 +144,53 @@ def bench_dense(args, L, torch):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
 
-    # roofline of the dominant kernel: the streaming butterfly update at the peak table size, average launch
-    # duration from HIP events on its stream; algorithmic bytes = read + write of every entry
+    # per-op executor on the same step, once, for comparison
+    t2 = time.perf_counter()
+    per_op_values = []
+    for pl, ops in zip(plans, encoded):
+        v = ctypes.c_int(-1)
+        _lib.check(L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(ops), pl.peak_bits, _lib.dptr(T0),
+                                  _lib.dptr(T1), ctypes.byref(v), sh), "tdp_run_plan")
+        per_op_values.append(kernel.final_value(v.value))
+    per_op_seconds = time.perf_counter() - t2
+
+    # roofline of the dominant kernel: the widest fused pass, replayed on its own input (T0 -> T1); algorithmic
+    # bytes = every input entry read once + every output entry written once
+    ps, desc = dominant_pass(progs)
+    kname = "tdp::k_pass<%d, %s, %s, unsigned int>" % (ps.D, "true" if ps.byte_store else "false",
+                                                     "true" if ps.runs()[2] else "false")
+    _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << ps.b_in, 7, sh), "fill")
+    pass_s = _time_launches(torch, stream, lambda: _lib.check(
+        L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "tdp_run_pass"),
+        args.roofline_launches)
+    pass_bytes = ps.bytes_moved
+    # the per-op streaming update at the peak table, for reference
     n = 1 << peak
-    masks = [int(m) for c, m, _ in encoded[0].tolist() if c == 1 and (n >> 1) <= m < n]
-    masks = (masks * (args.roofline_launches // max(1, len(masks)) + 1))[:args.roofline_launches]
-    _lib.check(L.tdp_fill(_lib.dptr(T0), n, 7, sh), "fill")
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    for m in masks:
-        _lib.check(L.tdp_update_w(_lib.dptr(T0), n, m, 2, sh), "update")
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    avg_launch = ev0.elapsed_time(ev1) / 1e3 / len(masks)
-    achieved = 2 * n / avg_launch
+    masks = [int(m) for c, m, _ in encoded[0].tolist() if c == 1 and (n >> 1) <= m < n][:1] or [n >> 1]
+    upd_s = _time_launches(torch, stream, lambda: _lib.check(L.tdp_update_w(_lib.dptr(T0), n, masks[0], 2, sh),
+                                                            "update"), args.roofline_launches)
     return p, plans, values, {
         "value": args.steps * total_work / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
         "config": {"workload": "single large dense Rank DP: banded R=600 N=6000 k=2-6 W=25, 4 logical rows "
                                "rho=0.05, costs 1-3, uint8 saturating, natural order, seed 2000",
                    "columns": p.ncols, "observables": len(plans), "peak_bits": peak, "work_per_step": total_work,
                    "distances": values, "distance": min(values), "host_planning_seconds": round(plan_seconds, 3),
-                   "executor": "per-op streaming kernels (tdp_run_plan)"},
-        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "tdp::k_update_hi",
-                     "launch_seconds": avg_launch, "alg_bytes_per_launch": 2 * n, "table_entries": n},
+                   "pass_compile_seconds": round(compile_seconds, 3),
+                   "passes_per_step": sum(len(pr.passes) for pr, _ in progs),
+                   "executor": "fused passes (tdp_run_program), per-op kernels below 2^12 entries",
+                   "per_op_executor": {"states_per_s": total_work / per_op_seconds,
+                                       "ms_per_step": 1e3 * per_op_seconds, "distances": per_op_values,
+                                       "identical": per_op_values == values}},
+        "roofline": {"bound": "hbm", "achieved": pass_bytes / pass_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": pass_bytes / pass_s / HBM_PEAK, "traffic": measured_traffic(kname, ps),
+                     "traffic_source": "profiles/pass_d5_banded_w25.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled "
+                                       "by a device copy of 2^26 bytes)",
+                     "kernel": kname,
+                     "launch_seconds": pass_s, "alg_bytes_per_launch": pass_bytes,
+                     "pass": {"b_in": ps.b_in, "b_out": ps.b_out, "updates": ps.D + len(ps.low_lam),
+                              "threads": 1 << ps.nfree},
+                     "per_op_update": {"kernel": "tdp::k_update_hi", "table_entries": n, "launch_seconds": upd_s,
+                                       "achieved": 2 * n / upd_s / 1e9, "frac": 2 * n / upd_s / HBM_PEAK}},
     }
 
 
@@ -197,6 +267,29 @@ def bench_serving(args, requests):
             "sparse_p99_s": serve.percentile(sl, 99), "errors": errors, "witnesses_ok": ok}
 
 
+def profile_pass(args):
+    """Replays for hardware counters: the dominant pass (T0 -> T1) and a torch device copy of 2^b_in bytes."""
+    import ctypes
+    from tropidist_amd import _lib
+    torch = _lib.torch()
+    L = _lib.lib()
+    p, plans, progs, peak, T0, T1, _, _ = dense_setup(L, torch)
+    ps, desc = dominant_pass(progs)
+    stream = torch.cuda.current_stream()
+    sh = _lib.stream_handle(stream)
+    _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << ps.b_in, 7, sh), "fill")
+    for _ in range(args.profile_pass):
+        _lib.check(L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "pass")
+    src = T0[:1 << ps.b_in]
+    dst = T1[:1 << ps.b_in]
+    for _ in range(args.profile_pass):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"pass": {"b_in": ps.b_in, "b_out": ps.b_out, "D": ps.D, "threads": 1 << ps.nfree,
+                               "alg_bytes": ps.bytes_moved}, "copy_bytes": 1 << ps.b_in,
+                      "launches": args.profile_pass}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,12 +306,18 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=600)
     ap.add_argument("--serving-rate", type=float, default=100.0)
     ap.add_argument("--serving-max-K", type=int, default=8)
+    ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
+                    help="only replay the dominant fused pass N times and a device copy of the same input size N "
+                         "times (the reference kernel for scaling hardware byte counters), then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the sharding design is written up, "
               "not built); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.profile_pass:
+        profile_pass(args)
+        return
     inputs = generate(args)
     from tropidist_amd import _lib
     torch = _lib.torch()

@@ -80,6 +80,8 @@ int tdp_run_plan(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_T0
  * cdist/kernel.py:94-138, over the ops of cdist/viterbi.c).  TDP_EINVAL if a descriptor addresses outside its
  * tables or does not match the table size the segments before it leave. */
 int tdp_pass_args_bytes(void);
+/* One pass: d_in holds 2^b_in entries, d_out receives 2^b_out (distinct buffers). */
+int tdp_run_pass(const uint8_t *d_in, uint8_t *d_out, const void *pass, void *stream);
 int tdp_run_program(const uint64_t *ops, int64_t nops, const int64_t *segments, int64_t nseg, const void *passes,
                     int64_t npasses, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value, void *stream);
 

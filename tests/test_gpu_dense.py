@@ -128,20 +128,22 @@ def test_chain_boundary_on_gpu():
     assert kernel.run_plan(trellis.build_plan(p.h, p.l, 1, [0, 1], p.nrows))[0] is None
 
 
-def test_recorded_distances_on_gpu():
+@pytest.mark.parametrize("executor", kernel.EXECUTORS)
+def test_recorded_distances_on_gpu(executor):
     """Every observable of the fixtures with tables up to 2^28 entries: the recorded distance."""
     for r in dense_fixtures():
         if r["distances"] is None or r["peak_bits"] > 28:
             continue
-        print("case fixture", r["name"], r["variant"], "peak", r["peak_bits"], flush=True)
+        print("case fixture", executor, r["name"], r["variant"], "peak", r["peak_bits"], flush=True)
         for p, d in zip(r["plans"], r["distances"]):
             plan = trellis.build_plan(r["h"], r["l"], 1 << p["observable"], r["order"], r["nrows"])
-            assert kernel.run_plan(plan)[0] == d, (r["name"], p["observable"])
+            assert kernel.run_plan(plan, executor=executor)[0] == d, (r["name"], p["observable"])
 
 
-def test_banded_width20_equals_oracle():
+@pytest.mark.parametrize("executor", kernel.EXECUTORS)
+def test_banded_width20_equals_oracle(executor):
     """A weighted banded model at width 20 (2^20-entry tables): GPU == oracle C on every observable."""
-    print("case banded W20", flush=True)
+    print("case banded W20", executor, flush=True)
     p = problem.banded(R=120, N=700, kmin=2, kmax=6, W=20, nobs=2, rho=0.05, weights=(1, 3), seed=2020)
     order = list(range(p.ncols))
     for obs in range(p.nobs):
@@ -149,4 +151,4 @@ def test_banded_width20_equals_oracle():
         if not plan.feasible:
             continue
         assert plan.peak_bits >= 18
-        assert kernel.run_plan(plan)[0] == oracle.run_plan_c(plan, "oracle")
+        assert kernel.run_plan(plan, executor=executor)[0] == oracle.run_plan_c(plan, "oracle")

@@ -76,7 +76,7 @@ def test_banded_against_oracle(W, weights):
     assert plan.feasible
     want = oracle.run_plan_c(plan, "oracle")
     assert fused.run_program(plan) == want
-    assert kernel.run_plan(plan)[0] == want
+    assert kernel.run_plan(plan, executor="per-op")[0] == want
 
 
 def test_full_size_equals_per_op():
@@ -87,7 +87,7 @@ def test_full_size_equals_per_op():
     assert plan.feasible and plan.peak_bits >= 24
     prog, _ = fused.program_for(plan)
     assert max(q.b_in for q in prog.passes) >= 24
-    assert fused.run_program(plan) == kernel.run_plan(plan)[0]
+    assert fused.run_program(plan) == kernel.run_plan(plan, executor="per-op")[0]
 
 
 def test_weighted_overflow():

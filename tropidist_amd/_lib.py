@@ -28,6 +28,7 @@ SIGNATURES = {
     "tdp_hist": (ci, [vp, u64, vp, vp]),
     "tdp_run_plan": (ci, [vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
     "tdp_pass_args_bytes": (ci, []),
+    "tdp_run_pass": (ci, [vp, vp, vp, vp]),
     "tdp_run_program": (ci, [vp, i64, vp, i64, vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
     "tdp_witness_workspace_bytes": (ci, [i64, ci, ctypes.POINTER(u64)]),
     "tdp_run_plan_witness": (ci, [vp, i64, ci, vp, u64, ctypes.POINTER(ci), vp, ctypes.POINTER(i64), vp]),

@@ -261,6 +261,11 @@ extern "C" {
 
 int tdp_pass_args_bytes(void) { return (int)sizeof(PassArgs); }
 
+int tdp_run_pass(const uint8_t *d_in, uint8_t *d_out, const void *pass, void *stream) {
+    TDP_REQUIRE(d_in && d_out && pass && d_in != d_out, "bad pass arguments");
+    return launch_pass(d_in, d_out, *static_cast<const PassArgs *>(pass), as_stream(stream));
+}
+
 int tdp_run_program(const uint64_t *ops, int64_t nops, const int64_t *segments, int64_t nseg, const void *passes,
                     int64_t npasses, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value, void *stream) {
     hipStream_t st = as_stream(stream);

@@ -522,8 +522,9 @@ def program_for(plan, dmax=DMAX, min_bits=12):
     return prog, packed
 
 
-def run_program(plan, program=None, T0=None, T1=None, stream=None):
-    """Min cost of a feasible plan by the fused-pass executor (tdp_run_program); OverflowError at 255."""
+def run_program(plan, program=None, T0=None, T1=None, stream=None, raw=False):
+    """Min cost of a feasible plan by the fused-pass executor (tdp_run_program); OverflowError at 255 (or the raw
+    final entry when raw)."""
     import ctypes
     from . import _lib, kernel
     L = _lib.lib()
@@ -543,4 +544,4 @@ def run_program(plan, program=None, T0=None, T1=None, stream=None):
                            len(segs), arr.ctypes.data_as(ctypes.c_void_p), len(arr), plan.peak_bits,
                            _lib.dptr(T0), _lib.dptr(T1), ctypes.byref(value), _lib.stream_handle(stream))
     _lib.check(rc, "tdp_run_program")
-    return kernel.final_value(value.value)
+    return value.value if raw else kernel.final_value(value.value)

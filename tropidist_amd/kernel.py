@@ -2,10 +2,13 @@
 
     INF, final_value(entry), run_plan(plan, log=None, progress_every=0.0) -> (value | None, stats)
 
-run_plan hands the whole op stream to libtdp's plan executor (tdp_run_plan), which streams every op over the uint8
-table in HBM on one HIP stream; control comes back to Python once per plan, not once per op as in the reference
-(cdist/kernel.py:112-135).  The reference's numpy executor run_plan_numpy is a CPU path and lives in the oracle
-(oracle/oracle.py) as a checker only."""
+run_plan hands the whole plan to libtdp on one HIP stream; control comes back to Python once per plan, not once per
+op as in the reference (cdist/kernel.py:112-135).  Two executors give identical values:
+    "fused"   (default) tdp_run_program: per-op kernels while the table is narrow, then passes of up to five
+              update columns per kernel (tropidist_amd/fused.py compiles them);
+    "per-op"  tdp_run_plan: one streaming kernel per op over the uint8 table in HBM.
+The reference's numpy executor run_plan_numpy is a CPU path and lives in the oracle (oracle/oracle.py) as a checker
+only."""
 from __future__ import annotations
 
 import ctypes
@@ -25,25 +28,35 @@ def final_value(entry):
     return v
 
 
-def run_plan(plan, log=None, progress_every=0.0, stream=None):
+EXECUTORS = ("fused", "per-op")
+
+
+def run_plan(plan, log=None, progress_every=0.0, stream=None, executor="fused"):
     """Run the plan on the GPU.  Returns (min cost or None if infeasible, stats); OverflowError if the cost is
     >= 255 (cdist/kernel.py:94-138).  stats: seconds, peak_bits, cus, executor."""
+    if executor not in EXECUTORS:
+        raise ValueError("executor must be one of %s" % (EXECUTORS,))
     if not plan.feasible:
         return None, {"feasible": False}
     L = _lib.lib()
     size = 1 << plan.peak_bits
     T0 = _lib.device_empty(size)
     T1 = _lib.device_empty(size)
-    ops = plan.encode()
-    value = ctypes.c_int(-1)
     t0 = time.time()
-    rc = L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(ops), plan.peak_bits, _lib.dptr(T0),
-                        _lib.dptr(T1), ctypes.byref(value), _lib.stream_handle(stream))
-    _lib.check(rc, "tdp_run_plan")
+    if executor == "fused":
+        from . import fused
+        raw = fused.run_program(plan, None, T0, T1, stream=stream, raw=True)
+    else:
+        ops = plan.encode()
+        value = ctypes.c_int(-1)
+        rc = L.tdp_run_plan(ops.ctypes.data_as(ctypes.c_void_p), len(ops), plan.peak_bits, _lib.dptr(T0),
+                            _lib.dptr(T1), ctypes.byref(value), _lib.stream_handle(stream))
+        _lib.check(rc, "tdp_run_plan")
+        raw = value.value
     seconds = time.time() - t0
     if log:
-        log("  plan of %d ops on 2^%d entries: %.4f s" % (len(ops), plan.peak_bits, seconds))
+        log("  plan of %d ops on 2^%d entries (%s): %.4f s" % (plan.nops, plan.peak_bits, executor, seconds))
     cus = ctypes.c_int(0)
     _lib.check(L.tdp_num_cus(ctypes.byref(cus)), "tdp_num_cus")
-    return final_value(value.value), {"seconds": seconds, "peak_bits": plan.peak_bits, "cus": cus.value,
-                                      "executor": "per-op"}
+    return final_value(raw), {"seconds": seconds, "peak_bits": plan.peak_bits, "cus": cus.value,
+                              "executor": executor}
