@@ -152,3 +152,22 @@ def test_banded_width20_equals_oracle(executor):
             continue
         assert plan.peak_bits >= 18
         assert kernel.run_plan(plan, executor=executor)[0] == oracle.run_plan_c(plan, "oracle")
+
+
+class _RefStylePlan:
+    """Only the attributes the reference's Plan has (cdist/trellis.py:125-144): ops as tuples, peak_bits, feasible."""
+
+    def __init__(self, plan):
+        self.ops = [tuple(op) for op in plan.ops]
+        self.peak_bits = plan.peak_bits
+        self.feasible = plan.feasible
+
+
+@pytest.mark.parametrize("executor", kernel.EXECUTORS)
+def test_run_plan_accepts_reference_style_plans(executor):
+    """The drop-in surface: a plan object built like the reference's runs unchanged through run_plan."""
+    p = problem.banded(R=80, N=500, kmin=2, kmax=5, W=14, nobs=1, rho=0.1, seed=77)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+    want = oracle.run_plan_c(plan, "oracle")
+    got, stats = kernel.run_plan(_RefStylePlan(plan), executor=executor)
+    assert got == want and stats["threads"] == stats["cus"] > 0

@@ -34,3 +34,17 @@ def test_recorded_fixtures_and_banded():
     p = workloads.single_cpu()
     order = list(range(p.ncols))
     _same(trellis.build_plan(p.h, p.l, 1, order, p.nrows), trellis.build_plans_native(p.h, p.l, [1], order, p.nrows)[0])
+
+
+def test_reference_style_plan_objects_encode_identically():
+    from tropidist_amd import problem
+    p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=3)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+
+    class Ref:
+        ops = [tuple(op) for op in plan.ops]
+        peak_bits = plan.peak_bits
+        feasible = True
+    q = trellis.as_plan(Ref())
+    assert (q.encode() == plan.encode()).all() and q.work() == plan.work() and q.peak_bits == plan.peak_bits
+    assert trellis.as_plan(plan) is plan

@@ -531,6 +531,8 @@ def run_program(plan, program=None, T0=None, T1=None, stream=None, raw=False):
     if int(L.tdp_pass_args_bytes()) != PASS_DTYPE.itemsize:
         raise RuntimeError("pass descriptor layout mismatch: library %d bytes, host %d"
                            % (L.tdp_pass_args_bytes(), PASS_DTYPE.itemsize))
+    from .trellis import as_plan
+    plan = as_plan(plan)
     if program is None:
         _, (segs, arr) = program_for(plan)
     else:

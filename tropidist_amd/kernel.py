@@ -33,9 +33,12 @@ EXECUTORS = ("fused", "per-op")
 
 def run_plan(plan, log=None, progress_every=0.0, stream=None, executor="fused"):
     """Run the plan on the GPU.  Returns (min cost or None if infeasible, stats); OverflowError if the cost is
-    >= 255 (cdist/kernel.py:94-138).  stats: seconds, peak_bits, cus, executor."""
+    >= 255 (cdist/kernel.py:94-138).  `plan` is a Plan or any object with the reference's plan attributes.
+    stats: seconds, peak_bits, cus (= threads), executor."""
     if executor not in EXECUTORS:
         raise ValueError("executor must be one of %s" % (EXECUTORS,))
+    from .trellis import as_plan
+    plan = as_plan(plan)
     if not plan.feasible:
         return None, {"feasible": False}
     L = _lib.lib()
@@ -58,5 +61,6 @@ def run_plan(plan, log=None, progress_every=0.0, stream=None, executor="fused"):
         log("  plan of %d ops on 2^%d entries (%s): %.4f s" % (plan.nops, plan.peak_bits, executor, seconds))
     cus = ctypes.c_int(0)
     _lib.check(L.tdp_num_cus(ctypes.byref(cus)), "tdp_num_cus")
+    # "threads" is the key of the reference's stats (its OpenMP thread count); here the device's compute units
     return final_value(raw), {"seconds": seconds, "peak_bits": plan.peak_bits, "cus": cus.value,
-                              "executor": executor}
+                              "threads": cus.value, "executor": executor}

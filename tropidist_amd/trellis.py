@@ -161,6 +161,16 @@ class Plan:
         return len(self.ops)
 
 
+def as_plan(plan):
+    """A Plan for any object with the reference's plan attributes (ops, peak_bits, feasible; cdist/trellis.py:125-144),
+    so the executors accept plans made by the reference's own planner; Plans pass through unchanged."""
+    if isinstance(plan, Plan):
+        return plan
+    ops = [tuple(op) for op in plan.ops]
+    return Plan(ops, int(plan.peak_bits), list(getattr(plan, "widths", []) or []),
+                int(getattr(plan, "ncols", 0) or 0), bool(plan.feasible))
+
+
 def make_plan(hrows, lrow, ncols, weights=None):
     """DP schedule for constraint rows (target 0) and one target-1 row (cdist/trellis.py:147-182); weights[j] is the
     cost of the column at position j (default 1)."""
