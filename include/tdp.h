@@ -85,6 +85,18 @@ int tdp_run_pass(const uint8_t *d_in, uint8_t *d_out, const void *pass, void *st
 int tdp_run_program(const uint64_t *ops, int64_t nops, const int64_t *segments, int64_t nseg, const void *passes,
                     int64_t npasses, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value, void *stream);
 
+/* Witness by fused passes: the program runs as nsteps steps (records of tdp_witness_step_bytes() bytes: a pass of
+ * `passes`, or one per-op op, each writing a fresh slot of d_arena so every step's input table is kept), then one
+ * wave walks the steps backward and names the updates on one minimum path (the traceback of
+ * cdist/auto.py:126-182: a column is used iff the entry before it differs from the value after it).
+ * d_arena: tdp_program_witness_bytes(steps) bytes of device memory; used_ops: host array with room for every
+ * update of the steps.  *value = the final entry (255 leaves *nused = 0). */
+int tdp_witness_step_bytes(void);
+int tdp_program_witness_bytes(const void *steps, int64_t nsteps, uint64_t *bytes);
+int tdp_run_program_witness(const uint64_t *ops, int64_t nops, const void *steps, int64_t nsteps,
+                            const void *passes, int64_t npasses, uint8_t *d_arena, uint64_t arena_bytes, int *value,
+                            int64_t *used_ops, int64_t *nused, void *stream);
+
 /* Witness of a plan: the plan's value and the op indices of the updates on one minimum path, by a checkpointed
  * traceback from the final state 0 (replaces the numpy replay of dp_witness, cdist/auto.py:126-182, which keeps
  * every table).  d_work: caller workspace of tdp_witness_workspace_bytes(nops, peak_bits) bytes; used_ops: host

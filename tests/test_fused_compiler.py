@@ -118,3 +118,44 @@ def test_pack_layout():
 def test_library_descriptor_size():
     from tropidist_amd import _lib
     assert int(_lib.lib().tdp_pass_args_bytes()) == fused.PASS_DTYPE.itemsize
+
+
+def _cols(plan, order, used):
+    ordinal = np.cumsum(plan.encode()[:, 0] == 1) - 1
+    return sorted(order[plan.columns[int(ordinal[i])]] for i in used)
+
+
+def test_witness_mode_traceback_on_cpu():
+    """The witness-mode program and the traceback arithmetic of the device kernel give a minimum-cost solution:
+    value == brute force, and the columns satisfy H e = 0, (L e)_obs = 1 with that cost (random problems of the
+    reference's generator, half with costs 1-3, and weighted banded models)."""
+    from tropidist_amd.problem import Problem, verify_witness
+    rng = random.Random(5)
+    checked = 0
+    for t in range(250):
+        m, k, h, l = random_problem(rng)
+        w = [rng.randint(1, 3) for _ in h] if t % 2 else None
+        order = list(range(len(h)))
+        rng.shuffle(order)
+        plan = trellis.build_plan(h, l, 1, order, m, weights=w)
+        if not plan.feasible:
+            continue
+        v, used = fused.witness_cpu(plan)
+        assert v == oracle.brute(h, l, 1, w), t
+        assert verify_witness(Problem("r", m, k, h, l, w=w), _cols(plan, order, used), v, 0), t
+        checked += 1
+    assert checked > 120
+    for W, seed in ((8, 1), (12, 2)):
+        p = problem.banded(R=60, N=400, kmin=2, kmax=6, W=W, nobs=1, rho=0.1, weights=(1, 3), seed=seed)
+        order = list(range(p.ncols))
+        plan = trellis.build_plan(p.h, p.l, 1, order, p.nrows, weights=p.w)
+        prog, steps, _ = fused.witness_program(plan)
+        assert all(len(q.upd_ops) <= fused.WITNESS_MAX_UPDATES for q in prog.passes)
+        v, used = fused.witness_cpu(plan)
+        assert v == oracle.run_plan_c(plan, "oracle")
+        assert verify_witness(p, _cols(plan, order, used), v, 0)
+
+
+def test_witness_step_layout():
+    from tropidist_amd import _lib
+    assert fused.WSTEP_DTYPE.itemsize == int(_lib.lib().tdp_witness_step_bytes()) == 240

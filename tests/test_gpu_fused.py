@@ -124,3 +124,76 @@ def test_bad_descriptor_rejected_on_host():
     assert rc == 1    # TDP_EINVAL
     assert b"outside" in L.tdp_last_error()
     _lib.torch().cuda.synchronize()
+
+
+def _cols(plan, order, used):
+    ordinal = np.cumsum(plan.encode()[:, 0] == 1) - 1
+    return sorted(order[plan.columns[int(ordinal[i])]] for i in used)
+
+
+def test_fused_witness_random_against_brute_force():
+    """Witness-mode program + one-wave traceback: value == brute force and a valid minimum solution, on the
+    reference's random generator (half with costs 1-3)."""
+    from tropidist_amd.problem import Problem, verify_witness
+    print("case fused witness random", flush=True)
+    rng = random.Random(5)
+    checked = 0
+    for t in range(200):
+        m, k, h, l = random_problem(rng)
+        w = [rng.randint(1, 3) for _ in h] if t % 2 else None
+        order = list(range(len(h)))
+        rng.shuffle(order)
+        plan = trellis.build_plan(h, l, 1, order, m, weights=w)
+        if not plan.feasible:
+            continue
+        v, used = fused.witness(plan)
+        assert v == oracle.brute(h, l, 1, w), t
+        assert verify_witness(Problem("r", m, k, h, l, w=w), _cols(plan, order, used), v, 0), t
+        assert (v, used) == fused.witness_cpu(plan), t
+        checked += 1
+    assert checked > 100
+
+
+@pytest.mark.parametrize("W,weights", [(12, (1, 3)), (18, None), (22, (1, 3))])
+def test_fused_witness_banded(W, weights):
+    from tropidist_amd import auto
+    from tropidist_amd.problem import verify_witness
+    print("case fused witness banded W", W, flush=True)
+    p = problem.banded(R=100, N=600, kmin=2, kmax=6, W=W, nobs=1, rho=0.1, weights=weights, seed=100 + W)
+    order = list(range(p.ncols))
+    plan = trellis.build_plan(p.h, p.l, 1, order, p.nrows, weights=p.w)
+    assert plan.feasible
+    want = kernel.run_plan(plan, executor="per-op")[0]
+    v, used = fused.witness(plan)
+    cols = _cols(plan, order, used)
+    assert v == want and verify_witness(p, cols, v, 0)
+    v2, cols2 = auto.witness_plan(plan, order, executor="per-op")
+    assert v2 == v and verify_witness(p, cols2, v2, 0)
+
+
+def test_fused_witness_recorded_instances():
+    """auto.witness_plan (fused by default) on the recorded audit instances up to 2^22 entries: the recorded
+    distance and a checked witness."""
+    from tropidist_amd import auto
+    from tropidist_amd.problem import Problem, verify_witness
+    checked = 0
+    for r in dense_fixtures():
+        if r["distances"] is None or r["peak_bits"] > 22:
+            continue
+        p = Problem(r["name"], r["nrows"], r["nobs"], r["h"], r["l"])
+        for q, d in zip(r["plans"], r["distances"]):
+            if d is None:
+                continue
+            print("case fused witness fixture", r["name"], q["observable"], flush=True)
+            plan = trellis.build_plan(r["h"], r["l"], 1 << q["observable"], r["order"], r["nrows"])
+            v, cols = auto.witness_plan(plan, r["order"])
+            assert v == d and verify_witness(p, cols, v, q["observable"]), (r["name"], q["observable"])
+            checked += 1
+    assert checked >= 10
+
+
+def test_fused_witness_overflow():
+    p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=5)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows, weights=[200] * p.ncols)
+    with pytest.raises(OverflowError):
+        fused.witness(plan)

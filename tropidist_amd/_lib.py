@@ -29,6 +29,10 @@ SIGNATURES = {
     "tdp_run_plan": (ci, [vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
     "tdp_pass_args_bytes": (ci, []),
     "tdp_run_pass": (ci, [vp, vp, vp, vp]),
+    "tdp_witness_step_bytes": (ci, []),
+    "tdp_program_witness_bytes": (ci, [vp, i64, ctypes.POINTER(u64)]),
+    "tdp_run_program_witness": (ci, [vp, i64, vp, i64, vp, i64, vp, u64, ctypes.POINTER(ci), vp,
+                                     ctypes.POINTER(i64), vp]),
     "tdp_run_program": (ci, [vp, i64, vp, i64, vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
     "tdp_witness_workspace_bytes": (ci, [i64, ci, ctypes.POINTER(u64)]),
     "tdp_run_plan_witness": (ci, [vp, i64, ci, vp, u64, ctypes.POINTER(ci), vp, ctypes.POINTER(i64), vp]),

@@ -93,9 +93,29 @@ def dp_distance(p, order, max_table_bytes=None, deadline=None):
     return best, per
 
 
-def witness_plan(plan, order, max_bytes=None, stream=None):
-    """(value, sorted problem columns of one minimum path) of a feasible plan, by the GPU traceback; None if the
-    workspace would exceed max_bytes.  OverflowError at 255."""
+# Device bytes the fused witness may use by default (the arena keeps every step's input table).
+FUSED_WITNESS_BYTES = 32 << 30
+
+
+def _columns_of(plan, order, op_indices):
+    ops = plan.encode()
+    ordinal = np.cumsum(ops[:, 0] == 1) - 1
+    return sorted(order[plan.columns[int(ordinal[int(i)])]] for i in op_indices)
+
+
+def witness_plan(plan, order, max_bytes=None, stream=None, executor="fused"):
+    """(value, sorted problem columns of one minimum path) of a feasible plan, by a GPU traceback; None if the
+    workspace would exceed max_bytes.  OverflowError at 255.
+
+    executor "fused": the witness-mode fused program (every step's input kept in one arena, a one-wave traceback),
+    used when its arena fits max_bytes (default FUSED_WITNESS_BYTES), else the per-op path; "per-op": forward op by
+    op with checkpoints every sqrt(ops) ops and segment recomputation (tdp_run_plan_witness)."""
+    if executor == "fused":
+        from . import fused
+        budget = FUSED_WITNESS_BYTES if max_bytes is None else max_bytes
+        if fused.witness_bytes(plan) <= budget:
+            v, used = fused.witness(plan, stream=stream)
+            return v, _columns_of(plan, order, used)
     L = _lib.lib()
     nbytes = ctypes.c_uint64(0)
     ops = plan.encode()
@@ -110,9 +130,7 @@ def witness_plan(plan, order, max_bytes=None, stream=None):
                                 used.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nused), _lib.stream_handle(stream))
     _lib.check(rc, "tdp_run_plan_witness")
     v = kernel.final_value(value.value)
-    ordinal = np.cumsum(ops[:, 0] == 1) - 1
-    cols = sorted(order[plan.columns[int(ordinal[int(i)])]] for i in used[:nused.value])
-    return v, cols
+    return v, _columns_of(plan, order, used[:nused.value])
 
 
 def dp_witness(p, order, obs, max_bytes=None):

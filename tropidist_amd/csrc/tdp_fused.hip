@@ -253,7 +253,119 @@ int launch_pass(const uint8_t *in, uint8_t *out, const PassArgs &a, hipStream_t 
     return TDP_OK;
 }
 
+// ------------------------------------------------------------------------------------------ witness mode
+// Every step (a pass, or one per-op op; an expand followed by the update of its new top bit is one step) writes a
+// fresh slot of the caller's arena, so each step's input table stays in HBM for the traceback.
+struct WStep {
+    uint64_t in_off, inmask, newmask, fixmask, fixval;   // input slot, its index mask, opened / ended-row variables
+    uint64_t m[6];                                      // update masks in the step's variable space, op order
+    int64_t op[6];                                      // op index of each update
+    uint64_t out_off, n_in;                             // output slot, input entries
+    int64_t op0;                                        // first op of a per-op step
+    int32_t kind, pass_index, nop, k, b_out, pad;       // kind 0 = pass, 1 = per-op; k updates; output bits
+    uint8_t w[6];                                       // update weights
+    uint8_t out_var[48];                                // variable held by each output bit
+    uint8_t pad2[2];
+};
+static_assert(sizeof(WStep) == 240, "witness step layout");
+
+constexpr uint64_t kSlotAlign = 256;
+static inline uint64_t align_up(uint64_t x) { return (x + kSlotAlign - 1) / kSlotAlign * kSlotAlign; }
+
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        uint32_t u = (uint32_t)__shfl_xor((int)v, o, 64);
+        v = u < v ? u : v;
+    }
+    return v;
+}
+
+// One wave walks the steps backward.  At a step with k updates, the state x after it and the value V there, lane l
+// holds X[x ^ XOR_{j in l} m_j] + sum_{j in l} w_j (X = 255 where an opened row is 1); update i, last first, was
+// used iff the least of the lanes that agree with the used set T above i differs from V.
+// io = {value, nused, error}: error 1 = an opened row is still 1 at the step's input, 2 = the walk does not end in
+// state 0 with value 0.
+__global__ __launch_bounds__(64) void k_traceback(const uint8_t *__restrict__ arena, const WStep *__restrict__ steps,
+                                                  int64_t nsteps, uint64_t final_off, int64_t *__restrict__ used,
+                                                  int64_t *__restrict__ io) {
+    const int lane = threadIdx.x;
+    uint32_t v = arena[final_off];
+    int64_t nused = 0, err = 0;
+    if (v < 255 && nsteps > 0) {
+        uint64_t x = steps[nsteps - 1].fixval;
+        for (int64_t s = nsteps - 1; s >= 0; s--) {
+            const WStep &st = steps[s];
+            const int k = st.k;
+            const uint32_t V = v;
+            uint32_t val = 0xFFFFFFFFu;
+            if (lane < (1 << k)) {
+                uint64_t y = x;
+                uint32_t ws = 0;
+                for (int j = 0; j < k; j++)
+                    if ((lane >> j) & 1) {
+                        y ^= st.m[j];
+                        ws += st.w[j];
+                    }
+                val = ((y & st.newmask) ? 255u : (uint32_t)arena[st.in_off + (y & st.inmask)]) + ws;
+            }
+            uint32_t T = 0;
+            for (int i = k - 1; i >= 0; i--) {
+                uint32_t cand = (lane < (1 << k) && ((uint32_t)lane >> i) == (T >> i)) ? val : 0xFFFFFFFFu;
+                if (wave_min(cand) != V) {
+                    T |= 1u << i;
+                    v -= st.w[i];
+                    if (lane == 0) used[nused] = st.op[i];
+                    nused++;
+                }
+            }
+            for (int j = 0; j < k; j++)
+                if ((T >> j) & 1) x ^= st.m[j];
+            if (x & st.newmask) {
+                err = 1;
+                break;
+            }
+            if (s > 0) {
+                const WStep &pv = steps[s - 1];
+                uint64_t xp = pv.fixval;
+                for (int q = 0; q < pv.b_out; q++)
+                    if ((x >> q) & 1) xp |= 1ull << pv.out_var[q];
+                x = xp;
+            }
+        }
+        if (!err && (x != 0 || v != 0)) err = 2;
+    }
+    if (lane == 0) {
+        io[0] = (int64_t)arena[final_off];
+        io[1] = nused;
+        io[2] = err;
+    }
+}
+
+struct WitnessLayout {
+    uint64_t tables, steps_off, used_off, io_off, total, nupd;
+};
+
+static int witness_layout(const WStep *steps, int64_t nsteps, WitnessLayout *L) {
+    TDP_REQUIRE(nsteps >= 1 && steps, "witness program without steps");
+    uint64_t nupd = 0, end = 0;
+    for (int64_t s = 0; s < nsteps; s++) {
+        TDP_REQUIRE(steps[s].k >= 0 && steps[s].k <= 6, "step %lld: %d updates", (long long)s, steps[s].k);
+        nupd += (uint64_t)steps[s].k;
+        uint64_t e = steps[s].out_off + (steps[s].b_out >= 0 && steps[s].b_out <= 40 ? (1ull << steps[s].b_out) : 0);
+        end = e > end ? e : end;
+    }
+    L->tables = align_up(end);
+    L->steps_off = L->tables;
+    L->used_off = align_up(L->steps_off + (uint64_t)nsteps * sizeof(WStep));
+    L->io_off = align_up(L->used_off + 8 * (nupd ? nupd : 1));
+    L->total = L->io_off + 64;
+    L->nupd = nupd;
+    return TDP_OK;
+}
+
 }  // namespace tdp
+
 
 using namespace tdp;
 
@@ -301,6 +413,110 @@ int tdp_run_program(const uint64_t *ops, int64_t nops, const int64_t *segments, 
     TDP_HIP_CHECK(hipMemcpyAsync(&v, cur, 1, hipMemcpyDeviceToHost, st));
     TDP_HIP_CHECK(hipStreamSynchronize(st));
     *value = v;
+    return TDP_OK;
+}
+
+int tdp_witness_step_bytes(void) { return (int)sizeof(WStep); }
+
+int tdp_program_witness_bytes(const void *steps, int64_t nsteps, uint64_t *bytes) {
+    WitnessLayout L;
+    int rc = witness_layout(static_cast<const WStep *>(steps), nsteps, &L);
+    if (rc) return rc;
+    *bytes = L.total;
+    return TDP_OK;
+}
+
+int tdp_run_program_witness(const uint64_t *ops, int64_t nops, const void *steps_v, int64_t nsteps,
+                            const void *passes, int64_t npasses, uint8_t *d_arena, uint64_t arena_bytes, int *value,
+                            int64_t *used, int64_t *nused, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const WStep *S = static_cast<const WStep *>(steps_v);
+    const PassArgs *P = static_cast<const PassArgs *>(passes);
+    WitnessLayout L;
+    int rc = witness_layout(S, nsteps, &L);
+    if (rc) return rc;
+    TDP_REQUIRE(d_arena && arena_bytes >= L.total, "witness arena of %llu bytes, %llu needed",
+                (unsigned long long)arena_bytes, (unsigned long long)L.total);
+    TDP_REQUIRE(S[0].n_in == 1 && S[0].in_off == 0 && S[nsteps - 1].b_out == 0,
+                "the steps must lead from the one-entry table to the one-entry table");
+    // validate the chain of slots and every step before launching anything
+    for (int64_t s = 0; s < nsteps; s++) {
+        const WStep &w = S[s];
+        TDP_REQUIRE(w.n_in >= 1 && (w.n_in & (w.n_in - 1)) == 0 && w.n_in <= (1ull << 40), "step %lld: bad size",
+                    (long long)s);
+        TDP_REQUIRE(w.b_out >= 0 && w.b_out <= 40 && w.out_off % kSlotAlign == 0 && w.in_off % kSlotAlign == 0,
+                    "step %lld: bad slot", (long long)s);
+        TDP_REQUIRE(w.in_off + w.n_in <= w.out_off && w.out_off + (1ull << w.b_out) <= L.tables,
+                    "step %lld: slots out of order or outside the arena", (long long)s);
+        TDP_REQUIRE(w.inmask == w.n_in - 1, "step %lld: index mask", (long long)s);
+        for (int q = 0; q < w.b_out; q++) TDP_REQUIRE(w.out_var[q] < 64, "step %lld: output variable", (long long)s);
+        if (s + 1 < nsteps)
+            TDP_REQUIRE(S[s + 1].in_off == w.out_off && S[s + 1].n_in == (1ull << w.b_out),
+                        "step %lld: the next step does not read this step's output", (long long)s);
+        const int bin = top_bit(w.n_in);
+        if (w.kind == 0) {
+            TDP_REQUIRE(w.pass_index >= 0 && w.pass_index < npasses, "step %lld: pass index", (long long)s);
+            const PassArgs &p = P[w.pass_index];
+            rc = check_pass(p);
+            if (rc) return rc;
+            TDP_REQUIRE(p.b_in == bin && p.b_out == w.b_out, "step %lld: pass sizes", (long long)s);
+        } else {
+            TDP_REQUIRE(w.kind == 1 && w.op0 >= 0 && w.nop >= 1 && w.nop <= 2 && w.op0 + w.nop <= nops,
+                        "step %lld: op range", (long long)s);
+            const uint64_t *o = ops + 3 * w.op0;
+            if (o[0] == 0) {
+                TDP_REQUIRE(w.b_out == bin + 1, "step %lld: expand size", (long long)s);
+                if (w.nop == 2)
+                    TDP_REQUIRE(o[3] == 1 && (o[4] >> bin) == 1 && o[5] >= 1, "step %lld: expand-update",
+                                (long long)s);
+            } else if (o[0] == 1) {
+                TDP_REQUIRE(w.nop == 1 && o[1] < w.n_in && o[1] >= 1 && o[2] >= 1 && w.b_out == bin,
+                            "step %lld: update", (long long)s);
+            } else {
+                TDP_REQUIRE(o[0] == 2 && w.nop == 1 && bin >= 1 && o[1] < (uint64_t)bin && o[2] <= 1 &&
+                            w.b_out == bin - 1, "step %lld: select", (long long)s);
+            }
+        }
+    }
+    rc = launch_fill(d_arena, 1, 0, st);
+    if (rc) return rc;
+    for (int64_t s = 0; s < nsteps && !rc; s++) {
+        const WStep &w = S[s];
+        uint8_t *in = d_arena + w.in_off, *out = d_arena + w.out_off;
+        const uint64_t n = w.n_in;
+        if (w.kind == 0) {
+            rc = launch_pass(in, out, P[w.pass_index], st);
+            continue;
+        }
+        const uint64_t *o = ops + 3 * w.op0;
+        if (o[0] == 2) {
+            uint64_t nn = 0;
+            rc = launch_select(out, in, n, (int)o[1], (int)o[2], &nn, st);
+            continue;
+        }
+        TDP_HIP_CHECK(hipMemcpyAsync(out, in, n, hipMemcpyDeviceToDevice, st));
+        if (o[0] == 0 && w.nop == 2)
+            rc = launch_expand(out, n, o[4] ^ n, (uint32_t)(o[5] > 255 ? 255 : o[5]), st);
+        else if (o[0] == 0)
+            rc = launch_fill(out + n, n, 255, st);
+        else
+            rc = launch_update(out, n, o[1], (uint32_t)(o[2] > 255 ? 255 : o[2]), st);
+    }
+    if (rc) return rc;
+    WStep *d_steps = reinterpret_cast<WStep *>(d_arena + L.steps_off);
+    int64_t *d_used = reinterpret_cast<int64_t *>(d_arena + L.used_off);
+    int64_t *d_io = reinterpret_cast<int64_t *>(d_arena + L.io_off);
+    TDP_HIP_CHECK(hipMemcpyAsync(d_steps, S, (size_t)nsteps * sizeof(WStep), hipMemcpyHostToDevice, st));
+    k_traceback<<<1, 64, 0, st>>>(d_arena, d_steps, nsteps, S[nsteps - 1].out_off, d_used, d_io);
+    TDP_LAUNCH_CHECK();
+    int64_t io[3] = {0, 0, 0};
+    TDP_HIP_CHECK(hipMemcpyAsync(io, d_io, sizeof(io), hipMemcpyDeviceToHost, st));
+    TDP_HIP_CHECK(hipStreamSynchronize(st));
+    TDP_REQUIRE(io[2] == 0, "traceback failed (code %lld)", (long long)io[2]);
+    TDP_REQUIRE(io[1] >= 0 && (uint64_t)io[1] <= L.nupd, "traceback used count %lld", (long long)io[1]);
+    if (io[1] > 0) TDP_HIP_CHECK(hipMemcpy(used, d_used, (size_t)io[1] * 8, hipMemcpyDeviceToHost));
+    *value = (int)io[0];
+    *nused = io[1];
     return TDP_OK;
 }
 

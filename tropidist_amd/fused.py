@@ -58,6 +58,13 @@ class Pass:
     low_lam: list = field(default_factory=list)     # updates acting inside each dword
     low_w: list = field(default_factory=list)
     byte_store: bool = False
+    # for the traceback: every update of the pass in op order as (variable mask, weight, op index), the new and
+    # the ending rows' variables (with the targets), and the variable held by each output bit
+    upd_ops: list = field(default_factory=list)
+    new_mask: int = 0
+    fix_mask: int = 0
+    fix_val: int = 0
+    out_vars: list = field(default_factory=list)
 
     @property
     def nfree(self):
@@ -147,6 +154,7 @@ def _build_pass(enc, first, last, in_log, logical_out):
     nvar = b_in
     new_vars, sel_vars, sel_t = [], [], {}
     upd = []                    # (var mask, weight)
+    upd_ops = []                # (var mask, weight, op index)
     for i in range(first, last):
         code, a, b = (int(x) for x in enc[i])
         if code == OP_EXPAND:
@@ -159,6 +167,8 @@ def _build_pass(enc, first, last, in_log, logical_out):
                 if (a >> p) & 1:
                     m |= 1 << lay[p]
             upd.append((m, b))
+            if m:                   # a zero mask changes nothing and is never on a path
+                upd_ops.append((m, b, i))
         else:
             u = lay.pop(a)
             sel_vars.append(u)
@@ -241,7 +251,9 @@ def _build_pass(enc, first, last, in_log, logical_out):
                 sum(sel_t[v] << sel_idx[v] for v in sel_vars),
                 [d[1] for d in dirs], [d[2] for d in dirs],
                 [x[0] for x in lows], [x[1] for x in lows],
-                byte_store=not (b_out >= 2 and phys[0] == 0 and phys[1] == 1)), out_log
+                byte_store=not (b_out >= 2 and phys[0] == 0 and phys[1] == 1),
+                upd_ops=upd_ops, new_mask=sum(1 << v for v in new_vars), fix_mask=sum(1 << v for v in sel_vars),
+                fix_val=sum(sel_t[v] << v for v in sel_vars), out_vars=list(phys)), out_log
 
 
 def _bits_after(enc, b, first, last):
@@ -251,9 +263,9 @@ def _bits_after(enc, b, first, last):
     return b
 
 
-def compile_program(plan, dmax=DMAX, min_bits=12):
+def compile_program(plan, dmax=DMAX, min_bits=12, max_updates=None):
     """Segments of a feasible plan: per-op ranges while the table is narrower than 2^min_bits, passes of up to
-    dmax independent update directions otherwise."""
+    dmax independent update directions otherwise (and at most max_updates updates in all, when given)."""
     enc = plan.encode()
     cols = _columns(enc)
     segs = []
@@ -314,6 +326,8 @@ def compile_program(plan, dmax=DMAX, min_bits=12):
                 if nlow >= MAX_LOW:
                     break
             if nv2 - 2 - (D + (1 if mh else 0)) > MAX_FREE:
+                break
+            if max_updates is not None and m and D + nlow + 1 > max_updates:
                 break
             nnew += ne
             nsel += ns
@@ -547,3 +561,179 @@ def run_program(plan, program=None, T0=None, T1=None, stream=None, raw=False):
                            _lib.dptr(T0), _lib.dptr(T1), ctypes.byref(value), _lib.stream_handle(stream))
     _lib.check(rc, "tdp_run_program")
     return value.value if raw else kernel.final_value(value.value)
+
+
+# ------------------------------------------------------------------------------------------ witness (traceback)
+# Witness mode runs the program with every step writing a fresh slot of one arena, so the input table of every
+# step stays in HBM.  A step is a pass or one per-op op (an expand directly followed by the update of its new top
+# bit is one step, as in the per-op executor).  The traceback walks the steps backward in the step's variable
+# space (input bits, then the rows it opens): at a step with updates u_0..u_{k-1} (k <= 6), the state x after it
+# and its value V there, lane l of one wave loads X[x ^ XOR_{j in l} m_j] + sum_{j in l} w_j (X = 255 where a row
+# opened in the step is 1), and update i, last first, was used iff the minimum over the lanes that agree with the
+# used set above i differs from V.  Every entry any decision needs is one of the 2^k loaded values.
+WITNESS_MAX_UPDATES = 6
+STEP_PASS, STEP_OPS = 0, 1
+WSTEP_DTYPE = np.dtype([("in_off", "<u8"), ("inmask", "<u8"), ("newmask", "<u8"), ("fixmask", "<u8"),
+                        ("fixval", "<u8"), ("m", "<u8", 6), ("op", "<i8", 6), ("out_off", "<u8"), ("n_in", "<u8"),
+                        ("op0", "<i8"), ("kind", "<i4"), ("pass_index", "<i4"), ("nop", "<i4"), ("k", "<i4"),
+                        ("b_out", "<i4"), ("pad", "<i4"), ("w", "u1", 6), ("out_var", "u1", 48), ("pad2", "u1", 2)],
+                       align=True)
+SLOT_ALIGN = 256
+
+
+def _align(x):
+    return (x + SLOT_ALIGN - 1) // SLOT_ALIGN * SLOT_ALIGN
+
+
+def witness_program(plan):
+    """(program, steps WSTEP_DTYPE array, table bytes of the arena) of a feasible plan in witness mode: passes of at
+    most six updates from 2^2 entries up, single per-op steps below."""
+    prog = compile_program(plan, min_bits=2, max_updates=WITNESS_MAX_UPDATES)
+    enc = plan.encode()
+    rows = []
+    off = 0
+    b = 0
+    pass_index = 0
+    for seg in prog.segments:
+        if seg[0] == "pass":
+            p = seg[1]
+            assert len(p.upd_ops) <= WITNESS_MAX_UPDATES and p.b_out <= 48
+            rows.append(dict(kind=STEP_PASS, pass_index=pass_index, op0=p.first, nop=p.last - p.first, b_in=p.b_in,
+                             b_out=p.b_out, newmask=p.new_mask, fixmask=p.fix_mask, fixval=p.fix_val,
+                             upd=p.upd_ops, out_var=p.out_vars))
+            pass_index += 1
+            b = p.b_out
+            continue
+        i = seg[1]
+        while i < seg[2]:
+            code, a, w = (int(x) for x in enc[i])
+            if code == OP_EXPAND:
+                fuse = i + 1 < seg[2] and int(enc[i + 1, 0]) == OP_UPDATE and (int(enc[i + 1, 1]) >> b) == 1
+                upd = [(int(enc[i + 1, 1]), int(enc[i + 1, 2]), i + 1)] if fuse else []
+                rows.append(dict(kind=STEP_OPS, pass_index=-1, op0=i, nop=2 if fuse else 1, b_in=b, b_out=b + 1,
+                                 newmask=1 << b, fixmask=0, fixval=0, upd=upd, out_var=list(range(b + 1))))
+                i += 2 if fuse else 1
+                b += 1
+            elif code == OP_UPDATE:
+                rows.append(dict(kind=STEP_OPS, pass_index=-1, op0=i, nop=1, b_in=b, b_out=b, newmask=0, fixmask=0,
+                                 fixval=0, upd=[(a, w, i)] if a else [], out_var=list(range(b))))
+                i += 1
+            else:
+                rows.append(dict(kind=STEP_OPS, pass_index=-1, op0=i, nop=1, b_in=b, b_out=b - 1, newmask=0,
+                                 fixmask=1 << a, fixval=w << a,
+                                 upd=[], out_var=[q if q < a else q + 1 for q in range(b - 1)]))
+                i += 1
+                b -= 1
+    steps = np.zeros(len(rows), dtype=WSTEP_DTYPE)
+    for r, row in zip(steps, rows):
+        r["in_off"] = off
+        r["n_in"] = 1 << row["b_in"]
+        off += _align(1 << row["b_in"])
+        r["out_off"] = off
+        r["inmask"] = (1 << row["b_in"]) - 1
+        r["newmask"], r["fixmask"], r["fixval"] = row["newmask"], row["fixmask"], row["fixval"]
+        r["kind"], r["pass_index"], r["op0"], r["nop"] = row["kind"], row["pass_index"], row["op0"], row["nop"]
+        r["k"] = len(row["upd"])
+        for j, (m, w, i) in enumerate(row["upd"]):
+            r["m"][j], r["w"][j], r["op"][j] = m, min(255, w), i
+        r["b_out"] = row["b_out"]
+        r["out_var"][:len(row["out_var"])] = row["out_var"]
+    table_bytes = off + SLOT_ALIGN          # the final one-entry table
+    return prog, steps, table_bytes
+
+
+def witness_cpu(plan):
+    """(value, used op indices) by the witness-mode program on the CPU: the forward steps by numpy, the traceback
+    with the device kernel's arithmetic.  Used by the tests."""
+    prog, steps, _ = witness_program(plan)
+    enc = plan.encode()
+    passes = prog.passes
+    tables = []
+    T = np.zeros(1, np.uint8)
+    for st in steps:
+        assert T.size == int(st["n_in"])
+        tables.append(T)
+        if st["kind"] == STEP_PASS:
+            T = run_pass_cpu(T, passes[int(st["pass_index"])])
+        else:
+            T = _apply_ops(T, enc, int(st["op0"]), int(st["op0"]) + int(st["nop"]))
+    assert T.size == 1
+    v = int(T[0])
+    if v == 255:
+        return v, []
+    used = []
+    x = int(steps[-1]["fixval"])
+    for s in range(len(steps) - 1, -1, -1):
+        st = steps[s]
+        k = int(st["k"])
+        X = tables[s]
+        vals = []
+        for lane in range(1 << k):
+            y, ws = x, 0
+            for j in range(k):
+                if (lane >> j) & 1:
+                    y ^= int(st["m"][j])
+                    ws += int(st["w"][j])
+            vals.append((255 if y & int(st["newmask"]) else int(X[y & int(st["inmask"])])) + ws)
+        # with T the used set above i, the entry before update i at the current state is
+        # min(255, min over lanes T | S (S below i) of vals - w(T)), and the current value is V - w(T), V the value
+        # after the step: update i was used iff that minimum differs from V (V < 255 on a path)
+        V = v
+        used_set = 0
+        for i in range(k - 1, -1, -1):
+            before = min(vals[lane] for lane in range(1 << k) if (lane >> i) == (used_set >> i))
+            if before != V:
+                used_set |= 1 << i
+                v -= int(st["w"][i])
+                used.append(int(st["op"][i]))
+        for j in range(k):
+            if (used_set >> j) & 1:
+                x ^= int(st["m"][j])
+        assert x & int(st["newmask"]) == 0, "traceback left a row opened in the step set"
+        if s > 0:
+            prev = steps[s - 1]
+            xp = int(prev["fixval"])
+            for q in range(int(prev["b_out"])):
+                if (x >> q) & 1:
+                    xp |= 1 << int(prev["out_var"][q])
+            x = xp
+    assert x == 0 and v == 0, (x, v)
+    return int(T[0]), sorted(used)
+
+
+def witness(plan, max_bytes=None, stream=None):
+    """(value, sorted op indices of the updates on one minimum path) of a feasible plan by the witness-mode program
+    on the GPU; None if its arena would exceed max_bytes.  OverflowError at 255."""
+    import ctypes
+    from . import _lib, kernel
+    from .trellis import as_plan
+    plan = as_plan(plan)
+    L = _lib.lib()
+    if int(L.tdp_witness_step_bytes()) != WSTEP_DTYPE.itemsize:
+        raise RuntimeError("witness step layout mismatch")
+    prog, steps, _ = witness_program(plan)
+    nbytes = ctypes.c_uint64(0)
+    _lib.check(L.tdp_program_witness_bytes(steps.ctypes.data_as(ctypes.c_void_p), len(steps), ctypes.byref(nbytes)),
+               "tdp_program_witness_bytes")
+    if max_bytes is not None and nbytes.value > max_bytes:
+        return None
+    _, arr = pack(prog)
+    ops = plan.encode()
+    arena = _lib.device_empty(nbytes.value)
+    used = np.zeros(max(1, int(steps["k"].sum())), dtype=np.int64)
+    value, nused = ctypes.c_int(-1), ctypes.c_int64(0)
+    rc = L.tdp_run_program_witness(ops.ctypes.data_as(ctypes.c_void_p), len(ops),
+                                   steps.ctypes.data_as(ctypes.c_void_p), len(steps),
+                                   arr.ctypes.data_as(ctypes.c_void_p), len(arr), _lib.dptr(arena), nbytes.value,
+                                   ctypes.byref(value), used.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nused),
+                                   _lib.stream_handle(stream))
+    _lib.check(rc, "tdp_run_program_witness")
+    v = kernel.final_value(value.value)
+    return v, sorted(int(i) for i in used[:nused.value])
+
+
+def witness_bytes(plan):
+    """Device bytes of the witness-mode arena of a plan (tables of every step plus the traceback's records)."""
+    _, steps, table_bytes = witness_program(plan)
+    nupd = int(steps["k"].sum())
+    return table_bytes + len(steps) * WSTEP_DTYPE.itemsize + 8 * max(1, nupd) + 3 * SLOT_ALIGN
