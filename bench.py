@@ -93,14 +93,14 @@ def dense_setup(L, torch):
 
 
 def dominant_pass(progs):
-    """The pass with the most bytes moved in the first program (the widest table), and its packed descriptor."""
-    from tropidist_amd import fused
-    prog, (segs, arr) = progs[0]
-    k = max(range(len(prog.passes)), key=lambda i: prog.passes[i].bytes_moved)
-    return prog.passes[k], arr[k:k + 1].copy()
+    """(kernel name, algorithmic bytes, descriptor record) of the pass with the most bytes moved in the first
+    program (the widest table)."""
+    prog = progs[0]
+    k = max(range(prog.npasses), key=prog.pass_bytes)
+    return prog.pass_kernel(k), prog.pass_bytes(k), prog.arr[k:k + 1].copy()
 
 
-def measured_traffic(kname, ps):
+def measured_traffic(kname, alg_bytes):
     """HBM bytes per launch of the dominant pass from the committed counter summary, when it was measured on the
     same kernel and pass geometry; None otherwise."""
     path = os.path.join(ROOT, "profiles", "pass_d5_banded_w25.json")
@@ -109,7 +109,7 @@ def measured_traffic(kname, ps):
             prof = json.load(f)
     except (OSError, ValueError):
         return None
-    if kname + "(" not in prof.get("kernel", "") or prof.get("alg_bytes_per_launch") != ps.bytes_moved:
+    if kname + "(" not in prof.get("kernel", "") or prof.get("alg_bytes_per_launch") != alg_bytes:
         return None
     return prof["traffic_bytes_per_launch"]
 
@@ -126,7 +126,8 @@ def bench_dense(args, L, torch):
 
     def step():
         vals = []
-        for pl, ops, (_, (segs, arr)) in zip(plans, encoded, progs):
+        for pl, ops, prog in zip(plans, encoded, progs):
+            segs, arr = prog.segs, prog.arr
             v = ctypes.c_int(-1)
             _lib.check(L.tdp_run_program(ops.ctypes.data_as(ctypes.c_void_p), len(ops),
                                          segs.ctypes.data_as(ctypes.c_void_p), len(segs),
@@ -156,14 +157,12 @@ def bench_dense(args, L, torch):
 
     # roofline of the dominant kernel: the widest fused pass, replayed on its own input (T0 -> T1); algorithmic
     # bytes = every input entry read once + every output entry written once
-    ps, desc = dominant_pass(progs)
-    kname = "tdp::k_pass<%d, %s, %s, unsigned int>" % (ps.D, "true" if ps.byte_store else "false",
-                                                     "true" if ps.runs()[2] else "false")
-    _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << ps.b_in, 7, sh), "fill")
+    kname, pass_bytes, desc = dominant_pass(progs)
+    d = desc[0]
+    _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << int(d["b_in"]), 7, sh), "fill")
     pass_s = _time_launches(torch, stream, lambda: _lib.check(
         L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "tdp_run_pass"),
         args.roofline_launches)
-    pass_bytes = ps.bytes_moved
     # the per-op streaming update at the peak table, for reference
     n = 1 << peak
     masks = [int(m) for c, m, _ in encoded[0].tolist() if c == 1 and (n >> 1) <= m < n][:1] or [n >> 1]
@@ -176,19 +175,19 @@ def bench_dense(args, L, torch):
                    "columns": p.ncols, "observables": len(plans), "peak_bits": peak, "work_per_step": total_work,
                    "distances": values, "distance": min(values), "host_planning_seconds": round(plan_seconds, 3),
                    "pass_compile_seconds": round(compile_seconds, 3),
-                   "passes_per_step": sum(len(pr.passes) for pr, _ in progs),
+                   "passes_per_step": sum(pr.npasses for pr in progs),
                    "executor": "fused passes (tdp_run_program), per-op kernels below 2^12 entries",
                    "per_op_executor": {"states_per_s": total_work / per_op_seconds,
                                        "ms_per_step": 1e3 * per_op_seconds, "distances": per_op_values,
                                        "identical": per_op_values == values}},
         "roofline": {"bound": "hbm", "achieved": pass_bytes / pass_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": pass_bytes / pass_s / HBM_PEAK, "traffic": measured_traffic(kname, ps),
+                     "frac": pass_bytes / pass_s / HBM_PEAK, "traffic": measured_traffic(kname, pass_bytes),
                      "traffic_source": "profiles/pass_d5_banded_w25.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled "
                                        "by a device copy of 2^26 bytes)",
                      "kernel": kname,
                      "launch_seconds": pass_s, "alg_bytes_per_launch": pass_bytes,
-                     "pass": {"b_in": ps.b_in, "b_out": ps.b_out, "updates": ps.D + len(ps.low_lam),
-                              "threads": 1 << ps.nfree},
+                     "pass": {"b_in": int(d["b_in"]), "b_out": int(d["b_out"]),
+                              "updates": int(d["D"]) + int(d["nlow"]), "threads": 1 << int(d["nfree"])},
                      "per_op_update": {"kernel": "tdp::k_update_hi", "table_entries": n, "launch_seconds": upd_s,
                                        "achieved": 2 * n / upd_s / 1e9, "frac": 2 * n / upd_s / HBM_PEAK}},
     }
@@ -274,19 +273,21 @@ def profile_pass(args):
     torch = _lib.torch()
     L = _lib.lib()
     p, plans, progs, peak, T0, T1, _, _ = dense_setup(L, torch)
-    ps, desc = dominant_pass(progs)
+    kname, alg_bytes, desc = dominant_pass(progs)
+    d = desc[0]
+    b_in = int(d["b_in"])
     stream = torch.cuda.current_stream()
     sh = _lib.stream_handle(stream)
-    _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << ps.b_in, 7, sh), "fill")
+    _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << b_in, 7, sh), "fill")
     for _ in range(args.profile_pass):
         _lib.check(L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "pass")
-    src = T0[:1 << ps.b_in]
-    dst = T1[:1 << ps.b_in]
+    src = T0[:1 << b_in]
+    dst = T1[:1 << b_in]
     for _ in range(args.profile_pass):
         dst.copy_(src)
     torch.cuda.synchronize()
-    print(json.dumps({"pass": {"b_in": ps.b_in, "b_out": ps.b_out, "D": ps.D, "threads": 1 << ps.nfree,
-                               "alg_bytes": ps.bytes_moved}, "copy_bytes": 1 << ps.b_in,
+    print(json.dumps({"pass": {"kernel": kname, "b_in": b_in, "b_out": int(d["b_out"]), "D": int(d["D"]),
+                               "threads": 1 << int(d["nfree"]), "alg_bytes": alg_bytes}, "copy_bytes": 1 << b_in,
                       "launches": args.profile_pass}))
 
 

@@ -92,6 +92,15 @@ int tdp_run_program(const uint64_t *ops, int64_t nops, const int64_t *segments, 
  * d_arena: tdp_program_witness_bytes(steps) bytes of device memory; used_ops: host array with room for every
  * update of the steps.  *value = the final entry (255 leaves *nused = 0). */
 int tdp_witness_step_bytes(void);
+
+/* Host compiler of fused programs (no device work): the segments (capacity seg_cap triples), pass descriptors
+ * (capacity pass_cap records) and, when steps is not NULL, the witness steps (capacity step_cap records) of an
+ * encoded op stream.  dmax <= 5 directions per pass, tables below 2^min_bits (>= 2) run per op, max_updates > 0
+ * caps the updates of a pass (the witness mode uses min_bits 2 and max_updates 6).  TDP_EINVAL when a column
+ * exceeds the pass limits on its own (callers then use tdp_run_plan).  No reference counterpart. */
+int tdp_fused_compile(const uint64_t *ops, int64_t nops, int dmax, int min_bits, int max_updates, int64_t *segments,
+                      int64_t seg_cap, int64_t *nseg, void *passes, int64_t pass_cap, int64_t *npasses, void *steps,
+                      int64_t step_cap, int64_t *nsteps);
 int tdp_program_witness_bytes(const void *steps, int64_t nsteps, uint64_t *bytes);
 int tdp_run_program_witness(const uint64_t *ops, int64_t nops, const void *steps, int64_t nsteps,
                             const void *passes, int64_t npasses, uint8_t *d_arena, uint64_t arena_bytes, int *value,

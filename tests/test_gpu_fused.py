@@ -85,8 +85,8 @@ def test_full_size_equals_per_op():
     p = problem.banded(R=120, N=600, kmin=2, kmax=6, W=25, nobs=1, rho=0.1, seed=1)
     plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
     assert plan.feasible and plan.peak_bits >= 24
-    prog, _ = fused.program_for(plan)
-    assert max(q.b_in for q in prog.passes) >= 24
+    prog = fused.program_for(plan)
+    assert max(int(r["b_in"]) for r in prog.arr[:prog.npasses]) >= 24
     assert fused.run_program(plan) == kernel.run_plan(plan, executor="per-op")[0]
 
 

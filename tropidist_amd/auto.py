@@ -113,7 +113,11 @@ def witness_plan(plan, order, max_bytes=None, stream=None, executor="fused"):
     if executor == "fused":
         from . import fused
         budget = FUSED_WITNESS_BYTES if max_bytes is None else max_bytes
-        if fused.witness_bytes(plan) <= budget:
+        try:
+            fits = fused.witness_bytes(plan) <= budget
+        except ValueError:              # a column beyond the pass limits: the per-op witness
+            fits = False
+        if fits:
             v, used = fused.witness(plan, stream=stream)
             return v, _columns_of(plan, order, used)
     L = _lib.lib()

@@ -1,11 +1,16 @@
-"""Batched certification of many dense distance problems in one persistent GPU launch.
+"""Batched certification of many dense distance problems: narrow plans in persistent GPU launches, wide plans by
+fused passes on concurrent streams.
 
     certify_batch(problems, orders=None)  -> one certificate record per problem (the fields of auto.certify)
 
-Pass 1 runs every (problem, observable) plan for its value; pass 2 runs, for each problem, the plan of its
-minimising observable again with the witness traceback.  Each pass is one launch of tdp_batch_run: a queue of plans
-sorted by decreasing work, one workgroup per plan (no launch per op, no host round trip per plan).  Witnesses are
-checked against H and L on the host.
+Narrow problems (every plan below 2^wide_bits entries): pass 1 runs every (problem, observable) plan for its value;
+pass 2 runs, for each problem, the plan of its minimising observable again with the witness traceback.  Each pass
+is one launch of tdp_batch_run: a queue of plans sorted by decreasing work, a group of workgroups per plan (no
+launch per op, no host round trip per plan).
+
+Wide problems: worker threads, each with its own HIP stream, take whole problems (largest first): the value of
+every observable by the fused executor, then the witness of the minimising one by the fused witness.  They run
+while the narrow launches occupy the device.  Witnesses are checked against H and L on the host.
 
 The per-plan checkpoint interval S and the number of resident workgroups come from tropidist_amd.autotune (the
 tuned value for the plan's width bucket, or sqrt(8 * ops) and 2 workgroups per CU by default)."""
@@ -185,11 +190,98 @@ def plan_problems(problems, orders=None):
     return orders, flat, owner
 
 
-def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=None, stream=None):
-    """The device part of certify_batch on plans made by plan_problems."""
+# Problems with a plan of at least 2^WIDE_BITS entries go to the fused executor; WIDE_STREAMS workers run them.
+WIDE_BITS = 18
+WIDE_STREAMS = 3
+
+
+def _certify_wide(problem, order, items, stream=None):
+    """Certificate record of one problem by the fused executor and the fused witness on `stream`
+    (items: (observable, plan) pairs)."""
+    from . import kernel
+    rec = {"distance": None, "per_observable": [None] * problem.nobs, "lower_bound": None, "upper_bound": None,
+           "witness": None, "observable": None, "witness_ok": False}
+    for i, plan in items:
+        rec["per_observable"][i] = kernel.run_plan(plan, stream=stream)[0]
+    vals = [v for v in rec["per_observable"] if v is not None]
+    if vals:
+        best = min(vals)
+        obs = rec["per_observable"].index(best)
+        v, cols = auto.witness_plan(dict(items)[obs], order, stream=stream)
+        ok = v == best and verify_witness(problem, cols, v, obs)
+        rec.update(distance=best, lower_bound=best, observable=obs, witness=cols, witness_ok=ok,
+                   upper_bound=problem.cost(cols))
+    return rec
+
+
+def _run_wide(problems, orders, by_problem, pis, nstreams, recs, errors):
+    """Worker threads over the wide problems (largest work first), each on its own stream."""
+    t = _lib.torch()
+    queue = sorted(pis, key=lambda pi: -sum(pl.work() for _, pl in by_problem[pi] if pl.feasible))
+    lock = threading.Lock()
+
+    def worker():
+        st = t.cuda.Stream()
+        with t.cuda.stream(st):
+            while True:
+                with lock:
+                    if not queue or errors:
+                        return
+                    pi = queue.pop(0)
+                try:
+                    recs[pi] = _certify_wide(problems[pi], orders[pi], by_problem[pi], stream=st)
+                except BaseException as e:          # re-raised by the caller
+                    errors.append(e)
+                    return
+
+    threads = [threading.Thread(target=worker, daemon=True) for _ in range(max(1, nstreams))]
+    for th in threads:
+        th.start()
+    return threads
+
+
+def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=None, stream=None,
+                    wide_bits=WIDE_BITS, wide_streams=WIDE_STREAMS):
+    """The device part of certify_batch on plans made by plan_problems.  wide_streams = 0 runs the wide problems on
+    the calling thread after the narrow launches; wide_bits = None keeps every problem in the persistent launches."""
+    by_problem = {}
+    for plan, (pi, i) in zip(flat, owner):
+        by_problem.setdefault(pi, []).append((i, plan))
+    wide = set()
+    if wide_bits is not None:
+        wide = {pi for pi, items in by_problem.items()
+                if any(pl.feasible and pl.peak_bits >= wide_bits for _, pl in items)}
+    recs = [None] * len(problems)
+    errors, threads = [], []
+    if wide and wide_streams > 0:
+        threads = _run_wide(problems, orders, by_problem, wide, wide_streams, recs, errors)
+    try:
+        keep = [k for k, (pi, _) in enumerate(owner) if pi not in wide]
+        narrow = _certify_narrow(problems, orders, [flat[k] for k in keep], [owner[k] for k in keep],
+                                 workgroups, intervals, stream)
+        for pi, rec in narrow.items():
+            recs[pi] = rec
+        if wide and wide_streams <= 0:
+            for pi in sorted(wide):
+                recs[pi] = _certify_wide(problems[pi], orders[pi], by_problem[pi], stream=stream)
+    finally:
+        for th in threads:
+            th.join()
+    if errors:
+        raise errors[0]
+    for pi, p in enumerate(problems):
+        if recs[pi] is None:
+            recs[pi] = {"distance": None, "per_observable": [None] * p.nobs, "lower_bound": None,
+                        "upper_bound": None, "witness": None, "observable": None, "witness_ok": False}
+    return recs
+
+
+def _certify_narrow(problems, orders, flat, owner, workgroups, intervals, stream):
+    """Records {problem index: record} of the problems owning `flat`, by the two persistent launches."""
     values = run_plans(flat, witness=False, workgroups=workgroups, stream=stream)
-    recs = [{"distance": None, "per_observable": [None] * p.nobs, "lower_bound": None, "upper_bound": None,
-             "witness": None, "observable": None, "witness_ok": False} for p in problems]
+    recs = {pi: {"distance": None, "per_observable": [None] * problems[pi].nobs, "lower_bound": None,
+                 "upper_bound": None, "witness": None, "observable": None, "witness_ok": False}
+            for pi, _ in owner}
     for (pi, i), (v, _) in zip(owner, values):
         if v is not None:
             if v == 255:
@@ -197,7 +289,7 @@ def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=No
             recs[pi]["per_observable"][i] = v
     where = {o: k for k, o in enumerate(owner)}
     wplans, wowner = [], []
-    for pi, rec in enumerate(recs):
+    for pi, rec in sorted(recs.items()):
         vals = [v for v in rec["per_observable"] if v is not None]
         if vals:
             best = min(vals)
@@ -216,8 +308,8 @@ def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=No
 
 
 def certify_planned_one_at_a_time(problems, orders, flat, owner):
-    """The plain-GPU baseline on the same plans: each problem alone, one kernel per op (kernel.run_plan) and the
-    per-op checkpointed witness (auto.witness_plan)."""
+    """The plain-GPU baseline on the same plans: each problem alone and synchronously, by the single-plan executors
+    (kernel.run_plan and auto.witness_plan, fused by default)."""
     from . import kernel
     recs = [{"distance": None, "per_observable": [None] * p.nobs, "witness": None, "witness_ok": False}
             for p in problems]

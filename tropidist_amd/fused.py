@@ -35,6 +35,7 @@ MAX_FREE = 48
 MAX_LOW = 8
 MAX_ROWS = 32                   # rows starting / ending in one pass (32-bit coordinate masks)
 MAX_RUNS = 12                   # runs of consecutive free variables per index map (see address_runs)
+MAX_VARS = 64                   # input bits plus opened rows of one pass (64-bit variable masks)
 
 
 @dataclass
@@ -201,7 +202,7 @@ def _build_pass(enc, first, last, in_log, logical_out):
         lam = m & low
         if mh:
             dirs.append((mh, lam, w))
-        else:
+        elif m:                     # a zero mask changes nothing
             lows.append((lam, w))
     D = len(dirs)
     # pivots: new variables first, then the highest input bits
@@ -329,6 +330,8 @@ def compile_program(plan, dmax=DMAX, min_bits=12, max_updates=None):
                 break
             if max_updates is not None and m and D + nlow + 1 > max_updates:
                 break
+            if nv2 > MAX_VARS:
+                break
             nnew += ne
             nsel += ns
             # accept the column
@@ -347,8 +350,8 @@ def compile_program(plan, dmax=DMAX, min_bits=12, max_updates=None):
                     lay.pop(a)
                     bits_now -= 1
             kk += 1
-        if kk == k:             # a column that cannot start a pass on its own (dependent mask): one-op pass
-            kk = k + 1
+        if kk == k:             # a column over the limits on its own: no pass can hold it
+            raise ValueError("plan column at op %d exceeds the fused pass limits" % i0)
         while True:
             last_op = cols[kk - 1][1]
             b_out = _bits_after(enc, b, start, last_op)
@@ -519,21 +522,50 @@ def pack(program):
     return segs, arr
 
 
+class NativeProgram:
+    """A compiled program as the executor reads it: segments and pass descriptors (and witness steps)."""
+
+    def __init__(self, segs, arr, steps=None):
+        self.segs, self.arr, self.steps = segs, arr, steps
+        self.npasses = int((segs[:, 0] == SEG_PASS).sum()) if len(segs) else 0
+
+    def pass_bytes(self, i):
+        r = self.arr[i]
+        return (1 << int(r["b_in"])) + (1 << int(r["b_out"]))
+
+    def pass_kernel(self, i):
+        r = self.arr[i]
+        return "tdp::k_pass<%d, %s, %s, %s>" % (int(r["D"]), "true" if r["byte_store"] else "false",
+                                                "true" if r["nruns"][2] > 0 else "false",
+                                                "unsigned int" if max(r["b_in"], r["b_out"]) <= 32
+                                                else "unsigned long")
+
+
 _compiled = {}
 
 
-def program_for(plan, dmax=DMAX, min_bits=12):
-    """compile_program with a per-plan cache (keyed by the plan object)."""
-    key = (id(plan), dmax, min_bits)
-    hit = _compiled.get(key)
+def _cached(plan, key, make):
+    k = (id(plan),) + key
+    hit = _compiled.get(k)
     if hit is not None and hit[0] is plan:
         return hit[1]
-    prog = compile_program(plan, dmax=dmax, min_bits=min_bits)
-    packed = pack(prog)
-    if len(_compiled) > 256:
+    val = make()
+    if len(_compiled) > 512:
         _compiled.clear()
-    _compiled[key] = (plan, (prog, packed))
-    return prog, packed
+    _compiled[k] = (plan, val)
+    return val
+
+
+def program_for(plan, dmax=DMAX, min_bits=12):
+    """The native program of a plan, cached per plan object.  ValueError if a column exceeds the pass limits."""
+    return _cached(plan, ("run", dmax, min_bits),
+                   lambda: NativeProgram(*compile_native(plan, dmax=dmax, min_bits=min_bits)))
+
+
+def witness_program_for(plan):
+    """The native witness-mode program of a plan, cached per plan object."""
+    return _cached(plan, ("witness",), lambda: NativeProgram(*compile_native(
+        plan, min_bits=2, max_updates=WITNESS_MAX_UPDATES, witness=True)))
 
 
 def run_program(plan, program=None, T0=None, T1=None, stream=None, raw=False):
@@ -548,7 +580,8 @@ def run_program(plan, program=None, T0=None, T1=None, stream=None, raw=False):
     from .trellis import as_plan
     plan = as_plan(plan)
     if program is None:
-        _, (segs, arr) = program_for(plan)
+        prog = program_for(plan)
+        segs, arr = prog.segs, prog.arr
     else:
         segs, arr = pack(program)
     ops = plan.encode()
@@ -711,20 +744,18 @@ def witness(plan, max_bytes=None, stream=None):
     L = _lib.lib()
     if int(L.tdp_witness_step_bytes()) != WSTEP_DTYPE.itemsize:
         raise RuntimeError("witness step layout mismatch")
-    prog, steps, _ = witness_program(plan)
-    nbytes = ctypes.c_uint64(0)
-    _lib.check(L.tdp_program_witness_bytes(steps.ctypes.data_as(ctypes.c_void_p), len(steps), ctypes.byref(nbytes)),
-               "tdp_program_witness_bytes")
-    if max_bytes is not None and nbytes.value > max_bytes:
+    prog = witness_program_for(plan)
+    steps, arr = prog.steps, prog.arr
+    nbytes = witness_bytes(plan)
+    if max_bytes is not None and nbytes > max_bytes:
         return None
-    _, arr = pack(prog)
     ops = plan.encode()
-    arena = _lib.device_empty(nbytes.value)
+    arena = _lib.device_empty(nbytes)
     used = np.zeros(max(1, int(steps["k"].sum())), dtype=np.int64)
     value, nused = ctypes.c_int(-1), ctypes.c_int64(0)
     rc = L.tdp_run_program_witness(ops.ctypes.data_as(ctypes.c_void_p), len(ops),
                                    steps.ctypes.data_as(ctypes.c_void_p), len(steps),
-                                   arr.ctypes.data_as(ctypes.c_void_p), len(arr), _lib.dptr(arena), nbytes.value,
+                                   arr.ctypes.data_as(ctypes.c_void_p), len(arr), _lib.dptr(arena), nbytes,
                                    ctypes.byref(value), used.ctypes.data_as(ctypes.c_void_p), ctypes.byref(nused),
                                    _lib.stream_handle(stream))
     _lib.check(rc, "tdp_run_program_witness")
@@ -733,7 +764,41 @@ def witness(plan, max_bytes=None, stream=None):
 
 
 def witness_bytes(plan):
-    """Device bytes of the witness-mode arena of a plan (tables of every step plus the traceback's records)."""
-    _, steps, table_bytes = witness_program(plan)
-    nupd = int(steps["k"].sum())
-    return table_bytes + len(steps) * WSTEP_DTYPE.itemsize + 8 * max(1, nupd) + 3 * SLOT_ALIGN
+    """Device bytes of the witness-mode arena of a plan (tables of every step plus the traceback's records).
+    ValueError if a column exceeds the pass limits."""
+    import ctypes
+    from . import _lib
+    steps = witness_program_for(plan).steps
+    nbytes = ctypes.c_uint64(0)
+    _lib.check(_lib.lib().tdp_program_witness_bytes(steps.ctypes.data_as(ctypes.c_void_p), len(steps),
+                                                    ctypes.byref(nbytes)), "tdp_program_witness_bytes")
+    return nbytes.value
+
+
+# ------------------------------------------------------------------------------------------ native compiler
+def compile_native(plan, dmax=DMAX, min_bits=12, max_updates=None, witness=False):
+    """(segments, pass descriptors[, witness steps]) by libtdp's C++ compiler (tdp_fused_compile): the same arrays as
+    pack(compile_program(...)) and witness_program(...), about a hundred times faster.  ValueError when a column
+    exceeds the pass limits."""
+    import ctypes
+    from . import _lib
+    L = _lib.lib()
+    enc = np.ascontiguousarray(plan.encode(), dtype=np.uint64)
+    nops = len(enc)
+    segs = np.zeros((nops + 2, 3), dtype=np.int64)
+    arr = np.zeros(max(1, nops + 1), dtype=PASS_DTYPE)
+    steps = np.zeros(nops + 2, dtype=WSTEP_DTYPE) if witness else None
+    ns, npass, nst = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
+    rc = L.tdp_fused_compile(enc.ctypes.data_as(ctypes.c_void_p), nops, dmax, min_bits, max_updates or 0,
+                             segs.ctypes.data_as(ctypes.c_void_p), len(segs), ctypes.byref(ns),
+                             arr.ctypes.data_as(ctypes.c_void_p), len(arr), ctypes.byref(npass),
+                             steps.ctypes.data_as(ctypes.c_void_p) if witness else None,
+                             len(steps) if witness else 0, ctypes.byref(nst))
+    if rc == 1:
+        raise ValueError(L.tdp_last_error().decode(errors="replace"))
+    _lib.check(rc, "tdp_fused_compile")
+    segs = segs[:ns.value]
+    arr = arr[:max(1, npass.value)]
+    if witness:
+        return segs, arr, steps[:nst.value]
+    return segs, arr

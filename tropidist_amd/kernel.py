@@ -48,6 +48,11 @@ def run_plan(plan, log=None, progress_every=0.0, stream=None, executor="fused"):
     t0 = time.time()
     if executor == "fused":
         from . import fused
+        try:
+            fused.program_for(plan)
+        except ValueError:              # a column beyond the pass limits: the per-op executor
+            executor = "per-op"
+    if executor == "fused":
         raw = fused.run_program(plan, None, T0, T1, stream=stream, raw=True)
     else:
         ops = plan.encode()
