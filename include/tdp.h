@@ -93,6 +93,35 @@ int tdp_run_program(const uint64_t *ops, int64_t nops, const int64_t *segments, 
  * update of the steps.  *value = the final entry (255 leaves *nused = 0). */
 int tdp_witness_step_bytes(void);
 
+/* Many programs at once: each record is a value run (segments + passes on d_T0/d_T1 of 2^peak_bits bytes, as
+ * tdp_run_program) or, when steps is not NULL, a witness run (as tdp_run_program_witness, on d_arena with room
+ * for used_ops in the host array).  One launch per round runs the next pass of every active program, so programs
+ * whose passes are too small to fill the device alone share it; per-op segments and steps are launched between the
+ * rounds.  d_work: tdp_run_programs_workspace(runs) bytes.  Outputs: value (the final entry, 255 = cost >= 255),
+ * nused / used_ops for witness runs.  Tables of passes must hold at most 2^32 entries.  No reference counterpart
+ * (the reference runs one plan at a time, cdist/auto.py:105-123). */
+typedef struct {
+    const uint64_t *ops;
+    int64_t nops;
+    const int64_t *segments;
+    int64_t nseg;
+    const void *passes;
+    int64_t npasses;
+    const void *steps;
+    int64_t nsteps;
+    uint8_t *d_T0;
+    uint8_t *d_T1;
+    uint8_t *d_arena;
+    uint64_t arena_bytes;
+    int64_t *used_ops;
+    int64_t nused;
+    int32_t peak_bits;
+    int32_t value;
+} tdp_run_t;
+int tdp_run_record_bytes(void);
+int tdp_run_programs_workspace(const tdp_run_t *runs, int nruns, uint64_t *bytes);
+int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *stream);
+
 /* Host compiler of fused programs (no device work): the segments (capacity seg_cap triples), pass descriptors
  * (capacity pass_cap records) and, when steps is not NULL, the witness steps (capacity step_cap records) of an
  * encoded op stream.  dmax <= 5 directions per pass, tables below 2^min_bits (>= 2) run per op, max_updates > 0

@@ -87,8 +87,8 @@ def test_banded_weighted_layouts():
 
 
 def test_pack_layout():
-    """Descriptor records: size of the C struct (416 bytes), fields where the executor reads them."""
-    assert fused.PASS_DTYPE.itemsize == 416
+    """Descriptor records: size of the C struct (496 bytes), fields where the executor reads them."""
+    assert fused.PASS_DTYPE.itemsize == 496
     p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=5)
     plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
     prog = fused.compile_program(plan, min_bits=4)
@@ -105,8 +105,7 @@ def test_pack_layout():
             assert list(r["din"][:q.D + 2]) == q.din
             for m, runs in enumerate(q.runs()):
                 assert r["nruns"][m] == len(runs)
-                got = [[int(r["run_f0"][m, i]), int(r["run_len"][m, i]), int(r["run_sh"][m, i])]
-                       for i in range(len(runs))]
+                got = [fused.unpack_run(r["runs"][m, i]) for i in range(len(runs))]
                 assert got == runs
             # the runs reproduce the per-variable maps
             tid = np.arange(1 << q.nfree, dtype=np.int64)

@@ -51,17 +51,17 @@ def test_batch_chain_boundary():
         batch.certify_batch([problem.chain(255)], [list(range(255))])
 
 
-@pytest.mark.parametrize("wide_streams", [0, 3])
-def test_hybrid_wide_path_equals_persistent_path(wide_streams):
+@pytest.mark.parametrize("concurrent", [False, True])
+def test_hybrid_wide_path_equals_persistent_path(concurrent):
     """Problems split between the persistent launches and the fused executor on worker streams (wide_bits lowered
     so both paths carry work): the same values as the persistent path alone, checked witnesses."""
-    print("case batch hybrid streams", wide_streams, flush=True)
+    print("case batch hybrid concurrent", concurrent, flush=True)
     rng = np.random.default_rng(11)
     probs = [workloads.banded_instance(rng, 7100 + i, int(rng.integers(8, 19)), bool(i % 2)) for i in range(40)]
     orders, flat, owner = batch.plan_problems(probs, None)
     assert min(pl.peak_bits for pl in flat if pl.feasible) < 14 <= max(pl.peak_bits for pl in flat if pl.feasible)
     ref = batch.certify_planned(probs, orders, flat, owner, wide_bits=None)
-    got = batch.certify_planned(probs, orders, flat, owner, wide_bits=14, wide_streams=wide_streams)
+    got = batch.certify_planned(probs, orders, flat, owner, wide_bits=14, wide_concurrent=concurrent)
     for p, a, b in zip(probs, ref, got):
         assert a["per_observable"] == b["per_observable"], p.name
         if b["distance"] is not None:
@@ -76,4 +76,4 @@ def test_hybrid_overflow_from_a_wide_worker():
     probs = [p, problem.chain(30)]
     orders, flat, owner = batch.plan_problems(probs, [list(range(q.ncols)) for q in probs])
     with pytest.raises(OverflowError):
-        batch.certify_planned(probs, orders, flat, owner, wide_bits=4, wide_streams=2)
+        batch.certify_planned(probs, orders, flat, owner, wide_bits=4)

@@ -112,7 +112,7 @@ def test_bad_descriptor_rejected_on_host():
     arr = arr.copy()
     assert arr[0]["nfree"] >= 1
     arr[0]["nruns"][0] = 1
-    arr[0]["run_f0"][0, 0], arr[0]["run_len"][0, 0], arr[0]["run_sh"][0, 0] = 0, 1, 38
+    arr[0]["runs"][0, 0] = 0 | 1 << 8 | 38 << 16
     L = _lib.lib()
     ops = plan.encode()
     T0 = _lib.device_empty(1 << plan.peak_bits)
@@ -197,3 +197,47 @@ def test_fused_witness_overflow():
     plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows, weights=[200] * p.ncols)
     with pytest.raises(OverflowError):
         fused.witness(plan)
+
+
+def test_run_many_equals_single_runs():
+    """tdp_run_programs on a mix of value and witness runs of different widths (the rounds interleave their
+    passes and per-op steps): every value equals the single-program executor's, every witness is valid."""
+    from tropidist_amd.problem import verify_witness
+    print("case run_many mixed", flush=True)
+    rng = random.Random(77)
+    plans, probs, orders = [], [], []
+    for t in range(30):
+        m, k, h, l = random_problem(rng)
+        order = list(range(len(h)))
+        rng.shuffle(order)
+        plan = trellis.build_plan(h, l, 1, order, m)
+        if plan.feasible:
+            plans.append(plan)
+            probs.append(problem.Problem("r%d" % t, m, k, h, l))
+            orders.append(order)
+    for W, weights in ((10, None), (14, (1, 3)), (18, None), (20, (1, 2))):
+        p = problem.banded(R=100, N=500, kmin=2, kmax=6, W=W, nobs=1, rho=0.1, weights=weights, seed=300 + W)
+        order = list(range(p.ncols))
+        plan = trellis.build_plan(p.h, p.l, 1, order, p.nrows, weights=p.w)
+        assert plan.feasible
+        plans.append(plan)
+        probs.append(p)
+        orders.append(order)
+    witness = [i % 2 == 1 for i in range(len(plans))]
+    got = fused.run_many(plans, witness)
+    for i, (plan, p, order, w, g) in enumerate(zip(plans, probs, orders, witness, got)):
+        want = fused.run_program(plan)
+        if w:
+            v, used = g
+            assert v == want, i
+            assert verify_witness(p, _cols(plan, order, used), v, 0), i
+        else:
+            assert g == want, i
+
+
+def test_run_many_overflow():
+    p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=5)
+    bad = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows, weights=[200] * p.ncols)
+    good = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+    with pytest.raises(OverflowError):
+        fused.run_many([good, bad])

@@ -34,3 +34,28 @@ def test_bindings_cover_the_header():
 
 def test_version_string_without_a_device():
     assert b"gfx950" in _lib.lib().tdp_version()
+
+
+def test_multi_program_records_and_validation_without_a_device():
+    """tdp_run_programs_workspace plans the rounds on the host only: record layout, and a descriptor addressing
+    outside its table is refused before any device work."""
+    import ctypes
+    import numpy as np
+    from tropidist_amd import _lib, fused, problem, trellis
+    L = _lib.lib()
+    TdpRun = fused._run_record_type()
+    assert L.tdp_run_record_bytes() == ctypes.sizeof(TdpRun) == 120
+    p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=5)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+    segs, arr = fused.compile_native(plan, min_bits=4)
+    ops = np.ascontiguousarray(plan.encode())
+    runs = (TdpRun * 1)()
+    r = runs[0]
+    r.ops, r.nops, r.segments, r.nseg = ops.ctypes.data, len(ops), segs.ctypes.data, len(segs)
+    r.passes, r.npasses = arr.ctypes.data, int((segs[:, 0] == 1).sum())
+    r.d_T0, r.d_T1, r.peak_bits = 1 << 20, 1 << 21, plan.peak_bits      # never dereferenced here
+    nb = ctypes.c_uint64(0)
+    assert L.tdp_run_programs_workspace(runs, 1, ctypes.byref(nb)) == 0 and nb.value > 0
+    arr[0]["runs"][0, 0] = 0 | 1 << 8 | 38 << 16
+    arr[0]["nruns"][0] = max(1, int(arr[0]["nruns"][0]))
+    assert L.tdp_run_programs_workspace(runs, 1, ctypes.byref(nb)) == 1

@@ -8,9 +8,9 @@ pass 2 runs, for each problem, the plan of its minimising observable again with 
 is one launch of tdp_batch_run: a queue of plans sorted by decreasing work, a group of workgroups per plan (no
 launch per op, no host round trip per plan).
 
-Wide problems: worker threads, each with its own HIP stream, take whole problems (largest first): the value of
-every observable by the fused executor, then the witness of the minimising one by the fused witness.  They run
-while the narrow launches occupy the device.  Witnesses are checked against H and L on the host.
+Wide problems: the multi-program fused executor (fused.run_many) on a stream of its own, while the narrow launches
+occupy the device: one launch per round runs the next pass of every wide plan, first the values of every
+observable, then the witnesses of the minimising ones.  Witnesses are checked against H and L on the host.
 
 The per-plan checkpoint interval S and the number of resident workgroups come from tropidist_amd.autotune (the
 tuned value for the plan's width bucket, or sqrt(8 * ops) and 2 workgroups per CU by default)."""
@@ -190,60 +190,85 @@ def plan_problems(problems, orders=None):
     return orders, flat, owner
 
 
-# Problems with a plan of at least 2^WIDE_BITS entries go to the fused executor; WIDE_STREAMS workers run them.
+# Problems with a plan of at least 2^WIDE_BITS entries go to the multi-program fused executor; each of its calls
+# holds at most WIDE_BUDGET device bytes of tables and witness arenas.
 WIDE_BITS = 18
-WIDE_STREAMS = 3
+WIDE_BUDGET = 48 << 30
 
 
-def _certify_wide(problem, order, items, stream=None):
-    """Certificate record of one problem by the fused executor and the fused witness on `stream`
-    (items: (observable, plan) pairs)."""
-    from . import kernel
-    rec = {"distance": None, "per_observable": [None] * problem.nobs, "lower_bound": None, "upper_bound": None,
-           "witness": None, "observable": None, "witness_ok": False}
-    for i, plan in items:
-        rec["per_observable"][i] = kernel.run_plan(plan, stream=stream)[0]
-    vals = [v for v in rec["per_observable"] if v is not None]
-    if vals:
-        best = min(vals)
-        obs = rec["per_observable"].index(best)
-        v, cols = auto.witness_plan(dict(items)[obs], order, stream=stream)
-        ok = v == best and verify_witness(problem, cols, v, obs)
-        rec.update(distance=best, lower_bound=best, observable=obs, witness=cols, witness_ok=ok,
-                   upper_bound=problem.cost(cols))
-    return rec
+def _chunks(items, cost, budget):
+    out, cur, used = [], [], 0
+    for it in items:
+        c = cost(it)
+        if cur and used + c > budget:
+            out.append(cur)
+            cur, used = [], 0
+        cur.append(it)
+        used += c
+    if cur:
+        out.append(cur)
+    return out
 
 
-def _run_wide(problems, orders, by_problem, pis, nstreams, recs, errors):
-    """Worker threads over the wide problems (largest work first), each on its own stream."""
-    t = _lib.torch()
-    queue = sorted(pis, key=lambda pi: -sum(pl.work() for _, pl in by_problem[pi] if pl.feasible))
-    lock = threading.Lock()
+def _certify_wide(problems, orders, by_problem, pis, budget, stream=None):
+    """Records {problem index: record} of the wide problems: the values of every observable's plan, then the
+    witnesses of the minimising ones, each as rounds of the multi-program fused executor (fused.run_many); plans
+    whose columns exceed the pass limits run alone on the per-op executor."""
+    from . import fused, kernel
+    recs = {pi: {"distance": None, "per_observable": [None] * problems[pi].nobs, "lower_bound": None,
+                 "upper_bound": None, "witness": None, "observable": None, "witness_ok": False} for pi in pis}
+    items = [(pi, i, pl) for pi in pis for i, pl in by_problem[pi] if pl.feasible]
+    multi = []
+    for it in items:
+        try:
+            fused.program_for(it[2])
+            multi.append(it)
+        except ValueError:
+            recs[it[0]]["per_observable"][it[1]] = kernel.run_plan(it[2], stream=stream, executor="per-op")[0]
+    multi.sort(key=lambda it: -it[2].work())
+    for chunk in _chunks(multi, lambda it: 2 << it[2].peak_bits, budget):
+        for (pi, i, _), v in zip(chunk, fused.run_many([it[2] for it in chunk], stream=stream)):
+            recs[pi]["per_observable"][i] = v
+    wit = []
+    for pi in pis:
+        rec = recs[pi]
+        vals = [v for v in rec["per_observable"] if v is not None]
+        if vals:
+            best = min(vals)
+            obs = rec["per_observable"].index(best)
+            rec.update(distance=best, lower_bound=best, observable=obs)
+            wit.append((pi, dict(by_problem[pi])[obs]))
 
-    def worker():
-        st = t.cuda.Stream()
-        with t.cuda.stream(st):
-            while True:
-                with lock:
-                    if not queue or errors:
-                        return
-                    pi = queue.pop(0)
-                try:
-                    recs[pi] = _certify_wide(problems[pi], orders[pi], by_problem[pi], stream=st)
-                except BaseException as e:          # re-raised by the caller
-                    errors.append(e)
-                    return
+    def settle(pi, v, cols):
+        rec = recs[pi]
+        ok = v == rec["distance"] and verify_witness(problems[pi], cols, v, rec["observable"])
+        rec.update(witness=cols, witness_ok=ok, upper_bound=problems[pi].cost(cols))
 
-    threads = [threading.Thread(target=worker, daemon=True) for _ in range(max(1, nstreams))]
-    for th in threads:
-        th.start()
-    return threads
+    inmulti = []
+    for pi, plan in wit:
+        try:
+            nb = fused.witness_bytes(plan)
+        except ValueError:
+            nb = None
+        if nb is None or nb > budget:
+            v, cols = auto.witness_plan(plan, orders[pi], stream=stream,
+                                        executor="per-op" if nb is None else "fused")
+            settle(pi, v, cols)
+        else:
+            inmulti.append((pi, plan, nb))
+    inmulti.sort(key=lambda t: -t[2])
+    for chunk in _chunks(inmulti, lambda t: t[2], budget):
+        res = fused.run_many([t[1] for t in chunk], witness=[True] * len(chunk), stream=stream)
+        for (pi, plan, _), (v, used) in zip(chunk, res):
+            settle(pi, v, _columns(plan, orders[pi], used))
+    return recs
 
 
 def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=None, stream=None,
-                    wide_bits=WIDE_BITS, wide_streams=WIDE_STREAMS):
-    """The device part of certify_batch on plans made by plan_problems.  wide_streams = 0 runs the wide problems on
-    the calling thread after the narrow launches; wide_bits = None keeps every problem in the persistent launches."""
+                    wide_bits=WIDE_BITS, wide_concurrent=True, wide_budget=WIDE_BUDGET):
+    """The device part of certify_batch on plans made by plan_problems.  Problems with a plan of 2^wide_bits
+    entries or more run on the multi-program fused executor (on a stream of their own, alongside the persistent
+    launches, when wide_concurrent); wide_bits = None keeps every problem in the persistent launches."""
     by_problem = {}
     for plan, (pi, i) in zip(flat, owner):
         by_problem.setdefault(pi, []).append((i, plan))
@@ -252,23 +277,35 @@ def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=No
         wide = {pi for pi, items in by_problem.items()
                 if any(pl.feasible and pl.peak_bits >= wide_bits for _, pl in items)}
     recs = [None] * len(problems)
-    errors, threads = [], []
-    if wide and wide_streams > 0:
-        threads = _run_wide(problems, orders, by_problem, wide, wide_streams, recs, errors)
+    result, errors, th = {}, [], None
+    if wide and wide_concurrent:
+        t = _lib.torch()
+
+        def worker():
+            try:
+                st = t.cuda.Stream()
+                with t.cuda.stream(st):
+                    result.update(_certify_wide(problems, orders, by_problem, sorted(wide), wide_budget, st))
+            except BaseException as e:          # re-raised by the caller
+                errors.append(e)
+
+        th = threading.Thread(target=worker, daemon=True)
+        th.start()
     try:
         keep = [k for k, (pi, _) in enumerate(owner) if pi not in wide]
         narrow = _certify_narrow(problems, orders, [flat[k] for k in keep], [owner[k] for k in keep],
                                  workgroups, intervals, stream)
         for pi, rec in narrow.items():
             recs[pi] = rec
-        if wide and wide_streams <= 0:
-            for pi in sorted(wide):
-                recs[pi] = _certify_wide(problems[pi], orders[pi], by_problem[pi], stream=stream)
+        if wide and not wide_concurrent:
+            result.update(_certify_wide(problems, orders, by_problem, sorted(wide), wide_budget, stream))
     finally:
-        for th in threads:
+        if th is not None:
             th.join()
     if errors:
         raise errors[0]
+    for pi, rec in result.items():
+        recs[pi] = rec
     for pi, p in enumerate(problems):
         if recs[pi] is None:
             recs[pi] = {"distance": None, "per_observable": [None] * p.nobs, "lower_bound": None,

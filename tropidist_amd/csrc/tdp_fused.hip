@@ -15,24 +15,49 @@
 // base_out(t) ^ XOR dout[...] when the ending rows' coordinates equal their targets.
 #include "tdp_common.h"
 
+#include <algorithm>
+#include <vector>
+
 namespace tdp {
 
 constexpr int kPassBlock = 256;
 constexpr int kMaxRuns = 12;
 enum { MAP_IN = 0, MAP_OUT = 1, MAP_NEW = 2, MAP_SEL = 3 };
 
-// A thread's base indices: map m = OR over runs r < nruns[m] of ((tid >> f0) & (2^len - 1)) << sh.
+// A thread's base indices: map m = OR over runs r < nruns[m] of ((tid >> f0) & (2^len - 1)) << sh, each run packed as
+// f0 | len << 8 | sh << 16.  Every field is a naturally aligned 32- or 64-bit word, so the whole descriptor is read
+// with scalar loads.
 struct PassArgs {
     uint64_t din[8], dout[8];               // per basis vector (e_0, e_1, m'_0 .. m'_{D-1}): index deltas
     uint32_t dnew[8], dsel[8];              // new-row / ending-row coordinate deltas
-    uint8_t run_f0[4][kMaxRuns], run_len[4][kMaxRuns], run_sh[4][kMaxRuns];
+    uint32_t runs[4][kMaxRuns];
     int32_t nruns[4];
     uint32_t sel_target;
     int32_t D, nlow, byte_store, b_in, b_out, nfree, pad;
-    uint8_t lam[8], w[8], low_lam[8], low_w[8];
+    uint32_t lamw[8];                       // update j: lane mask | weight << 8
+    uint32_t loww[8];                       // in-dword update t: lane mask | weight << 8
 };
+static_assert(sizeof(PassArgs) == 496, "pass descriptor layout");
+
+__device__ __forceinline__ uint32_t run_f0(uint32_t r) { return r & 0xFFu; }
+__device__ __forceinline__ uint32_t run_len(uint32_t r) { return (r >> 8) & 0xFFu; }
+__device__ __forceinline__ uint32_t run_sh(uint32_t r) { return (r >> 16) & 0xFFu; }
 
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Table accesses through the global address space (tables are device memory; the multi-program kernel reads its
+// table pointers from memory, where the compiler could not tell them from generic pointers).
+typedef __attribute__((address_space(1))) uint32_t gu32;
+typedef __attribute__((address_space(1))) uint8_t gu8;
+__device__ __forceinline__ uint32_t gload32(const uint8_t *base, uint64_t off) {
+    return *reinterpret_cast<const gu32 *>(reinterpret_cast<uintptr_t>(base) + off);
+}
+__device__ __forceinline__ void gstore32(uint8_t *base, uint64_t off, uint32_t v) {
+    *reinterpret_cast<gu32 *>(reinterpret_cast<uintptr_t>(base) + off) = v;
+}
+__device__ __forceinline__ void gstore8(uint8_t *base, uint64_t off, uint8_t v) {
+    *reinterpret_cast<gu8 *>(reinterpret_cast<uintptr_t>(base) + off) = v;
+}
 
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
 __device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
@@ -55,10 +80,11 @@ __device__ __forceinline__ uint32_t sel_pack(uint32_t s) {
     return v;
 }
 
+// Butterflies of directions J.. of a cube of 2^D elements.
 template <int D, int J>
 struct DirLoop {
     static __device__ __forceinline__ void run(u16x2 *lo, u16x2 *hi, const PassArgs &a) {
-        const unsigned short w = a.w[J];
+        const unsigned short w = (unsigned short)(a.lamw[J] >> 8);
         const u16x2 w2 = u16x2{w, w};
 #pragma unroll
         for (int q = 0; q < (1 << D); q++) {
@@ -98,9 +124,10 @@ __device__ __forceinline__ IDX map_bits(const PassArgs &a, int m, uint64_t bits,
     IDX x = 0;
 #pragma unroll
     for (int r = 0; r < kMaxRuns; r++) {
-        if (r < a.nruns[m] && (!lane_runs_only || a.run_f0[m][r] < 8)) {
-            uint64_t mask = (1ull << a.run_len[m][r]) - 1;
-            x |= (IDX)(((bits >> a.run_f0[m][r]) & mask) << a.run_sh[m][r]);
+        const uint32_t run = a.runs[m][r];
+        if (r < a.nruns[m] && (!lane_runs_only || run_f0(run) < 8)) {
+            uint64_t mask = (1ull << run_len(run)) - 1;
+            x |= (IDX)(((bits >> run_f0(run)) & mask) << run_sh(run));
         }
     }
     return x;
@@ -109,9 +136,11 @@ __device__ __forceinline__ IDX map_bits(const PassArgs &a, int m, uint64_t bits,
 // IDX: uint32_t when both tables hold at most 2^32 entries (byte offsets from a scalar base), else uint64_t.
 // FREE_NEW: some new row is a free variable, so whether an element starts at 255 varies across lanes; otherwise it
 // depends on the element only and the test is uniform.
-template <int D, bool BYTE, bool FREE_NEW, typename IDX>
+// FREE_NEW_ < 0: read from the descriptor at run time (the multi-program kernel, whose jobs differ).
+template <int D, bool BYTE, int FREE_NEW_, typename IDX>
 __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
                                           uint64_t block_bits, uint32_t lane, const PassArgs &a) {
+    const bool FREE_NEW = FREE_NEW_ < 0 ? a.nruns[MAP_NEW] > 0 : FREE_NEW_ != 0;
     const IDX bin = map_bits<IDX>(a, MAP_IN, block_bits, false) | map_bits<IDX>(a, MAP_IN, lane, true);
     const IDX bout = map_bits<IDX>(a, MAP_OUT, block_bits, false) | map_bits<IDX>(a, MAP_OUT, lane, true);
     const uint32_t xs = map_bits<uint32_t>(a, MAP_SEL, block_bits, false) | map_bits<uint32_t>(a, MAP_SEL, lane, true);
@@ -128,10 +157,10 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
             if ((q >> j) & 1) {
                 ai ^= (IDX)a.din[2 + j];
                 nz ^= a.dnew[2 + j];
-                s ^= a.lam[j];
+                s ^= a.lamw[j] & 3u;
             }
         uint32_t d = 0xFFFFFFFFu;
-        if (!nz) d = *reinterpret_cast<const uint32_t *>(in + ai);
+        if (!nz) d = gload32(in, ai);
         lo[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_lo(s)));
         hi[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_hi(s)));
     }
@@ -139,8 +168,8 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
 #pragma unroll
     for (int t = 0; t < 8; t++) {                  // constant indices keep the descriptor in kernel arguments
         if (t >= a.nlow) break;
-        const unsigned short w = a.low_w[t];
-        const bool sh = a.low_lam[t] & 1, sl = (a.low_lam[t] >> 1) & 1;
+        const unsigned short w = (unsigned short)(a.loww[t] >> 8);
+        const bool sh = a.loww[t] & 1, sl = (a.loww[t] >> 1) & 1;
 #pragma unroll
         for (int q = 0; q < Q; q++) low_update(lo[q], hi[q], sh, sl, u16x2{w, w});
     }
@@ -153,11 +182,11 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
             if ((q >> j) & 1) {
                 ao ^= (IDX)a.dout[2 + j];
                 sel ^= a.dsel[2 + j];
-                s ^= a.lam[j];
+                s ^= a.lamw[j] & 3u;
             }
         if (!BYTE) {
             if (sel == a.sel_target)
-                *reinterpret_cast<uint32_t *>(out + ao) = __builtin_amdgcn_perm(as_u32(hi[q]), as_u32(lo[q]), sel_pack(s));
+                gstore32(out, ao, __builtin_amdgcn_perm(as_u32(hi[q]), as_u32(lo[q]), sel_pack(s)));
         } else {
             const unsigned short r[4] = {lo[q].x, hi[q].x, lo[q].y, hi[q].y};
 #pragma unroll
@@ -165,7 +194,7 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
                 const uint32_t m = (uint32_t)k ^ s;           // byte of the dword this register entry came from
                 IDX ak = ao ^ (IDX)((m & 1) ? a.dout[0] : 0) ^ (IDX)((m & 2) ? a.dout[1] : 0);
                 uint32_t sk = sel ^ ((m & 1) ? a.dsel[0] : 0) ^ ((m & 2) ? a.dsel[1] : 0);
-                if (sk == a.sel_target) out[ak] = (uint8_t)r[k];
+                if (sk == a.sel_target) gstore8(out, ak, (uint8_t)r[k]);
             }
         }
     }
@@ -176,7 +205,7 @@ __global__ __launch_bounds__(kPassBlock) void k_pass(const uint8_t *__restrict__
                                                      uint64_t nthreads, const PassArgs a) {
     const uint64_t block = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
     if (block * kPassBlock + threadIdx.x < nthreads)
-        pass_cube<D, BYTE, FREE_NEW, IDX>(in, out, block * kPassBlock, threadIdx.x, a);
+        pass_cube<D, BYTE, FREE_NEW ? 1 : 0, IDX>(in, out, block * kPassBlock, threadIdx.x, a);
 }
 
 template <int D, bool BYTE, bool FREE_NEW>
@@ -214,7 +243,9 @@ static int check_pass(const PassArgs &a) {
     for (int m = 0; m < 4; m++) {
         TDP_REQUIRE(a.nruns[m] >= 0 && a.nruns[m] <= kMaxRuns, "pass index map %d: %d runs", m, a.nruns[m]);
         for (int r = 0; r < a.nruns[m]; r++) {
-            int f0 = a.run_f0[m][r], len = a.run_len[m][r], sh = a.run_sh[m][r];
+            const uint32_t run = a.runs[m][r];
+            int f0 = run & 0xFF, len = (run >> 8) & 0xFF, sh = (run >> 16) & 0xFF;
+            TDP_REQUIRE((run >> 24) == 0, "pass index map %d run %d: bad packing", m, r);
             TDP_REQUIRE(len >= 1 && f0 + len <= a.nfree && sh + len <= (m < 2 ? 40 : 32),
                         "pass index map %d run %d out of range", m, r);
             u[m] |= ((1ull << len) - 1) << sh;
@@ -231,9 +262,12 @@ static int check_pass(const PassArgs &a) {
     TDP_REQUIRE(a.byte_store || (a.b_out >= 2 && a.dout[0] == 1 && a.dout[1] == 2 && a.dsel[0] == 0 &&
                                  a.dsel[1] == 0 && (uout & 3) == 0),
                 "dword stores need the two low output bits on the dword lanes");
-    for (int j = 0; j < a.D; j++) TDP_REQUIRE(a.lam[j] < 4 && a.w[j] >= 1, "pass update %d: bad lane mask/weight", j);
+    for (int j = 0; j < a.D; j++)
+        TDP_REQUIRE((a.lamw[j] & 0xFF) < 4 && (a.lamw[j] >> 8) >= 1 && (a.lamw[j] >> 8) <= 255,
+                    "pass update %d: bad lane mask/weight", j);
     for (int t = 0; t < a.nlow; t++)
-        TDP_REQUIRE(a.low_lam[t] >= 1 && a.low_lam[t] < 4 && a.low_w[t] >= 1, "pass low update %d: bad mask", t);
+        TDP_REQUIRE((a.loww[t] & 0xFF) >= 1 && (a.loww[t] & 0xFF) < 4 && (a.loww[t] >> 8) >= 1 &&
+                    (a.loww[t] >> 8) <= 255, "pass low update %d: bad mask", t);
     return TDP_OK;
 }
 
@@ -286,9 +320,8 @@ __device__ __forceinline__ uint32_t wave_min(uint32_t v) {
 // used iff the least of the lanes that agree with the used set T above i differs from V.
 // io = {value, nused, error}: error 1 = an opened row is still 1 at the step's input, 2 = the walk does not end in
 // state 0 with value 0.
-__global__ __launch_bounds__(64) void k_traceback(const uint8_t *__restrict__ arena, const WStep *__restrict__ steps,
-                                                  int64_t nsteps, uint64_t final_off, int64_t *__restrict__ used,
-                                                  int64_t *__restrict__ io) {
+__device__ void traceback_wave(const uint8_t *__restrict__ arena, const WStep *__restrict__ steps, int64_t nsteps,
+                               uint64_t final_off, int64_t *__restrict__ used, int64_t *__restrict__ io) {
     const int lane = threadIdx.x;
     uint32_t v = arena[final_off];
     int64_t nused = 0, err = 0;
@@ -340,6 +373,67 @@ __global__ __launch_bounds__(64) void k_traceback(const uint8_t *__restrict__ ar
         io[1] = nused;
         io[2] = err;
     }
+}
+
+__global__ __launch_bounds__(64) void k_traceback(const uint8_t *__restrict__ arena, const WStep *__restrict__ steps,
+                                                  int64_t nsteps, uint64_t final_off, int64_t *__restrict__ used,
+                                                  int64_t *__restrict__ io) {
+    traceback_wave(arena, steps, nsteps, final_off, used, io);
+}
+
+// ------------------------------------------------------------------------------------------ many programs at once
+// One launch runs the next pass of every active program ("round"): block b belongs to job block_job[b], whose
+// descriptor, tables and thread count it reads from device memory.  Jobs of different programs touch disjoint
+// tables, so a round is a plain data-parallel launch.
+struct Job {
+    const uint8_t *in;
+    uint8_t *out;
+    uint64_t nthreads;
+    uint32_t desc, first_block;
+};
+
+__global__ __launch_bounds__(kPassBlock) void k_pass_multi(const PassArgs *__restrict__ descs,
+                                                           const Job *__restrict__ jobs,
+                                                           const uint32_t *__restrict__ block_job) {
+    const Job &J = jobs[block_job[blockIdx.x]];
+    const uint64_t local = (uint64_t)(blockIdx.x - J.first_block);
+    if (local * kPassBlock + threadIdx.x >= J.nthreads) return;
+    const PassArgs &a = descs[J.desc];
+    const uint64_t bb = local * kPassBlock;
+    // one compiled shape per (D, store mode); the branch is uniform over the block
+    switch (a.D * 2 + (a.byte_store ? 1 : 0)) {
+        case 10: pass_cube<5, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 11: pass_cube<5, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 8: pass_cube<4, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 9: pass_cube<4, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 6: pass_cube<3, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 7: pass_cube<3, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 4: pass_cube<2, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 5: pass_cube<2, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 2: pass_cube<1, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 3: pass_cube<1, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 0: pass_cube<0, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        default: pass_cube<0, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+    }
+}
+
+struct TraceJob {
+    const uint8_t *arena;
+    const WStep *steps;
+    int64_t nsteps;
+    uint64_t final_off;
+    int64_t *used, *io;
+};
+
+__global__ __launch_bounds__(64) void k_traceback_multi(const TraceJob *__restrict__ jobs) {
+    const TraceJob &J = jobs[blockIdx.x];
+    traceback_wave(J.arena, J.steps, J.nsteps, J.final_off, J.used, J.io);
+}
+
+// out[i] = *ptrs[i]: the final entries of many value programs.
+__global__ void k_gather(const uint8_t *const *__restrict__ ptrs, int n, uint8_t *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *ptrs[i];
 }
 
 struct WitnessLayout {
@@ -426,17 +520,19 @@ int tdp_program_witness_bytes(const void *steps, int64_t nsteps, uint64_t *bytes
     return TDP_OK;
 }
 
-int tdp_run_program_witness(const uint64_t *ops, int64_t nops, const void *steps_v, int64_t nsteps,
-                            const void *passes, int64_t npasses, uint8_t *d_arena, uint64_t arena_bytes, int *value,
-                            int64_t *used, int64_t *nused, void *stream) {
-    hipStream_t st = as_stream(stream);
-    const WStep *S = static_cast<const WStep *>(steps_v);
-    const PassArgs *P = static_cast<const PassArgs *>(passes);
+}  // extern "C"
+
+namespace tdp {
+
+// Every check of a witness program before anything is launched: the chain of slots inside the arena, and each
+// pass or per-op step against the op stream.
+static int validate_witness(const uint64_t *ops, int64_t nops, const WStep *S, int64_t nsteps, const PassArgs *P,
+                            int64_t npasses, uint64_t arena_bytes, WitnessLayout *Lout) {
     WitnessLayout L;
     int rc = witness_layout(S, nsteps, &L);
     if (rc) return rc;
-    TDP_REQUIRE(d_arena && arena_bytes >= L.total, "witness arena of %llu bytes, %llu needed",
-                (unsigned long long)arena_bytes, (unsigned long long)L.total);
+    TDP_REQUIRE(arena_bytes >= L.total, "witness arena of %llu bytes, %llu needed", (unsigned long long)arena_bytes,
+                (unsigned long long)L.total);
     TDP_REQUIRE(S[0].n_in == 1 && S[0].in_off == 0 && S[nsteps - 1].b_out == 0,
                 "the steps must lead from the one-entry table to the one-entry table");
     // validate the chain of slots and every step before launching anything
@@ -478,30 +574,43 @@ int tdp_run_program_witness(const uint64_t *ops, int64_t nops, const void *steps
             }
         }
     }
+    *Lout = L;
+    return TDP_OK;
+}
+
+// Step s of a validated witness program: a pass into its own slot, or one per-op op on a copy of its input.
+static int run_witness_step(const uint64_t *ops, const WStep &w, uint8_t *d_arena, const PassArgs *P, hipStream_t st) {
+    uint8_t *in = d_arena + w.in_off, *out = d_arena + w.out_off;
+    const uint64_t n = w.n_in;
+    if (w.kind == 0) return launch_pass(in, out, P[w.pass_index], st);
+    const uint64_t *o = ops + 3 * w.op0;
+    if (o[0] == 2) {
+        uint64_t nn = 0;
+        return launch_select(out, in, n, (int)o[1], (int)o[2], &nn, st);
+    }
+    TDP_HIP_CHECK(hipMemcpyAsync(out, in, n, hipMemcpyDeviceToDevice, st));
+    if (o[0] == 0 && w.nop == 2) return launch_expand(out, n, o[4] ^ n, (uint32_t)(o[5] > 255 ? 255 : o[5]), st);
+    if (o[0] == 0) return launch_fill(out + n, n, 255, st);
+    return launch_update(out, n, o[1], (uint32_t)(o[2] > 255 ? 255 : o[2]), st);
+}
+
+}  // namespace tdp
+
+extern "C" {
+
+int tdp_run_program_witness(const uint64_t *ops, int64_t nops, const void *steps_v, int64_t nsteps,
+                            const void *passes, int64_t npasses, uint8_t *d_arena, uint64_t arena_bytes, int *value,
+                            int64_t *used, int64_t *nused, void *stream) {
+    hipStream_t st = as_stream(stream);
+    const WStep *S = static_cast<const WStep *>(steps_v);
+    const PassArgs *P = static_cast<const PassArgs *>(passes);
+    TDP_REQUIRE(d_arena && S && ops, "bad witness arguments");
+    WitnessLayout L;
+    int rc = validate_witness(ops, nops, S, nsteps, P, npasses, arena_bytes, &L);
+    if (rc) return rc;
     rc = launch_fill(d_arena, 1, 0, st);
     if (rc) return rc;
-    for (int64_t s = 0; s < nsteps && !rc; s++) {
-        const WStep &w = S[s];
-        uint8_t *in = d_arena + w.in_off, *out = d_arena + w.out_off;
-        const uint64_t n = w.n_in;
-        if (w.kind == 0) {
-            rc = launch_pass(in, out, P[w.pass_index], st);
-            continue;
-        }
-        const uint64_t *o = ops + 3 * w.op0;
-        if (o[0] == 2) {
-            uint64_t nn = 0;
-            rc = launch_select(out, in, n, (int)o[1], (int)o[2], &nn, st);
-            continue;
-        }
-        TDP_HIP_CHECK(hipMemcpyAsync(out, in, n, hipMemcpyDeviceToDevice, st));
-        if (o[0] == 0 && w.nop == 2)
-            rc = launch_expand(out, n, o[4] ^ n, (uint32_t)(o[5] > 255 ? 255 : o[5]), st);
-        else if (o[0] == 0)
-            rc = launch_fill(out + n, n, 255, st);
-        else
-            rc = launch_update(out, n, o[1], (uint32_t)(o[2] > 255 ? 255 : o[2]), st);
-    }
+    for (int64_t s = 0; s < nsteps && !rc; s++) rc = run_witness_step(ops, S[s], d_arena, P, st);
     if (rc) return rc;
     WStep *d_steps = reinterpret_cast<WStep *>(d_arena + L.steps_off);
     int64_t *d_used = reinterpret_cast<int64_t *>(d_arena + L.used_off);
@@ -517,6 +626,296 @@ int tdp_run_program_witness(const uint64_t *ops, int64_t nops, const void *steps
     if (io[1] > 0) TDP_HIP_CHECK(hipMemcpy(used, d_used, (size_t)io[1] * 8, hipMemcpyDeviceToHost));
     *value = (int)io[0];
     *nused = io[1];
+    return TDP_OK;
+}
+
+}  // extern "C"
+
+namespace tdp {
+
+// Table-size bookkeeping of an op range without launching anything (the checks and effects of run_ops_range):
+// n entries, `swapped` flips on every select.
+static int dry_ops_range(const uint64_t *ops, int64_t first, int64_t last, uint64_t &n, int peak_bits,
+                         bool &swapped) {
+    const uint64_t cap = 1ull << peak_bits;
+    for (int64_t i = first; i < last; i++) {
+        uint64_t code = ops[3 * i], a = ops[3 * i + 1], b = ops[3 * i + 2];
+        if (code == 0) {
+            TDP_REQUIRE(2 * n <= cap, "op %lld: expand beyond 2^%d entries", (long long)i, peak_bits);
+            const uint64_t *nx = ops + 3 * (i + 1);
+            if (i + 1 < last && nx[0] == 1 && (nx[1] >> top_bit(n)) == 1) {
+                TDP_REQUIRE(nx[2] >= 1, "op %lld: weight < 1", (long long)(i + 1));
+                i++;
+            }
+            n <<= 1;
+        } else if (code == 1) {
+            TDP_REQUIRE(a < n && b >= 1, "op %lld: bad update", (long long)i);
+        } else if (code == 2) {
+            TDP_REQUIRE(n >= 2 && a < (uint64_t)top_bit(n) && b <= 1, "op %lld: bad select", (long long)i);
+            n >>= 1;
+            swapped = !swapped;
+        } else {
+            TDP_REQUIRE(false, "op %lld: unknown code %llu", (long long)i, (unsigned long long)code);
+        }
+    }
+    return TDP_OK;
+}
+
+// The whole schedule of a set of programs, computed on the host before anything is launched.
+struct Action {
+    int kind;             // 0 = per-op range of a value run, 1 = per-op witness step, 2 = one multi-pass round
+    int run;
+    int64_t a, b;         // kind 0: op range; kind 1: step index; kind 2: first job, number of jobs
+    uint8_t *cur, *other; // kind 0: tables at that point
+    uint64_t n;           // kind 0: entries at that point
+    uint64_t block0, nblocks;  // kind 2: first entry of the block map, blocks
+};
+
+struct Schedule {
+    std::vector<PassArgs> descs;
+    std::vector<Job> jobs;
+    std::vector<uint32_t> block_job;
+    std::vector<Action> actions;
+    std::vector<const uint8_t *> finals;  // value runs: the final one-entry table
+    std::vector<int> value_runs, witness_runs;
+    std::vector<WitnessLayout> layouts;   // per run (witness runs only)
+    uint64_t off_jobs, off_map, off_finals, off_results, off_trace, total;
+};
+
+static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
+    TDP_REQUIRE(nruns >= 1 && runs, "no programs");
+    struct Ev {
+        int kind;          // 0 = op range, 1 = witness per-op step, 2 = pass
+        int64_t a, b;      // op range / step index / (pass index, step index)
+    };
+    std::vector<std::vector<Ev>> evs(nruns);
+    std::vector<uint32_t> desc_base(nruns);
+    S.layouts.resize(nruns);
+    for (int r = 0; r < nruns; r++) {
+        const tdp_run_t &R = runs[r];
+        const PassArgs *P = static_cast<const PassArgs *>(R.passes);
+        TDP_REQUIRE(R.ops && R.nops >= 0 && (R.npasses == 0 || P), "run %d: bad op stream or passes", r);
+        desc_base[r] = (uint32_t)S.descs.size();
+        for (int64_t k = 0; k < R.npasses; k++) {
+            int rc = check_pass(P[k]);
+            if (rc) return rc;
+            TDP_REQUIRE(P[k].b_in <= 32 && P[k].b_out <= 32, "run %d: multi-program passes are limited to 2^32 "
+                        "entries", r);
+            S.descs.push_back(P[k]);
+        }
+        if (R.steps) {
+            const WStep *W = static_cast<const WStep *>(R.steps);
+            TDP_REQUIRE(R.d_arena, "run %d: witness arena missing", r);
+            int rc = validate_witness(R.ops, R.nops, W, R.nsteps, P, R.npasses, R.arena_bytes, &S.layouts[r]);
+            if (rc) return rc;
+            TDP_REQUIRE(R.used_ops, "run %d: used_ops missing", r);
+            for (int64_t k = 0; k < R.nsteps; k++)
+                evs[r].push_back(W[k].kind == 0 ? Ev{2, W[k].pass_index, k} : Ev{1, k, 0});
+            S.witness_runs.push_back(r);
+        } else {
+            TDP_REQUIRE(R.d_T0 && R.d_T1 && R.peak_bits >= 0 && R.peak_bits <= 32 && R.segments && R.nseg >= 0,
+                        "run %d: bad value program", r);
+            uint64_t n = 1;
+            bool sw = false;
+            for (int64_t k = 0; k < R.nseg; k++) {
+                const int64_t kind = R.segments[3 * k], a = R.segments[3 * k + 1], b = R.segments[3 * k + 2];
+                if (kind == 0) {
+                    TDP_REQUIRE(a >= 0 && a <= b && b <= R.nops, "run %d segment %lld: op range", r, (long long)k);
+                    int rc = dry_ops_range(R.ops, a, b, n, R.peak_bits, sw);
+                    if (rc) return rc;
+                    evs[r].push_back(Ev{0, a, b});
+                } else {
+                    TDP_REQUIRE(kind == 1 && a >= 0 && a < R.npasses, "run %d segment %lld: pass", r, (long long)k);
+                    TDP_REQUIRE(n == (1ull << P[a].b_in) && (1ull << P[a].b_out) <= (1ull << R.peak_bits),
+                                "run %d segment %lld: pass sizes", r, (long long)k);
+                    n = 1ull << P[a].b_out;
+                    evs[r].push_back(Ev{2, a, -1});
+                }
+            }
+            TDP_REQUIRE(n == 1, "run %d: the program does not end with one entry", r);
+            S.value_runs.push_back(r);
+        }
+    }
+    // rounds: every active program advances through its per-op work to its next pass; the passes form one launch
+    std::vector<size_t> cur(nruns, 0);
+    std::vector<uint8_t *> tc(nruns), to(nruns);
+    std::vector<uint64_t> nn(nruns, 1);
+    for (int r = 0; r < nruns; r++) {
+        tc[r] = runs[r].steps ? nullptr : runs[r].d_T0;
+        to[r] = runs[r].steps ? nullptr : runs[r].d_T1;
+    }
+    S.finals.assign(nruns, nullptr);
+    while (true) {
+        const size_t j0 = S.jobs.size();
+        const uint64_t b0 = S.block_job.size();
+        for (int r = 0; r < nruns; r++) {
+            const tdp_run_t &R = runs[r];
+            while (cur[r] < evs[r].size() && evs[r][cur[r]].kind != 2) {
+                const Ev &e = evs[r][cur[r]++];
+                Action act{};
+                act.run = r;
+                if (e.kind == 0) {
+                    act.kind = 0;
+                    act.a = e.a;
+                    act.b = e.b;
+                    act.cur = tc[r];
+                    act.other = to[r];
+                    act.n = nn[r];
+                    bool sw = false;
+                    int rc = dry_ops_range(R.ops, e.a, e.b, nn[r], R.peak_bits, sw);
+                    if (rc) return rc;
+                    if (sw) std::swap(tc[r], to[r]);
+                } else {
+                    act.kind = 1;
+                    act.a = e.a;
+                }
+                S.actions.push_back(act);
+            }
+            if (cur[r] >= evs[r].size()) {
+                if (!R.steps && !S.finals[r]) S.finals[r] = tc[r];
+                continue;
+            }
+            const Ev &e = evs[r][cur[r]++];
+            const PassArgs &p = S.descs[desc_base[r] + (uint32_t)e.a];
+            Job J;
+            if (R.steps) {
+                const WStep &w = static_cast<const WStep *>(R.steps)[e.b];
+                J.in = R.d_arena + w.in_off;
+                J.out = R.d_arena + w.out_off;
+            } else {
+                J.in = tc[r];
+                J.out = to[r];
+                std::swap(tc[r], to[r]);
+                nn[r] = 1ull << p.b_out;
+            }
+            J.nthreads = 1ull << p.nfree;
+            J.desc = desc_base[r] + (uint32_t)e.a;
+            const uint64_t nb = (J.nthreads + kPassBlock - 1) / kPassBlock;
+            TDP_REQUIRE(S.block_job.size() - b0 + nb < (1ull << 31), "round of more than 2^31 blocks");
+            J.first_block = (uint32_t)(S.block_job.size() - b0);
+            S.block_job.insert(S.block_job.end(), nb, (uint32_t)S.jobs.size());
+            S.jobs.push_back(J);
+        }
+        if (S.jobs.size() == j0) break;
+        Action act{};
+        act.kind = 2;
+        act.a = (int64_t)j0;
+        act.b = (int64_t)(S.jobs.size() - j0);
+        act.block0 = b0;
+        act.nblocks = S.block_job.size() - b0;
+        S.actions.push_back(act);
+    }
+    auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
+    S.off_jobs = up(S.descs.size() * sizeof(PassArgs));
+    S.off_map = S.off_jobs + up(S.jobs.size() * sizeof(Job));
+    S.off_finals = S.off_map + up(S.block_job.size() * 4);
+    S.off_results = S.off_finals + up(S.value_runs.size() * 8);
+    S.off_trace = S.off_results + up(S.value_runs.size());
+    S.total = S.off_trace + up(S.witness_runs.size() * sizeof(TraceJob)) + 256;
+    return TDP_OK;
+}
+
+}  // namespace tdp
+
+extern "C" {
+
+int tdp_run_record_bytes(void) { return (int)sizeof(tdp_run_t); }
+
+int tdp_run_programs_workspace(const tdp_run_t *runs, int nruns, uint64_t *bytes) {
+    Schedule S;
+    int rc = build_schedule(runs, nruns, S);
+    if (rc) return rc;
+    *bytes = S.total;
+    return TDP_OK;
+}
+
+int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *stream) {
+    hipStream_t st = as_stream(stream);
+    Schedule S;
+    int rc = build_schedule(runs, nruns, S);
+    if (rc) return rc;
+    TDP_REQUIRE(d_work && work_bytes >= S.total, "workspace of %llu bytes, %llu needed",
+                (unsigned long long)work_bytes, (unsigned long long)S.total);
+    // one upload: descriptors, jobs, block map, final-table pointers, traceback records
+    std::vector<uint8_t> host(S.total, 0);
+    memcpy(host.data(), S.descs.data(), S.descs.size() * sizeof(PassArgs));
+    memcpy(host.data() + S.off_jobs, S.jobs.data(), S.jobs.size() * sizeof(Job));
+    memcpy(host.data() + S.off_map, S.block_job.data(), S.block_job.size() * 4);
+    for (size_t k = 0; k < S.value_runs.size(); k++) {
+        const uint8_t *f = S.finals[S.value_runs[k]];
+        memcpy(host.data() + S.off_finals + 8 * k, &f, 8);
+    }
+    for (size_t k = 0; k < S.witness_runs.size(); k++) {
+        const tdp_run_t &R = runs[S.witness_runs[k]];
+        const WitnessLayout &L = S.layouts[S.witness_runs[k]];
+        const WStep *W = static_cast<const WStep *>(R.steps);
+        TraceJob T{R.d_arena, reinterpret_cast<const WStep *>(R.d_arena + L.steps_off), R.nsteps,
+                   W[R.nsteps - 1].out_off, reinterpret_cast<int64_t *>(R.d_arena + L.used_off),
+                   reinterpret_cast<int64_t *>(R.d_arena + L.io_off)};
+        memcpy(host.data() + S.off_trace + k * sizeof(TraceJob), &T, sizeof(T));
+    }
+    TDP_HIP_CHECK(hipMemcpyAsync(d_work, host.data(), S.total, hipMemcpyHostToDevice, st));
+    for (int r = 0; r < nruns; r++) {
+        rc = launch_fill(runs[r].steps ? runs[r].d_arena : runs[r].d_T0, 1, 0, st);
+        if (rc) return rc;
+    }
+    const PassArgs *d_descs = reinterpret_cast<const PassArgs *>(d_work);
+    const Job *d_jobs = reinterpret_cast<const Job *>(d_work + S.off_jobs);
+    const uint32_t *d_map = reinterpret_cast<const uint32_t *>(d_work + S.off_map);
+    for (const Action &act : S.actions) {
+        const tdp_run_t &R = runs[act.run];
+        if (act.kind == 0) {
+            uint8_t *c = act.cur, *o = act.other;
+            uint64_t n = act.n;
+            rc = run_ops_range(R.ops, act.a, act.b, c, o, n, R.peak_bits, st);
+        } else if (act.kind == 1) {
+            rc = run_witness_step(R.ops, static_cast<const WStep *>(R.steps)[act.a], R.d_arena,
+                                  static_cast<const PassArgs *>(R.passes), st);
+        } else {
+            k_pass_multi<<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs, d_map + act.block0);
+            TDP_LAUNCH_CHECK();
+        }
+        if (rc) return rc;
+    }
+    std::vector<uint8_t> vals(S.value_runs.size());
+    if (!S.value_runs.empty()) {
+        const int nv = (int)S.value_runs.size();
+        k_gather<<<(nv + 255) / 256, 256, 0, st>>>(reinterpret_cast<const uint8_t *const *>(d_work + S.off_finals),
+                                                   nv, d_work + S.off_results);
+        TDP_LAUNCH_CHECK();
+        TDP_HIP_CHECK(hipMemcpyAsync(vals.data(), d_work + S.off_results, nv, hipMemcpyDeviceToHost, st));
+    }
+    std::vector<int64_t> io(3 * S.witness_runs.size());
+    if (!S.witness_runs.empty()) {
+        for (size_t k = 0; k < S.witness_runs.size(); k++) {
+            const tdp_run_t &R = runs[S.witness_runs[k]];
+            const WitnessLayout &L = S.layouts[S.witness_runs[k]];
+            TDP_HIP_CHECK(hipMemcpyAsync(R.d_arena + L.steps_off, R.steps, (size_t)R.nsteps * sizeof(WStep),
+                                         hipMemcpyHostToDevice, st));
+        }
+        k_traceback_multi<<<(unsigned)S.witness_runs.size(), 64, 0, st>>>(
+            reinterpret_cast<const TraceJob *>(d_work + S.off_trace));
+        TDP_LAUNCH_CHECK();
+        for (size_t k = 0; k < S.witness_runs.size(); k++) {
+            const tdp_run_t &R = runs[S.witness_runs[k]];
+            const WitnessLayout &L = S.layouts[S.witness_runs[k]];
+            TDP_HIP_CHECK(hipMemcpyAsync(&io[3 * k], R.d_arena + L.io_off, 24, hipMemcpyDeviceToHost, st));
+        }
+    }
+    TDP_HIP_CHECK(hipStreamSynchronize(st));
+    for (size_t k = 0; k < S.value_runs.size(); k++) runs[S.value_runs[k]].value = vals[k];
+    for (size_t k = 0; k < S.witness_runs.size(); k++) {
+        tdp_run_t &R = runs[S.witness_runs[k]];
+        const WitnessLayout &L = S.layouts[S.witness_runs[k]];
+        TDP_REQUIRE(io[3 * k + 2] == 0, "run %d: traceback failed (code %lld)", S.witness_runs[k],
+                    (long long)io[3 * k + 2]);
+        TDP_REQUIRE(io[3 * k + 1] >= 0 && (uint64_t)io[3 * k + 1] <= L.nupd, "run %d: traceback used count",
+                    S.witness_runs[k]);
+        R.value = (int32_t)io[3 * k];
+        R.nused = io[3 * k + 1];
+        if (R.nused > 0)
+            TDP_HIP_CHECK(hipMemcpy(R.used_ops, R.d_arena + L.used_off, (size_t)R.nused * 8, hipMemcpyDeviceToHost));
+    }
     return TDP_OK;
 }
 

@@ -23,11 +23,11 @@ constexpr uint64_t kSlotAlign = 256;
 struct PassArgs {
     uint64_t din[8], dout[8];
     uint32_t dnew[8], dsel[8];
-    uint8_t run_f0[4][kMaxRuns], run_len[4][kMaxRuns], run_sh[4][kMaxRuns];
+    uint32_t runs[4][kMaxRuns];             // f0 | len << 8 | sh << 16
     int32_t nruns[4];
     uint32_t sel_target;
     int32_t D, nlow, byte_store, b_in, b_out, nfree, pad;
-    uint8_t lam[8], w[8], low_lam[8], low_w[8];
+    uint32_t lamw[8], loww[8];              // lane mask | weight << 8
 };
 struct WStep {
     uint64_t in_off, inmask, newmask, fixmask, fixval;
@@ -40,7 +40,7 @@ struct WStep {
     uint8_t out_var[48];
     uint8_t pad2[2];
 };
-static_assert(sizeof(PassArgs) == 416, "pass descriptor layout");
+static_assert(sizeof(PassArgs) == 496, "pass descriptor layout");
 static_assert(sizeof(WStep) == 240, "witness step layout");
 
 struct Update {
@@ -83,9 +83,13 @@ struct Basis {
 };
 
 // runs [f0, len, shift] of consecutive thread-index bits mapped to consecutive single bits (fused.address_runs)
-int address_runs(const uint64_t *vals, int n, uint8_t *f0s, uint8_t *lens, uint8_t *shs) {
+int address_runs(const uint64_t *vals, int n, uint32_t *runs) {
     int cnt = 0;
     int lf = -1, ll = 0, ls = 0;
+    auto emit = [&]() {
+        if (cnt < kMaxRuns) runs[cnt] = (uint32_t)lf | (uint32_t)ll << 8 | (uint32_t)ls << 16;
+        cnt++;
+    };
     for (int f = 0; f < n; f++) {
         uint64_t v = vals[f];
         if (!v) continue;
@@ -93,27 +97,13 @@ int address_runs(const uint64_t *vals, int n, uint8_t *f0s, uint8_t *lens, uint8
         if (lf >= 0 && lf + ll == f && ls + ll == sh) {
             ll++;
         } else {
-            if (lf >= 0) {
-                if (cnt < kMaxRuns) {
-                    f0s[cnt] = (uint8_t)lf;
-                    lens[cnt] = (uint8_t)ll;
-                    shs[cnt] = (uint8_t)ls;
-                }
-                cnt++;
-            }
+            if (lf >= 0) emit();
             lf = f;
             ll = 1;
             ls = sh;
         }
     }
-    if (lf >= 0) {
-        if (cnt < kMaxRuns) {
-            f0s[cnt] = (uint8_t)lf;
-            lens[cnt] = (uint8_t)ll;
-            shs[cnt] = (uint8_t)ls;
-        }
-        cnt++;
-    }
+    if (lf >= 0) emit();
     return cnt;
 }
 
@@ -240,7 +230,7 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
     const uint64_t *maps[4] = {fin, fout, fnew, fsel};
     P.max_runs = 0;
     for (int m = 0; m < 4; m++) {
-        int c = address_runs(maps[m], nfree, A.run_f0[m], A.run_len[m], A.run_sh[m]);
+        int c = address_runs(maps[m], nfree, A.runs[m]);
         A.nruns[m] = c > kMaxRuns ? kMaxRuns : c;
         P.max_runs = std::max(P.max_runs, c);
     }
@@ -254,12 +244,10 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
     A.b_out = b_out;
     A.byte_store = !(b_out >= 2 && phys[0] == 0 && phys[1] == 1);
     for (int j = 0; j < D && j < 8; j++) {
-        A.lam[j] = (uint8_t)dirs[j].lam;
-        A.w[j] = (uint8_t)(dirs[j].w > 255 ? 255 : dirs[j].w);
+        A.lamw[j] = dirs[j].lam | (uint32_t)(dirs[j].w > 255 ? 255 : dirs[j].w) << 8;
     }
     for (size_t t = 0; t < lows.size() && t < 8; t++) {
-        A.low_lam[t] = (uint8_t)lows[t].first;
-        A.low_w[t] = (uint8_t)(lows[t].second > 255 ? 255 : lows[t].second);
+        A.loww[t] = lows[t].first | (uint32_t)(lows[t].second > 255 ? 255 : lows[t].second) << 8;
     }
     P.first = first;
     P.last = last;

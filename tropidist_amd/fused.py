@@ -478,12 +478,16 @@ def simulate(program, plan):
 
 
 # ------------------------------------------------------------------------------------------ GPU executor
+# runs: f0 | len << 8 | shift << 16; lamw / loww: lane mask | weight << 8 (all 32-bit words for scalar loads)
 PASS_DTYPE = np.dtype([("din", "<u8", 8), ("dout", "<u8", 8), ("dnew", "<u4", 8), ("dsel", "<u4", 8),
-                       ("run_f0", "u1", (4, MAX_RUNS)), ("run_len", "u1", (4, MAX_RUNS)),
-                       ("run_sh", "u1", (4, MAX_RUNS)), ("nruns", "<i4", 4), ("sel_target", "<u4"),
+                       ("runs", "<u4", (4, MAX_RUNS)), ("nruns", "<i4", 4), ("sel_target", "<u4"),
                        ("D", "<i4"), ("nlow", "<i4"), ("byte_store", "<i4"), ("b_in", "<i4"), ("b_out", "<i4"),
-                       ("nfree", "<i4"), ("pad", "<i4"),
-                       ("lam", "u1", 8), ("w", "u1", 8), ("low_lam", "u1", 8), ("low_w", "u1", 8)], align=True)
+                       ("nfree", "<i4"), ("pad", "<i4"), ("lamw", "<u4", 8), ("loww", "<u4", 8)], align=True)
+
+
+def unpack_run(word):
+    w = int(word)
+    return [w & 0xFF, (w >> 8) & 0xFF, (w >> 16) & 0xFF]
 SEG_OPS, SEG_PASS = 0, 1
 
 
@@ -509,14 +513,12 @@ def pack(program):
                 raise ValueError("pass index map with %d runs (at most %d)" % (len(runs), MAX_RUNS))
             r["nruns"][m] = len(runs)
             for i, (f0, ln, sh) in enumerate(runs):
-                r["run_f0"][m, i], r["run_len"][m, i], r["run_sh"][m, i] = f0, ln, sh
+                r["runs"][m, i] = f0 | ln << 8 | sh << 16
         r["sel_target"] = p.sel_target
         r["nfree"], r["nlow"], r["D"] = p.nfree, len(p.low_lam), p.D
         r["byte_store"], r["b_in"], r["b_out"] = int(p.byte_store), p.b_in, p.b_out
-        r["lam"][:p.D] = p.lam
-        r["w"][:p.D] = [min(255, int(x)) for x in p.w]
-        r["low_lam"][:len(p.low_lam)] = p.low_lam
-        r["low_w"][:len(p.low_w)] = [min(255, int(x)) for x in p.low_w]
+        r["lamw"][:p.D] = [int(lam) | min(255, int(w)) << 8 for lam, w in zip(p.lam, p.w)]
+        r["loww"][:len(p.low_lam)] = [int(lam) | min(255, int(w)) << 8 for lam, w in zip(p.low_lam, p.low_w)]
         segs[s] = (SEG_PASS, ip, 0)
         ip += 1
     return segs, arr
@@ -541,18 +543,12 @@ class NativeProgram:
                                                 else "unsigned long")
 
 
-_compiled = {}
-
-
 def _cached(plan, key, make):
-    k = (id(plan),) + key
-    hit = _compiled.get(k)
-    if hit is not None and hit[0] is plan:
-        return hit[1]
-    val = make()
-    if len(_compiled) > 512:
-        _compiled.clear()
-    _compiled[k] = (plan, val)
+    """Compiled programs live on the plan object (per compile options), so they last as long as the plan."""
+    cache = plan.__dict__.setdefault("_fused_programs", {})
+    val = cache.get(key)
+    if val is None:
+        val = cache[key] = make()
     return val
 
 
@@ -802,3 +798,70 @@ def compile_native(plan, dmax=DMAX, min_bits=12, max_updates=None, witness=False
     if witness:
         return segs, arr, steps[:nst.value]
     return segs, arr
+
+
+# ------------------------------------------------------------------------------------------ many programs at once
+def _run_record_type():
+    import ctypes
+    vp, i64, u64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_int32
+
+    class TdpRun(ctypes.Structure):
+        """tdp_run_t of include/tdp.h."""
+        _fields_ = [("ops", vp), ("nops", i64), ("segments", vp), ("nseg", i64), ("passes", vp), ("npasses", i64),
+                    ("steps", vp), ("nsteps", i64), ("d_T0", vp), ("d_T1", vp), ("d_arena", vp),
+                    ("arena_bytes", u64), ("used_ops", vp), ("nused", i64), ("peak_bits", i32), ("value", i32)]
+    return TdpRun
+
+
+def run_many(plans, witness=None, stream=None):
+    """Run many feasible plans together by tdp_run_programs (one launch per round runs the next pass of every
+    program).  witness[i] true: a witness run, result (value, sorted used op indices); else the value.  Values of
+    255 or more raise OverflowError.  ValueError if a plan's column exceeds the pass limits (run it alone)."""
+    import ctypes
+    from . import _lib, kernel
+    from .trellis import as_plan
+    L = _lib.lib()
+    TdpRun = _run_record_type()
+    if int(L.tdp_run_record_bytes()) != ctypes.sizeof(TdpRun):
+        raise RuntimeError("tdp_run_t layout mismatch")
+    plans = [as_plan(p) for p in plans]
+    witness = [False] * len(plans) if witness is None else list(witness)
+    runs = (TdpRun * len(plans))()
+    keep, used_of = [], {}
+    for r, (plan, w) in enumerate(zip(plans, witness)):
+        if not plan.feasible:
+            raise ValueError("run_many takes feasible plans only")
+        ops = np.ascontiguousarray(plan.encode(), dtype=np.uint64)
+        R = runs[r]
+        R.ops, R.nops = ops.ctypes.data, len(ops)
+        R.peak_bits = plan.peak_bits
+        if w:
+            prog = witness_program_for(plan)
+            nbytes = witness_bytes(plan)
+            arena = _lib.device_empty(nbytes)
+            used = np.zeros(max(1, int(prog.steps["k"].sum())), dtype=np.int64)
+            R.steps, R.nsteps = prog.steps.ctypes.data, len(prog.steps)
+            R.d_arena, R.arena_bytes = _lib.dptr(arena), nbytes
+            R.used_ops = used.ctypes.data
+            used_of[r] = used
+            keep.append(arena)
+        else:
+            # passes from 2^2 entries up: narrow heads and tails are rounds too, not one launch per op
+            prog = program_for(plan, min_bits=2)
+            T0 = _lib.device_empty(1 << plan.peak_bits)
+            T1 = _lib.device_empty(1 << plan.peak_bits)
+            R.segments, R.nseg = prog.segs.ctypes.data, len(prog.segs)
+            R.d_T0, R.d_T1 = _lib.dptr(T0), _lib.dptr(T1)
+            keep += [T0, T1]
+        R.passes, R.npasses = prog.arr.ctypes.data, prog.npasses
+        keep += [ops, prog]
+    nb = ctypes.c_uint64(0)
+    _lib.check(L.tdp_run_programs_workspace(runs, len(plans), ctypes.byref(nb)), "tdp_run_programs_workspace")
+    work = _lib.device_empty(nb.value)
+    _lib.check(L.tdp_run_programs(runs, len(plans), _lib.dptr(work), nb.value, _lib.stream_handle(stream)),
+               "tdp_run_programs")
+    out = []
+    for r, w in enumerate(witness):
+        v = kernel.final_value(runs[r].value)
+        out.append((v, sorted(int(i) for i in used_of[r][:runs[r].nused])) if w else v)
+    return out
