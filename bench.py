@@ -12,8 +12,8 @@ Beside it (each can be skipped):
             certificates (distance + checked witness) per second from one persistent launch per pass, against the
             same plans certified one at a time with per-op kernels on a sample ("plain GPU");
   sparse    the two-block quasi-cyclic code l = m = 12 (288 columns), meet-in-the-middle sweep at threshold K;
-  serving   a Poisson stream of requests from the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse
-            MITM on quasi-cyclic codes), p50 / p99 latency and certificates per second;
+  serving   the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse MITM on quasi-cyclic codes):
+            capacity with every request submitted at once, then a Poisson stream at 80% of it, p50 / p99 latency;
   cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--skip-batched] [--skip-sparse]
@@ -199,10 +199,8 @@ def bench_batched(args, scan):
     orders = [list(range(p.ncols)) for p in scan]
     orders, flat, owner = batch.plan_problems(scan, orders)
     plan_s = time.time() - t0
-    # warm the allocator and the code objects on a few problems
-    k = 8
-    sub = [i for i, (pi, _) in enumerate(owner) if pi < k]
-    batch.certify_planned(scan[:k], orders[:k], [flat[i] for i in sub], [owner[i] for i in sub])
+    # one untimed pass over the whole scan: device allocations (tables, witness arenas) and code objects
+    batch.certify_planned(scan, orders, flat, owner)
     import torch
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -247,23 +245,48 @@ def bench_sparse(args):
             "peak_states": r["peak_states"], "witness_cols": len(r["witness"] or [])}
 
 
+def _serve_run(serve, reqs, rate, seed):
+    srv = serve.Server(memory_bytes=96 << 30, dense_lanes=3, sparse_lanes=4)
+    try:
+        if rate is None:                        # closed loop: every request waiting at t = 0
+            t0 = time.perf_counter()
+            tickets = [srv.submit_dense(*a) if k == "dense" else srv.submit_sparse(*a) for k, a in reqs]
+            for tk in tickets:
+                tk._event.wait()
+            wall = time.perf_counter() - t0
+        else:
+            tickets, wall = serve.run_poisson(srv, reqs, rate=rate, seed=seed)
+    finally:
+        srv.close()
+    return tickets, wall
+
+
 def bench_serving(args, requests):
+    """Capacity first (all requests submitted at once), then a Poisson stream at a fraction of it (or at
+    --serving-rate), both on fresh servers and freshly planned requests."""
     from tropidist_amd import serve
-    srv = serve.Server(memory_bytes=64 << 30, dense_lanes=3, sparse_lanes=4)
     reqs = [("dense", (p,)) if kind == "dense" else ("sparse", (p, K)) for kind, p, K in requests]
-    tickets, wall = serve.run_poisson(srv, reqs, rate=args.serving_rate, seed=7)
-    srv.close()
+    cap_tickets, cap_wall = _serve_run(serve, reqs, None, 0)
+    capacity = len(reqs) / cap_wall
+    rate = args.serving_rate if args.serving_rate else args.serving_load * capacity
+    tickets, wall = _serve_run(serve, reqs, rate, 7)
     lat = [tk.latency for tk in tickets]
-    errors = sum(tk.error is not None for tk in tickets)
+    errs = [tk for tk in tickets if tk.error is not None]
     ok = all(tk.result["witness_ok"] for tk in tickets if tk.error is None and tk.result["distance"] is not None)
+    same = all(a.error is None and b.error is None and a.result["distance"] == b.result["distance"]
+               for a, b in zip(cap_tickets, tickets) if a.error is None and b.error is None)
     dl = [tk.latency for tk in tickets if tk.kind == "dense"]
     sl = [tk.latency for tk in tickets if tk.kind == "sparse"]
-    return {"workload": "Poisson stream of %d requests at %.0f/s: 70%% dense width 10-20, 20%% width 21-25, 10%% "
-                        "sparse MITM K<=%d on QC codes l,m 6-15" % (len(reqs), args.serving_rate, args.serving_max_K),
+    return {"workload": "%d requests: 70%% dense width 10-20, 20%% width 21-25, 10%% sparse MITM K<=%d on QC codes "
+                        "l,m 6-15; capacity with all submitted at once, then Poisson arrivals at %.0f/s (%.0f%% of "
+                        "capacity)" % (len(reqs), args.serving_max_K, rate, 100 * rate / capacity),
+            "capacity_certs_per_s": capacity, "capacity_wall_seconds": cap_wall, "rate": rate,
             "certs_per_s": len(reqs) / wall, "wall_seconds": wall, "p50_s": serve.percentile(lat, 50),
             "p99_s": serve.percentile(lat, 99), "dense_p50_s": serve.percentile(dl, 50),
             "dense_p99_s": serve.percentile(dl, 99), "sparse_p50_s": serve.percentile(sl, 50),
-            "sparse_p99_s": serve.percentile(sl, 99), "errors": errors, "witnesses_ok": ok}
+            "sparse_p99_s": serve.percentile(sl, 99), "errors": len(errs),
+            "error_kinds": sorted({type(tk.error).__name__ + ": " + str(tk.error)[:80] for tk in errs}),
+            "witnesses_ok": ok, "same_distances_both_runs": same}
 
 
 def profile_pass(args):
@@ -305,7 +328,9 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=24)
     ap.add_argument("--sparse-K", type=int, default=8)
     ap.add_argument("--serving-requests", type=int, default=600)
-    ap.add_argument("--serving-rate", type=float, default=100.0)
+    ap.add_argument("--serving-rate", type=float, default=None,
+                    help="Poisson arrival rate (default: --serving-load times the measured capacity)")
+    ap.add_argument("--serving-load", type=float, default=0.8)
     ap.add_argument("--serving-max-K", type=int, default=8)
     ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
                     help="only replay the dominant fused pass N times and a device copy of the same input size N "

@@ -97,7 +97,8 @@ int tdp_witness_step_bytes(void);
  * tdp_run_program) or, when steps is not NULL, a witness run (as tdp_run_program_witness, on d_arena with room
  * for used_ops in the host array).  One launch per round runs the next pass of every active program, so programs
  * whose passes are too small to fill the device alone share it; per-op segments and steps are launched between the
- * rounds.  d_work: tdp_run_programs_workspace(runs) bytes.  Outputs: value (the final entry, 255 = cost >= 255),
+ * rounds.  d_work: tdp_run_programs_workspace(runs) bytes; host_staging (optional, NULL): pinned host memory of at
+ * least as many bytes, through which the schedule is uploaded.  Outputs: value (the final entry, 255 = cost >= 255),
  * nused / used_ops for witness runs.  Tables of passes must hold at most 2^32 entries.  No reference counterpart
  * (the reference runs one plan at a time, cdist/auto.py:105-123). */
 typedef struct {
@@ -120,7 +121,8 @@ typedef struct {
 } tdp_run_t;
 int tdp_run_record_bytes(void);
 int tdp_run_programs_workspace(const tdp_run_t *runs, int nruns, uint64_t *bytes);
-int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *stream);
+int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
+                     uint64_t staging_bytes, void *stream);
 
 /* Host compiler of fused programs (no device work): the segments (capacity seg_cap triples), pass descriptors
  * (capacity pass_cap records) and, when steps is not NULL, the witness steps (capacity step_cap records) of an

@@ -32,7 +32,7 @@ SIGNATURES = {
     "tdp_witness_step_bytes": (ci, []),
     "tdp_run_record_bytes": (ci, []),
     "tdp_run_programs_workspace": (ci, [vp, ci, ctypes.POINTER(u64)]),
-    "tdp_run_programs": (ci, [vp, ci, vp, u64, vp]),
+    "tdp_run_programs": (ci, [vp, ci, vp, u64, vp, u64, vp]),
     "tdp_fused_compile": (ci, [vp, i64, ci, ci, ci, vp, i64, ctypes.POINTER(i64), vp, i64, ctypes.POINTER(i64), vp,
                                i64, ctypes.POINTER(i64)]),
     "tdp_program_witness_bytes": (ci, [vp, i64, ctypes.POINTER(u64)]),

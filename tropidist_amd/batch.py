@@ -176,9 +176,10 @@ def _columns(plan, order, used):
     return sorted(order[plan.columns[int(ordinal[int(i)])]] for i in used)
 
 
-def plan_problems(problems, orders=None):
+def plan_problems(problems, orders=None, compile_wide=True):
     """Host planning of a batch with the C++ planner: (orders, flat list of plans, owner (problem, observable) of
-    each plan)."""
+    each plan).  compile_wide: also compile the fused programs (value and witness) of plans with 2^WIDE_BITS
+    entries or more, which the wide path runs (cached on the plans)."""
     from .trellis import build_plans_native
     orders = orders or [auto.best_order(p)[0] for p in problems]
     flat, owner = [], []
@@ -187,12 +188,21 @@ def plan_problems(problems, orders=None):
         for i, plan in enumerate(plans):
             flat.append(plan)
             owner.append((pi, i))
+    if compile_wide:
+        from . import fused
+        for plan in flat:
+            if plan.feasible and plan.peak_bits >= WIDE_BITS:
+                try:
+                    fused.program_for(plan, min_bits=2)
+                    fused.witness_program_for(plan)
+                except ValueError:          # runs on the per-op executor
+                    pass
     return orders, flat, owner
 
 
 # Problems with a plan of at least 2^WIDE_BITS entries go to the multi-program fused executor; each of its calls
 # holds at most WIDE_BUDGET device bytes of tables and witness arenas.
-WIDE_BITS = 18
+WIDE_BITS = 20
 WIDE_BUDGET = 48 << 30
 
 

@@ -382,9 +382,11 @@ __global__ __launch_bounds__(64) void k_traceback(const uint8_t *__restrict__ ar
 }
 
 // ------------------------------------------------------------------------------------------ many programs at once
-// One launch runs the next pass of every active program ("round"): block b belongs to job block_job[b], whose
-// descriptor, tables and thread count it reads from device memory.  Jobs of different programs touch disjoint
-// tables, so a round is a plain data-parallel launch.
+// One launch runs the next pass of every active program ("round").  The round's jobs own consecutive ranges of
+// blocks (first_block ascending); a block finds its job by counting, with all its threads at once, the jobs that
+// start at or before it -- one coalesced read of the round's job records, no per-block map.  A job's descriptor,
+// tables and thread count are read from device memory.  Jobs of different programs touch disjoint tables, so a
+// round is a plain data-parallel launch.
 struct Job {
     const uint8_t *in;
     uint8_t *out;
@@ -393,9 +395,17 @@ struct Job {
 };
 
 __global__ __launch_bounds__(kPassBlock) void k_pass_multi(const PassArgs *__restrict__ descs,
-                                                           const Job *__restrict__ jobs,
-                                                           const uint32_t *__restrict__ block_job) {
-    const Job &J = jobs[block_job[blockIdx.x]];
+                                                           const Job *__restrict__ jobs, uint32_t njobs) {
+    __shared__ uint32_t s_count;
+    if (threadIdx.x == 0) s_count = 0;
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t j = threadIdx.x; j < njobs; j += kPassBlock) c += jobs[j].first_block <= blockIdx.x ? 1u : 0u;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) c += (uint32_t)__shfl_xor((int)c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_count, c);
+    __syncthreads();
+    const Job &J = jobs[s_count - 1];
     const uint64_t local = (uint64_t)(blockIdx.x - J.first_block);
     if (local * kPassBlock + threadIdx.x >= J.nthreads) return;
     const PassArgs &a = descs[J.desc];
@@ -668,18 +678,17 @@ struct Action {
     int64_t a, b;         // kind 0: op range; kind 1: step index; kind 2: first job, number of jobs
     uint8_t *cur, *other; // kind 0: tables at that point
     uint64_t n;           // kind 0: entries at that point
-    uint64_t block0, nblocks;  // kind 2: first entry of the block map, blocks
+    uint64_t nblocks;     // kind 2: blocks of the round
 };
 
 struct Schedule {
     std::vector<PassArgs> descs;
     std::vector<Job> jobs;
-    std::vector<uint32_t> block_job;
     std::vector<Action> actions;
     std::vector<const uint8_t *> finals;  // value runs: the final one-entry table
     std::vector<int> value_runs, witness_runs;
     std::vector<WitnessLayout> layouts;   // per run (witness runs only)
-    uint64_t off_jobs, off_map, off_finals, off_results, off_trace, total;
+    uint64_t off_jobs, off_finals, off_results, off_trace, total;
 };
 
 static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
@@ -747,7 +756,7 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
     S.finals.assign(nruns, nullptr);
     while (true) {
         const size_t j0 = S.jobs.size();
-        const uint64_t b0 = S.block_job.size();
+        uint64_t nblocks = 0;
         for (int r = 0; r < nruns; r++) {
             const tdp_run_t &R = runs[r];
             while (cur[r] < evs[r].size() && evs[r][cur[r]].kind != 2) {
@@ -791,9 +800,9 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
             J.nthreads = 1ull << p.nfree;
             J.desc = desc_base[r] + (uint32_t)e.a;
             const uint64_t nb = (J.nthreads + kPassBlock - 1) / kPassBlock;
-            TDP_REQUIRE(S.block_job.size() - b0 + nb < (1ull << 31), "round of more than 2^31 blocks");
-            J.first_block = (uint32_t)(S.block_job.size() - b0);
-            S.block_job.insert(S.block_job.end(), nb, (uint32_t)S.jobs.size());
+            TDP_REQUIRE(nblocks + nb < (1ull << 31), "round of more than 2^31 blocks");
+            J.first_block = (uint32_t)nblocks;
+            nblocks += nb;
             S.jobs.push_back(J);
         }
         if (S.jobs.size() == j0) break;
@@ -801,14 +810,12 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
         act.kind = 2;
         act.a = (int64_t)j0;
         act.b = (int64_t)(S.jobs.size() - j0);
-        act.block0 = b0;
-        act.nblocks = S.block_job.size() - b0;
+        act.nblocks = nblocks;
         S.actions.push_back(act);
     }
     auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
     S.off_jobs = up(S.descs.size() * sizeof(PassArgs));
-    S.off_map = S.off_jobs + up(S.jobs.size() * sizeof(Job));
-    S.off_finals = S.off_map + up(S.block_job.size() * 4);
+    S.off_finals = S.off_jobs + up(S.jobs.size() * sizeof(Job));
     S.off_results = S.off_finals + up(S.value_runs.size() * 8);
     S.off_trace = S.off_results + up(S.value_runs.size());
     S.total = S.off_trace + up(S.witness_runs.size() * sizeof(TraceJob)) + 256;
@@ -829,18 +836,28 @@ int tdp_run_programs_workspace(const tdp_run_t *runs, int nruns, uint64_t *bytes
     return TDP_OK;
 }
 
-int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *stream) {
+int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
+                     uint64_t staging_bytes, void *stream) {
     hipStream_t st = as_stream(stream);
     Schedule S;
     int rc = build_schedule(runs, nruns, S);
     if (rc) return rc;
     TDP_REQUIRE(d_work && work_bytes >= S.total, "workspace of %llu bytes, %llu needed",
                 (unsigned long long)work_bytes, (unsigned long long)S.total);
-    // one upload: descriptors, jobs, block map, final-table pointers, traceback records
-    std::vector<uint8_t> host(S.total, 0);
+    // one upload: descriptors, jobs, final-table pointers, traceback records (through the caller's pinned staging
+    // buffer when it is large enough)
+    std::vector<uint8_t> own;
+    uint8_t *hbuf = static_cast<uint8_t *>(host_staging);
+    if (!hbuf || staging_bytes < S.total) {
+        own.assign(S.total, 0);
+        hbuf = own.data();
+    }
+    struct HostView {
+        uint8_t *p;
+        uint8_t *data() { return p; }
+    } host{hbuf};
     memcpy(host.data(), S.descs.data(), S.descs.size() * sizeof(PassArgs));
     memcpy(host.data() + S.off_jobs, S.jobs.data(), S.jobs.size() * sizeof(Job));
-    memcpy(host.data() + S.off_map, S.block_job.data(), S.block_job.size() * 4);
     for (size_t k = 0; k < S.value_runs.size(); k++) {
         const uint8_t *f = S.finals[S.value_runs[k]];
         memcpy(host.data() + S.off_finals + 8 * k, &f, 8);
@@ -861,7 +878,6 @@ int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_
     }
     const PassArgs *d_descs = reinterpret_cast<const PassArgs *>(d_work);
     const Job *d_jobs = reinterpret_cast<const Job *>(d_work + S.off_jobs);
-    const uint32_t *d_map = reinterpret_cast<const uint32_t *>(d_work + S.off_map);
     for (const Action &act : S.actions) {
         const tdp_run_t &R = runs[act.run];
         if (act.kind == 0) {
@@ -872,7 +888,7 @@ int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_
             rc = run_witness_step(R.ops, static_cast<const WStep *>(R.steps)[act.a], R.d_arena,
                                   static_cast<const PassArgs *>(R.passes), st);
         } else {
-            k_pass_multi<<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs, d_map + act.block0);
+            k_pass_multi<<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, (uint32_t)act.b);
             TDP_LAUNCH_CHECK();
         }
         if (rc) return rc;

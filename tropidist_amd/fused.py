@@ -813,6 +813,20 @@ def _run_record_type():
     return TdpRun
 
 
+_STAGING = __import__("threading").local()
+
+
+def _staging(nbytes):
+    """The calling thread's pinned host buffer of at least nbytes for schedule uploads (grown on demand, reused by
+    the thread's later calls)."""
+    from . import _lib
+    t = _lib.torch()
+    buf = getattr(_STAGING, "buf", None)
+    if buf is None or buf.numel() < nbytes:
+        buf = _STAGING.buf = t.empty(max(nbytes, 1 << 24), dtype=t.uint8, pin_memory=True)
+    return buf
+
+
 def run_many(plans, witness=None, stream=None):
     """Run many feasible plans together by tdp_run_programs (one launch per round runs the next pass of every
     program).  witness[i] true: a witness run, result (value, sorted used op indices); else the value.  Values of
@@ -858,8 +872,9 @@ def run_many(plans, witness=None, stream=None):
     nb = ctypes.c_uint64(0)
     _lib.check(L.tdp_run_programs_workspace(runs, len(plans), ctypes.byref(nb)), "tdp_run_programs_workspace")
     work = _lib.device_empty(nb.value)
-    _lib.check(L.tdp_run_programs(runs, len(plans), _lib.dptr(work), nb.value, _lib.stream_handle(stream)),
-               "tdp_run_programs")
+    stage = _staging(nb.value)
+    _lib.check(L.tdp_run_programs(runs, len(plans), _lib.dptr(work), nb.value, stage.data_ptr(), stage.numel(),
+                                  _lib.stream_handle(stream)), "tdp_run_programs")
     out = []
     for r, w in enumerate(witness):
         v = kernel.final_value(runs[r].value)

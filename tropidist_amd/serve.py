@@ -48,6 +48,21 @@ class Ticket:
         return None if self.t_done is None else self.t_done - self.t_submit
 
 
+def _dense_footprint(plans):
+    """Device bytes a dense request may hold at once: persistent-launch items for narrow plans; for a wide request
+    (a plan of 2^batch.WIDE_BITS entries or more) the fused tables of every plan plus the largest witness arena."""
+    from . import fused
+    if any(p.peak_bits >= batch.WIDE_BITS for p in plans):
+        wit = 0
+        for p in plans:
+            try:
+                wit = max(wit, fused.witness_bytes(p))
+            except ValueError:
+                wit = max(wit, batch.plan_bytes(p, True))
+        return sum(2 << p.peak_bits for p in plans) + wit
+    return sum(batch.plan_bytes(p, False) for p in plans) + max((batch.plan_bytes(p, True) for p in plans), default=0)
+
+
 class Server:
     def __init__(self, memory_bytes=64 << 30, dense_lanes=2, sparse_lanes=2, max_batch=1024, sparse_bytes=1 << 30):
         self.budget = memory_bytes
@@ -115,8 +130,7 @@ class Server:
             tk.order = order
             orders, flat, owner = batch.plan_problems([tk.problem], [order])
             tk.plans = (flat, owner)
-            tk.footprint = sum(batch.plan_bytes(p, False) for p in flat if p.feasible) + \
-                max((batch.plan_bytes(p, True) for p in flat if p.feasible), default=0)
+            tk.footprint = _dense_footprint([p for p in flat if p.feasible])
         return tk
 
     def _dense_lane(self, stream):
