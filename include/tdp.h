@@ -257,6 +257,10 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
                    int nq, const int32_t *weights, uint64_t *ops, int64_t ops_cap, int64_t *nops, int32_t *peak,
                    double *work, int32_t *columns, int32_t *feasible);
 
+/* r[i] = GF(2) rank of the first i columns of `order` (i = 0..n; out has n + 1 entries); columns as h_words, ncols
+ * bitsets of rw uint64 words.  Replaces the rank profile of cdist/trellis.py:20-36 (cut ranks, best_order). */
+int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *order, int n, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

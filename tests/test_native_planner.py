@@ -48,3 +48,20 @@ def test_reference_style_plan_objects_encode_identically():
     q = trellis.as_plan(Ref())
     assert (q.encode() == plan.encode()).all() and q.work() == plan.work() and q.peak_bits == plan.peak_bits
     assert trellis.as_plan(plan) is plan
+
+
+def test_native_cut_profile_equals_python():
+    """tdp_rank_profile gives the cut ranks of trellis.cut_profile (random problems, multi-word rows, banded)."""
+    from tropidist_amd import problem
+    rng = random.Random(9)
+    for t in range(200):
+        m, k, h, l = random_problem(rng)
+        order = list(range(len(h)))
+        rng.shuffle(order)
+        assert trellis.cut_profile_native(h, order, m) == trellis.cut_profile(h, order), t
+    for R, W in ((150, 12), (300, 20)):
+        p = problem.banded(R=R, N=4 * R, kmin=2, kmax=6, W=W, nobs=2, rho=0.1, seed=R)
+        order = list(range(p.ncols))
+        assert trellis.cut_profile_native(p.h, order, p.nrows) == trellis.cut_profile(p.h, order)
+        rev = order[::-1]
+        assert trellis.cut_profile_native(p.h, rev, p.nrows) == trellis.cut_profile(p.h, rev)

@@ -32,8 +32,9 @@ def best_order(p, seeds=0, time_limit=None):
     """(order, (peak cut rank, log2 volume)) of the narrower of the natural and the reversed column order (the
     first two candidates of cdist/auto.py:48-64)."""
     best = None
+    words = p.words()
     for c in (list(range(p.ncols)), list(reversed(range(p.ncols)))):
-        prof = trellis.cut_profile(p.h, c)
+        prof = trellis.cut_profile_native(p.h, c, p.nrows, words)
         cost = (max(prof), math.log2(sum(2.0 ** x for x in prof)))
         if best is None or cost < best[0]:
             best = (cost, c)

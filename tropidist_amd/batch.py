@@ -184,7 +184,7 @@ def plan_problems(problems, orders=None, compile_wide=True):
     orders = orders or [auto.best_order(p)[0] for p in problems]
     flat, owner = [], []
     for pi, (p, o) in enumerate(zip(problems, orders)):
-        plans = build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], o, p.nrows, p.w)
+        plans = build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], o, p.nrows, p.w, words=p.words())
         for i, plan in enumerate(plans):
             flat.append(plan)
             owner.append((pi, i))

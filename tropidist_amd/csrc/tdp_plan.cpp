@@ -228,4 +228,43 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
     return TDP_OK;
 }
 
+
+// r[i] = GF(2) rank of the first i columns of the order, i = 0..n (trellis.rank_profile, cdist/trellis.py:20-36).
+// Columns are ncols bitsets of rw words (h_words, column-major); pivots are kept by top bit with their word span.
+int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *order, int n, int32_t *out) {
+    if (!h_words || ncols < 0 || rw < 1 || !order || n < 0 || !out) {
+        tdp::set_error("tdp_rank_profile: bad arguments");
+        return TDP_EINVAL;
+    }
+    std::vector<Row> piv(64 * (size_t)rw);
+    std::vector<char> has(64 * (size_t)rw, 0);
+    int rank = 0;
+    out[0] = 0;
+    Row v;
+    v.w.assign(rw, 0);
+    for (int i = 0; i < n; i++) {
+        const int j = order[i];
+        if (j < 0 || j >= ncols) {
+            tdp::set_error("tdp_rank_profile: order entry %d out of range", j);
+            return TDP_EINVAL;
+        }
+        memcpy(v.w.data(), h_words + (size_t)j * rw, 8 * (size_t)rw);
+        v.lo = 0;
+        v.hi = rw;
+        v.trim();
+        while (!v.zero()) {
+            const int top = v.bitlen() - 1;
+            if (!has[top]) {
+                piv[top] = v;
+                has[top] = 1;
+                rank++;
+                break;
+            }
+            v.xor_with(piv[top]);
+        }
+        out[i + 1] = rank;
+    }
+    return TDP_OK;
+}
+
 }  // extern "C"

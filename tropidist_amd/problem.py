@@ -44,6 +44,14 @@ class Problem:
     def cost(self, support):
         return sum(self.weight(j) for j in support)
 
+    def words(self):
+        """The columns as the planner's uint64 bitsets (trellis.column_words), computed once per column list."""
+        from .trellis import column_words
+        c = self.__dict__.get("_words")
+        if c is None or c[0] is not self.h:
+            c = self.__dict__["_words"] = (self.h, column_words(self.h, self.nrows))
+        return c[1]
+
 
 def dedupe(h, l, w=None):
     """Drop all-zero columns and merge identical (h, l) columns, keeping the least cost (cdist/problem.py:47-64 for

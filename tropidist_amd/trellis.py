@@ -48,6 +48,51 @@ def cut_profile(cols, order):
     return [a + b - fwd[-1] for a, b in zip(fwd, bwd)]
 
 
+def column_words(cols, nrows):
+    """Columns as a (ncols, ceil(nrows / 64)) uint64 array (the bitset layout of the C++ planner)."""
+    rw = max(1, (nrows + 63) // 64)
+    if not cols:
+        return np.zeros((1, rw), dtype=np.uint64)
+    nb = 8 * rw
+    raw = b"".join(int(c).to_bytes(nb, "little") for c in cols)
+    return np.frombuffer(raw, dtype=np.uint64).reshape(len(cols), rw).copy()
+
+
+def parity_targets(l, obs_masks):
+    """targets[q, j] = parity of l[j] & obs_masks[q] (uint8), for the C++ planner."""
+    nq = len(obs_masks)
+    out = np.zeros((max(1, nq), max(1, len(l))), dtype=np.uint8)
+    if not l:
+        return out
+    if max(l) < (1 << 64) and max(obs_masks, default=0) < (1 << 64):
+        la = np.asarray(l, dtype=np.uint64)
+        for q, m in enumerate(obs_masks):
+            out[q, :len(l)] = (np.bitwise_count(la & np.uint64(m)) & 1).astype(np.uint8)
+        return out
+    for q, m in enumerate(obs_masks):
+        for j, x in enumerate(l):
+            out[q, j] = bin(x & m).count("1") & 1
+    return out
+
+
+def cut_profile_native(cols, order, nrows, words=None):
+    """cut_profile by libtdp's C++ rank profile (tdp_rank_profile); words: column_words(cols, nrows) if known."""
+    import ctypes
+    from . import _lib
+    L = _lib.lib()
+    hw = column_words(cols, nrows) if words is None else words
+    n = len(order)
+    fwd_o = np.ascontiguousarray(order, dtype=np.int32)
+    bwd_o = np.ascontiguousarray(fwd_o[::-1])
+    fwd = np.zeros(n + 1, dtype=np.int32)
+    bwd = np.zeros(n + 1, dtype=np.int32)
+    for o, r in ((fwd_o, fwd), (bwd_o, bwd)):
+        _lib.check(L.tdp_rank_profile(hw.ctypes.data_as(ctypes.c_void_p), len(cols), hw.shape[1],
+                                      o.ctypes.data_as(ctypes.c_void_p), n, r.ctypes.data_as(ctypes.c_void_p)),
+                   "tdp_rank_profile")
+    return (fwd + bwd[::-1] - fwd[-1]).tolist()
+
+
 def rows_from_columns(cols, order, nrows):
     """Row r as a bitmask over column positions of the order (cdist/trellis.py:47-57)."""
     rows = [0] * nrows
@@ -267,22 +312,16 @@ def _words(x, nw):
     return np.frombuffer(int(x).to_bytes(8 * nw, "little"), dtype=np.uint64)
 
 
-def build_plans_native(h, l, obs_masks, order, nrows, weights=None):
+def build_plans_native(h, l, obs_masks, order, nrows, weights=None, words=None):
     """Plans of several queries (one per obs_mask) by the C++ planner of libtdp (tdp_plan_build); identical op
-    streams to make_plan.  Plans with more than 63 active rows are refused (ValueError)."""
+    streams to make_plan.  Plans with more than 63 active rows are refused (ValueError).  words: column_words(h,
+    nrows) when the caller has it."""
     import ctypes
     from . import _lib
     L = _lib.lib()
     ncols, nq = len(order), len(obs_masks)
-    rw = max(1, (nrows + 63) // 64)
-    hw = np.zeros((max(1, len(h)), rw), dtype=np.uint64)
-    for j, c in enumerate(h):
-        if c:
-            hw[j] = _words(c, rw)
-    targets = np.zeros((max(1, nq), max(1, len(h))), dtype=np.uint8)
-    for q, m in enumerate(obs_masks):
-        for j, x in enumerate(l):
-            targets[q, j] = bin(x & m).count("1") & 1
+    hw = column_words(h, nrows) if words is None else words
+    targets = parity_targets(l, obs_masks)
     order_a = np.asarray(order, dtype=np.int32)
     w = None if weights is None else np.asarray(weights, dtype=np.int32)
     cap = ncols + 2 * (nrows + 1) + 4
