@@ -277,10 +277,21 @@ def bench_serving(args, requests):
                for a, b in zip(cap_tickets, tickets) if a.error is None and b.error is None)
     dl = [tk.latency for tk in tickets if tk.kind == "dense"]
     sl = [tk.latency for tk in tickets if tk.kind == "sparse"]
+    # plain GPU: a sample of the same requests answered one at a time, synchronously, by the single-problem paths
+    from tropidist_amd import auto
+    ns = min(args.serving_plain_sample, len(reqs))
+    t0 = time.perf_counter()
+    for kind, a in reqs[:ns]:
+        try:
+            auto.certify(a[0]) if kind == "dense" else serve.sparse_certify(a[0], a[1])
+        except Exception:                       # answered with an error, as the server does
+            pass
+    plain = ns / (time.perf_counter() - t0)
     return {"workload": "%d requests: 70%% dense width 10-20, 20%% width 21-25, 10%% sparse MITM K<=%d on QC codes "
                         "l,m 6-15; capacity with all submitted at once, then Poisson arrivals at %.0f/s (%.0f%% of "
                         "capacity)" % (len(reqs), args.serving_max_K, rate, 100 * rate / capacity),
             "capacity_certs_per_s": capacity, "capacity_wall_seconds": cap_wall, "rate": rate,
+            "plain_gpu_certs_per_s": plain, "plain_sample": ns, "capacity_vs_plain": capacity / plain,
             "certs_per_s": len(reqs) / wall, "wall_seconds": wall, "p50_s": serve.percentile(lat, 50),
             "p99_s": serve.percentile(lat, 99), "dense_p50_s": serve.percentile(dl, 50),
             "dense_p99_s": serve.percentile(dl, 99), "sparse_p50_s": serve.percentile(sl, 50),
@@ -331,6 +342,7 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=None,
                     help="Poisson arrival rate (default: --serving-load times the measured capacity)")
     ap.add_argument("--serving-load", type=float, default=0.8)
+    ap.add_argument("--serving-plain-sample", type=int, default=60)
     ap.add_argument("--serving-max-K", type=int, default=8)
     ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
                     help="only replay the dominant fused pass N times and a device copy of the same input size N "

@@ -36,6 +36,10 @@ const char *tdp_last_error(void);
 const char *tdp_version(void);
 int tdp_num_cus(int *cus);
 
+/* Copy `bytes` of device memory to host memory after the work queued on `stream` and wait for it (the small
+ * read-backs of counters and results between launches).  No reference counterpart. */
+int tdp_read_back(void *host, const void *d_src, uint64_t bytes, void *stream);
+
 /* ------------------------------------------------------------------ dense trellis, one op per call
  * All act on a uint8 device table d_T of n = 2^b entries, INF = 255, additions saturating at 255. */
 

@@ -31,6 +31,7 @@ SIGNATURES = {
     "tdp_run_pass": (ci, [vp, vp, vp, vp]),
     "tdp_witness_step_bytes": (ci, []),
     "tdp_run_record_bytes": (ci, []),
+    "tdp_read_back": (ci, [vp, vp, u64, vp]),
     "tdp_rank_profile": (ci, [vp, ci, ci, vp, ci, vp]),
     "tdp_run_programs_workspace": (ci, [vp, ci, ctypes.POINTER(u64)]),
     "tdp_run_programs": (ci, [vp, ci, vp, u64, vp, u64, vp]),

@@ -260,6 +260,14 @@ const char *tdp_last_error(void) { return g_err; }
 
 const char *tdp_version(void) { return "tropidist_amd 0.1 gfx950"; }
 
+int tdp_read_back(void *host, const void *d_src, uint64_t bytes, void *stream) {
+    hipStream_t st = as_stream(stream);
+    TDP_REQUIRE(host && d_src, "tdp_read_back: null pointer");
+    TDP_HIP_CHECK(hipMemcpyAsync(host, d_src, bytes, hipMemcpyDeviceToHost, st));
+    TDP_HIP_CHECK(hipStreamSynchronize(st));
+    return TDP_OK;
+}
+
 int tdp_num_cus(int *cus) {
     int dev = 0;
     TDP_HIP_CHECK(hipGetDevice(&dev));

@@ -195,16 +195,20 @@ class HipTables:
         self.L = _lib.lib()
         self.stream = stream
         self.counters = _lib.device_empty(64)
+        self._host_counters = np.zeros(8, dtype=np.uint64)
         self._bounds = {}
+        self._sh = _lib.stream_handle(stream)       # the query's stream, fixed for the table set's life
 
     def sh(self):
-        return _lib.stream_handle(self.stream)
+        return self._sh
 
     def _alloc(self, nstates):
         return _lib.device_empty(16 * max(1, nstates))
 
     def _sync_counters(self, k):
-        return self.counters[:8 * k].cpu().numpy().view(np.uint64).tolist()
+        _lib.check(self.L.tdp_read_back(self._host_counters.ctypes.data_as(ctypes.c_void_p), self.counters.data_ptr(),
+                                        8 * k, self._sh), "tdp_read_back")
+        return self._host_counters[:k].tolist()
 
     def index(self, tab):
         if tab.slots is None:
