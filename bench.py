@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "oracle")]
 
 HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
+T_START = time.time()
 
 
 def _gen_scan(i):
@@ -53,13 +54,19 @@ def generate(args):
     import multiprocessing as mp
     from tropidist_amd import workloads
     out = {}
+    reqs = workloads.serving_mix(args.serving_requests, K=(4, args.serving_max_K))
+    if args.gen_workers <= 0:                   # in process (profiled runs: no child processes)
+        if not args.skip_batched:
+            out["scan"] = [_gen_scan(i) for i in range(args.batch_instances)]
+        if not args.skip_serving:
+            out["requests"] = [_gen_request(r) for r in reqs]
+        return out
     ctx = mp.get_context("spawn")
-    with ctx.Pool(min(16, os.cpu_count() or 4)) as pool:
+    with ctx.Pool(min(args.gen_workers, os.cpu_count() or 4)) as pool:
         if not args.skip_batched:
             out["scan"] = pool.map(_gen_scan, range(args.batch_instances), chunksize=8)
         if not args.skip_serving:
-            out["requests"] = pool.map(_gen_request, workloads.serving_mix(args.serving_requests,
-                                                                            K=(4, args.serving_max_K)), chunksize=8)
+            out["requests"] = pool.map(_gen_request, reqs, chunksize=8)
     return out
 
 
@@ -332,6 +339,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--roofline-launches", type=int, default=60)
+    ap.add_argument("--gen-workers", type=int, default=16, help="input generator processes (0: in process)")
+    ap.add_argument("--skip-dense", action="store_true",
+                    help="leave out the headline section (value null; for profiling the other sections alone)")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-sparse", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
@@ -356,25 +366,33 @@ def main():
     if args.profile_pass:
         profile_pass(args)
         return
+    def progress(msg):
+        print("bench: %s (%.1f s)" % (msg, time.time() - T_START), file=sys.stderr, flush=True)
+
     inputs = generate(args)
+    progress("inputs generated")
     from tropidist_amd import _lib
     torch = _lib.torch()
     L = _lib.lib()
-    p, plans, values, dense = bench_dense(args, L, torch)
-    out = {
-        "metric": "Rank-DP states/s (dense trellis, sum of Plan.work / s)",
-        "value": dense["value"], "unit": "states/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dense["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u8", "data": "synthetic (seeded banded detector error models, quasi-cyclic codes)",
-        "config": dense["config"], "roofline": dense["roofline"],
-    }
+    out = {"metric": "Rank-DP states/s (dense trellis, sum of Plan.work / s)", "value": None, "unit": "states/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+           "data": "synthetic (seeded banded detector error models, quasi-cyclic codes)"}
+    if not args.skip_dense:
+        p, plans, values, dense = bench_dense(args, L, torch)
+        out.update(value=dense["value"], ms_per_step=dense["ms_per_step"], config=dense["config"],
+                   roofline=dense["roofline"])
+        progress("dense done")
     if not args.skip_batched:
         out["batched"] = bench_batched(args, inputs["scan"])
+        progress("batched done")
     if not args.skip_sparse:
         out["sparse"] = bench_sparse(args)
+        progress("sparse done")
     if not args.skip_serving:
         out["serving"] = bench_serving(args, inputs["requests"])
-    if not args.no_cpu:
+        progress("serving done")
+    if not args.no_cpu and not args.skip_dense:
         import oracle
         if oracle.have_ref():
             R = oracle.rlib("viterbi")
