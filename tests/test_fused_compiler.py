@@ -87,8 +87,8 @@ def test_banded_weighted_layouts():
 
 
 def test_pack_layout():
-    """Descriptor records: size of the C struct (496 bytes), fields where the executor reads them."""
-    assert fused.PASS_DTYPE.itemsize == 496
+    """Descriptor records: size of the C struct (880 bytes), fields where the executor reads them."""
+    assert fused.PASS_DTYPE.itemsize == 880
     p = problem.banded(R=40, N=200, kmin=2, kmax=5, W=8, nobs=1, rho=0.1, seed=5)
     plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
     prog = fused.compile_program(plan, min_bits=4)

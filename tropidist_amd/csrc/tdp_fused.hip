@@ -36,8 +36,11 @@ struct PassArgs {
     int32_t D, nlow, byte_store, b_in, b_out, nfree, pad;
     uint32_t lamw[8];                       // update j: lane mask | weight << 8
     uint32_t loww[8];                       // in-dword update t: lane mask | weight << 8
+    // per cube element q < 2^D, precomputed on the host for the stores: XOR over the directions in q of dout (low
+    // 32 bits, for tables of at most 2^32 entries) and dsel, and the v_perm selector that packs q's twisted lanes
+    uint32_t qout[32], qsel[32], qpk[32];
 };
-static_assert(sizeof(PassArgs) == 496, "pass descriptor layout");
+static_assert(sizeof(PassArgs) == 880, "pass descriptor layout");
 
 __device__ __forceinline__ uint32_t run_f0(uint32_t r) { return r & 0xFFu; }
 __device__ __forceinline__ uint32_t run_len(uint32_t r) { return (r >> 8) & 0xFFu; }
@@ -148,21 +151,27 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
     if (FREE_NEW) xn = map_bits<uint32_t>(a, MAP_NEW, block_bits, false) | map_bits<uint32_t>(a, MAP_NEW, lane, true);
     constexpr int Q = 1 << D;
     u16x2 lo[Q], hi[Q];
+    // uniform per-element deltas, each the previous one (q with its lowest bit cleared) XOR one direction's:
+    // one scalar XOR per element and quantity.  The twist selectors are affine in sigma:
+    // sel_lo(s) = sel_lo(0) ^ s * 0x10001, likewise sel_hi.
+    IDX cin[Q];
+    uint32_t cnew[Q], csig[Q];
+    cin[0] = 0;
+    cnew[0] = 0;
+    csig[0] = 0;
+#pragma unroll
+    for (int q = 1; q < Q; q++) {
+        const int j = __builtin_ctz(q), p = q & (q - 1);
+        cin[q] = cin[p] ^ (IDX)a.din[2 + j];
+        cnew[q] = cnew[p] ^ a.dnew[2 + j];
+        csig[q] = csig[p] ^ (a.lamw[j] & 3u) * 0x10001u;
+    }
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        IDX ai = bin;
-        uint32_t nz = xn, s = 0;
-#pragma unroll
-        for (int j = 0; j < D; j++)
-            if ((q >> j) & 1) {
-                ai ^= (IDX)a.din[2 + j];
-                nz ^= a.dnew[2 + j];
-                s ^= a.lamw[j] & 3u;
-            }
         uint32_t d = 0xFFFFFFFFu;
-        if (!nz) d = gload32(in, ai);
-        lo[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_lo(s)));
-        hi[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_hi(s)));
+        if (!(xn ^ cnew[q])) d = gload32(in, bin ^ cin[q]);
+        lo[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_lo(0) ^ csig[q]));
+        hi[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_hi(0) ^ csig[q]));
     }
     DirLoop<D, 0>::run(lo, hi, a);
 #pragma unroll
@@ -176,17 +185,24 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         IDX ao = bout;
-        uint32_t sel = xs, s = 0;
+        uint32_t sel = xs, s = 0, spk;
+        if (sizeof(IDX) == 4 && !BYTE) {
+            ao ^= (IDX)a.qout[q];
+            sel ^= a.qsel[q];
+            spk = a.qpk[q];
+        } else {
 #pragma unroll
-        for (int j = 0; j < D; j++)
-            if ((q >> j) & 1) {
-                ao ^= (IDX)a.dout[2 + j];
-                sel ^= a.dsel[2 + j];
-                s ^= a.lamw[j] & 3u;
-            }
+            for (int j = 0; j < D; j++)
+                if ((q >> j) & 1) {
+                    ao ^= (IDX)a.dout[2 + j];
+                    sel ^= a.dsel[2 + j];
+                    s ^= a.lamw[j] & 3u;
+                }
+            spk = sel_pack(s);
+        }
         if (!BYTE) {
             if (sel == a.sel_target)
-                gstore32(out, ao, __builtin_amdgcn_perm(as_u32(hi[q]), as_u32(lo[q]), sel_pack(s)));
+                gstore32(out, ao, __builtin_amdgcn_perm(as_u32(hi[q]), as_u32(lo[q]), spk));
         } else {
             const unsigned short r[4] = {lo[q].x, hi[q].x, lo[q].y, hi[q].y};
 #pragma unroll
@@ -265,6 +281,28 @@ static int check_pass(const PassArgs &a) {
     for (int j = 0; j < a.D; j++)
         TDP_REQUIRE((a.lamw[j] & 0xFF) < 4 && (a.lamw[j] >> 8) >= 1 && (a.lamw[j] >> 8) <= 255,
                     "pass update %d: bad lane mask/weight", j);
+    // the per-element tables must be the XORs of the direction deltas (they carry the addresses of 32-bit passes)
+    for (int q = 0; q < (1 << a.D); q++) {
+        uint64_t xi = 0, xo = 0;
+        uint32_t xn = 0, xs = 0, sg = 0;
+        for (int j = 0; j < a.D; j++)
+            if ((q >> j) & 1) {
+                xi ^= a.din[2 + j];
+                xo ^= a.dout[2 + j];
+                xn ^= a.dnew[2 + j];
+                xs ^= a.dsel[2 + j];
+                sg ^= a.lamw[j] & 3u;
+            }
+        uint32_t pk = 0;
+        for (uint32_t k = 0; k < 4; k++) {
+            uint32_t r = k ^ sg;
+            pk |= (((r & 1u) << 2) | (r & 2u)) << (8 * k);
+        }
+        (void)xi;
+        (void)xn;
+        TDP_REQUIRE(a.qout[q] == (uint32_t)xo && a.qsel[q] == xs && a.qpk[q] == pk,
+                    "pass element table %d inconsistent with its directions", q);
+    }
     for (int t = 0; t < a.nlow; t++)
         TDP_REQUIRE((a.loww[t] & 0xFF) >= 1 && (a.loww[t] & 0xFF) < 4 && (a.loww[t] >> 8) >= 1 &&
                     (a.loww[t] >> 8) <= 255, "pass low update %d: bad mask", t);

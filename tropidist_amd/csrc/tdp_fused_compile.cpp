@@ -28,6 +28,7 @@ struct PassArgs {
     uint32_t sel_target;
     int32_t D, nlow, byte_store, b_in, b_out, nfree, pad;
     uint32_t lamw[8], loww[8];              // lane mask | weight << 8
+    uint32_t qout[32], qsel[32], qpk[32];
 };
 struct WStep {
     uint64_t in_off, inmask, newmask, fixmask, fixval;
@@ -40,7 +41,7 @@ struct WStep {
     uint8_t out_var[48];
     uint8_t pad2[2];
 };
-static_assert(sizeof(PassArgs) == 496, "pass descriptor layout");
+static_assert(sizeof(PassArgs) == 880, "pass descriptor layout");
 static_assert(sizeof(WStep) == 240, "witness step layout");
 
 struct Update {
@@ -248,6 +249,28 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
     }
     for (size_t t = 0; t < lows.size() && t < 8; t++) {
         A.loww[t] = lows[t].first | (uint32_t)(lows[t].second > 255 ? 255 : lows[t].second) << 8;
+    }
+    for (int q = 0; q < (1 << std::min(D, 5)); q++) {
+        uint64_t xi = 0, xo = 0;
+        uint32_t xn = 0, xs = 0, sg = 0;
+        for (int j = 0; j < D && j < 5; j++)
+            if ((q >> j) & 1) {
+                xi ^= A.din[2 + j];
+                xo ^= A.dout[2 + j];
+                xn ^= A.dnew[2 + j];
+                xs ^= A.dsel[2 + j];
+                sg ^= dirs[j].lam;
+            }
+        uint32_t pk = 0;
+        for (uint32_t k = 0; k < 4; k++) {
+            uint32_t r = k ^ sg;
+            pk |= (((r & 1u) << 2) | (r & 2u)) << (8 * k);
+        }
+        (void)xi;
+        (void)xn;
+        A.qout[q] = (uint32_t)xo;
+        A.qsel[q] = xs;
+        A.qpk[q] = pk;
     }
     P.first = first;
     P.last = last;

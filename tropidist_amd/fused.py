@@ -482,7 +482,29 @@ def simulate(program, plan):
 PASS_DTYPE = np.dtype([("din", "<u8", 8), ("dout", "<u8", 8), ("dnew", "<u4", 8), ("dsel", "<u4", 8),
                        ("runs", "<u4", (4, MAX_RUNS)), ("nruns", "<i4", 4), ("sel_target", "<u4"),
                        ("D", "<i4"), ("nlow", "<i4"), ("byte_store", "<i4"), ("b_in", "<i4"), ("b_out", "<i4"),
-                       ("nfree", "<i4"), ("pad", "<i4"), ("lamw", "<u4", 8), ("loww", "<u4", 8)], align=True)
+                       ("nfree", "<i4"), ("pad", "<i4"), ("lamw", "<u4", 8), ("loww", "<u4", 8),
+                       ("qout", "<u4", 32), ("qsel", "<u4", 32), ("qpk", "<u4", 32)], align=True)
+
+
+def element_tables(p):
+    """Per cube element q < 2^D: the XORs over q's directions of dout (low 32 bits) and dsel, and the v_perm
+    selector that packs q's twisted lanes -- the tables the kernel's stores read."""
+    out = {k: [0] * 32 for k in ("qout", "qsel", "qpk")}
+    for q in range(1 << p.D):
+        xi = xo = xn = xs = sg = 0
+        for j in range(p.D):
+            if (q >> j) & 1:
+                xi ^= p.din[2 + j]
+                xo ^= p.dout[2 + j]
+                xn ^= p.dnew[2 + j]
+                xs ^= p.dsel[2 + j]
+                sg ^= p.lam[j]
+        pk = 0
+        for k in range(4):
+            r = k ^ sg
+            pk |= (((r & 1) << 2) | (r & 2)) << (8 * k)
+        out["qout"][q], out["qsel"][q], out["qpk"][q] = xo & 0xFFFFFFFF, xs, pk
+    return out
 
 
 def unpack_run(word):
@@ -519,6 +541,8 @@ def pack(program):
         r["byte_store"], r["b_in"], r["b_out"] = int(p.byte_store), p.b_in, p.b_out
         r["lamw"][:p.D] = [int(lam) | min(255, int(w)) << 8 for lam, w in zip(p.lam, p.w)]
         r["loww"][:len(p.low_lam)] = [int(lam) | min(255, int(w)) << 8 for lam, w in zip(p.low_lam, p.low_w)]
+        for k, v in element_tables(p).items():
+            r[k] = v
         segs[s] = (SEG_PASS, ip, 0)
         ip += 1
     return segs, arr
