@@ -12,6 +12,8 @@ Beside it (each can be skipped):
             certificates (distance + checked witness) per second from one persistent launch per pass, against the
             same plans certified one at a time with per-op kernels on a sample ("plain GPU");
   sparse    the two-block quasi-cyclic code l = m = 12 (288 columns), meet-in-the-middle sweep at threshold K;
+  tree      the tree-realization DP of a banded model (R=75 N=300 W=10) on a balanced tree: join pairs per second,
+            against the reference's tree_dp.c joins on the host;
   serving   the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse MITM on quasi-cyclic codes):
             capacity with every request submitted at once, then a Poisson stream at 80% of it, p50 / p99 latency;
   cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
@@ -236,6 +238,52 @@ def bench_batched(args, scan):
             "latency_s": batched_s}
 
 
+def _balanced(lo, hi):
+    if hi - lo == 1:
+        return lo
+    mid = (lo + hi) // 2
+    return (_balanced(lo, mid), _balanced(mid, hi))
+
+
+def bench_tree(args):
+    """Tree-realization DP (T rows): banded R=75 N=300 W=10, one logical row, a balanced tree over the natural
+    column order; join pairs per second on the GPU, and the reference's tree_dp.c joins (single-threaded, in the
+    same postorder schedule) on the host for the CPU figure."""
+    import oracle
+    from tropidist_amd import _lib, kernel, problem, trellis, tree_dp
+    t = _lib.torch()
+    p = problem.banded(R=75, N=300, kmin=2, kmax=5, W=10, nobs=1, rho=0.1, seed=10)
+    cols = [h | ((l & 1) << p.nrows) for h, l in zip(p.h, p.l)]
+    t0 = time.perf_counter()
+    prog = tree_dp.CompiledTreeDP(cols, _balanced(0, len(cols)))
+    compile_s = time.perf_counter() - t0
+    target = 1 << p.nrows
+    lim = dict(max_join_work=1 << 40, max_memory_bytes=1 << 36, max_seconds=600.0)
+    value, stats = prog.run(target, **lim)
+    t.cuda.synchronize()
+    reps = 3
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        value, stats = prog.run(target, **lim)
+    t.cuda.synchronize()
+    dt = (time.perf_counter() - t1) / reps
+    pairs = stats["join_work"]
+    dense = kernel.run_plan(trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows))[0]
+    out = {"workload": "banded R=75 N=300 k=2-5 W=10, 1 logical row, balanced tree over the natural order, seed 10",
+           "distance": value, "dense_distance": dense, "identical": value == dense, "join_pairs": int(pairs),
+           "max_join_dimension": stats["max_join_dimension"], "dtype": stats["dtype"],
+           "seconds": dt, "pairs_per_s": pairs / dt, "host_compile_seconds": round(compile_s, 3)}
+    if not args.no_cpu and oracle.have_ref():
+        c0 = time.perf_counter()
+        cv = oracle.tree_run(prog, target, impl="ref")
+        cs = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": pairs / cs, "unit": "join pairs/s", "cores": 1, "kind": "reference",
+                               "sample": "the whole query: the reference's tree_dp.c joins (single-threaded) in the "
+                                         "same postorder schedule", "seconds": cs, "distance": cv,
+                               "matches_gpu": cv == value}
+    return out
+
+
 def bench_sparse(args):
     from tropidist_amd import ordering, sparse_dp, workloads
     p = workloads.sparse_qc()
@@ -344,6 +392,7 @@ def main():
                     help="leave out the headline section (value null; for profiling the other sections alone)")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-sparse", action="store_true")
+    ap.add_argument("--skip-tree", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--plain-sample", type=int, default=24)
@@ -389,6 +438,9 @@ def main():
     if not args.skip_sparse:
         out["sparse"] = bench_sparse(args)
         progress("sparse done")
+    if not args.skip_tree:
+        out["tree"] = bench_tree(args)
+        progress("tree done")
     if not args.skip_serving:
         out["serving"] = bench_serving(args, inputs["requests"])
         progress("serving done")
