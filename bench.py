@@ -19,7 +19,7 @@ Beside it (each can be skipped):
   cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--skip-batched] [--skip-sparse]
-                    [--skip-serving] [--serving-requests 600] [--batch-instances 512] [--sparse-K 8]
+                    [--skip-serving] [--serving-requests 600] [--batch-instances 512] [--sparse-K 12]
 """
 import argparse
 import json
@@ -285,19 +285,43 @@ def bench_tree(args):
 
 
 def bench_sparse(args):
-    from tropidist_amd import ordering, sparse_dp, workloads
+    """cfg3: the QC code l = m = 12 at threshold K (its 77 row slots exceed the reference's 56, so the reference
+    cannot run it); beside it, QC l = m = 9 (50 slots) at K = 12 on the GPU and on the reference's sparse_dp.c."""
+    import oracle
+    from tropidist_amd import ordering, problem, sparse_dp, workloads
+    from tropidist_amd.problem import verify_witness
     p = workloads.sparse_qc()
     order = ordering.greedy_straddle_order(p.h, p.nrows)
     tr = sparse_dp.Trellis.for_query(p, 0, order)
     sparse_dp.query(tr, 2, mode="mitm", use_h=False, backend="hip")            # warm-up
     t0 = time.perf_counter()
-    r = sparse_dp.query(tr, args.sparse_K, mode="mitm", use_h=False, backend="hip")
+    r = sparse_dp.query(tr, args.sparse_K, mode="mitm", use_h=False, backend="hip", max_states=400_000_000)
     dt = time.perf_counter() - t0
     steps = sum(r["forward_counts"]) + sum(r["backward_counts"])
-    return {"workload": "two-block QC code l=m=12 (288 columns), seed 3000, observable 0, greedy order, "
-                        "MITM split K//2+1", "K": args.sparse_K, "value": r["value"], "width": max(tr.width),
-            "slots": tr.nslots, "state_steps": int(steps), "state_steps_per_s": steps / dt, "seconds": dt,
-            "peak_states": r["peak_states"], "witness_cols": len(r["witness"] or [])}
+    wit = r["witness"]
+    out = {"workload": "two-block QC code l=m=12 (288 columns), seed 3000, observable 0, greedy order, "
+                       "MITM split K//2+1", "K": args.sparse_K, "value": r["value"], "width": max(tr.width),
+           "slots": tr.nslots, "state_steps": int(steps), "state_steps_per_s": steps / dt, "seconds": dt,
+           "peak_states": r["peak_states"], "witness_cols": len(wit or []),
+           "witness_ok": bool(wit) and verify_witness(p, sorted(wit), r["value"], 0) if r["value"] else None}
+    q = problem.quasi_cyclic(9, 9, seed=3000)
+    t2 = sparse_dp.Trellis.for_query(q, 0, ordering.greedy_straddle_order(q.h, q.nrows))
+    t0 = time.perf_counter()
+    rh = sparse_dp.query(t2, 12, mode="mitm", use_h=False, backend="hip")
+    dh = time.perf_counter() - t0
+    st2 = sum(rh["forward_counts"]) + sum(rh["backward_counts"])
+    cmp_ = {"workload": "QC l=m=9 (162 columns, 50 slots), seed 3000, observable 0, K=12", "value": rh["value"],
+            "state_steps": int(st2), "seconds": dh, "state_steps_per_s": st2 / dh}
+    if not args.no_cpu and oracle.have_ref():
+        t0 = time.perf_counter()
+        rr = sparse_dp.query(t2, 12, mode="mitm", use_h=False, backend=oracle.RefTables(t2))
+        dr = time.perf_counter() - t0
+        cmp_["cpu_baseline"] = {"value": st2 / dr, "unit": "state-steps/s", "cores": oracle.rlib("viterbi").dp_num_threads(),
+                                "kind": "reference", "sample": "the same query through the reference's sparse_dp.c "
+                                "(sp_candidates / sp_merge / sp_join) under the same sweep logic", "seconds": dr,
+                                "matches_gpu": rr["value"] == rh["value"]}
+    out["reference_sized"] = cmp_
+    return out
 
 
 def _serve_run(serve, reqs, rate, seed):
@@ -396,7 +420,7 @@ def main():
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--plain-sample", type=int, default=24)
-    ap.add_argument("--sparse-K", type=int, default=8)
+    ap.add_argument("--sparse-K", type=int, default=12)
     ap.add_argument("--serving-requests", type=int, default=600)
     ap.add_argument("--serving-rate", type=float, default=None,
                     help="Poisson arrival rate (default: --serving-load times the measured capacity)")
