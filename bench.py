@@ -84,14 +84,15 @@ def _time_launches(torch, stream, launch, count):
     return ev0.elapsed_time(ev1) / 1e3 / count
 
 
-def dense_setup(L, torch):
-    """The headline instance, its plans and fused programs, and device tables (shared with the profiling mode)."""
+def dense_setup(L, torch, W=25, nobs=None):
+    """The headline instance (window W, its first `nobs` observables), its plans and fused programs, and device
+    tables (shared with the profiling mode)."""
     from tropidist_amd import _lib, fused, trellis, workloads
     t0 = time.time()
-    p = workloads.single_large()
+    p = workloads.single_large(W=W)
     order = list(range(p.ncols))
-    plans = [pl for pl in trellis.build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], order, p.nrows, p.w)
-             if pl.feasible]
+    obs = [1 << i for i in range(p.nobs if nobs is None else nobs)]
+    plans = [pl for pl in trellis.build_plans_native(p.h, p.l, obs, order, p.nrows, p.w) if pl.feasible]
     plan_seconds = time.time() - t0
     t0 = time.time()
     progs = [fused.program_for(pl) for pl in plans]
@@ -177,6 +178,8 @@ def bench_dense(args, L, torch):
     masks = [int(m) for c, m, _ in encoded[0].tolist() if c == 1 and (n >> 1) <= m < n][:1] or [n >> 1]
     upd_s = _time_launches(torch, stream, lambda: _lib.check(L.tdp_update_w(_lib.dptr(T0), n, masks[0], 2, sh),
                                                             "update"), args.roofline_launches)
+    del T0, T1
+    dram = dram_pass(args, L, torch, stream, sh)
     return p, plans, values, {
         "value": args.steps * total_work / elapsed, "ms_per_step": 1e3 * elapsed / args.steps,
         "config": {"workload": "single large dense Rank DP: banded R=600 N=6000 k=2-6 W=25, 4 logical rows "
@@ -198,8 +201,30 @@ def bench_dense(args, L, torch):
                      "pass": {"b_in": int(d["b_in"]), "b_out": int(d["b_out"]),
                               "updates": int(d["D"]) + int(d["nlow"]), "threads": 1 << int(d["nfree"])},
                      "per_op_update": {"kernel": "tdp::k_update_hi", "table_entries": n, "launch_seconds": upd_s,
-                                       "achieved": 2 * n / upd_s / 1e9, "frac": 2 * n / upd_s / HBM_PEAK}},
+                                       "achieved": 2 * n / upd_s / 1e9, "frac": 2 * n / upd_s / HBM_PEAK},
+                     "dram": dram},
     }
+
+
+def dram_pass(args, L, torch, stream, sh, W=27):
+    """The same pass kernel on tables too large for the 256 MB MALL: the widest pass of observable 0 of the W = 27
+    instance (2^28 -> 2^28 entries, 512 MiB moved per launch), timed by HIP events on its stream."""
+    import ctypes
+    from tropidist_amd import _lib
+    _, _, progs, _, T0, T1, _, _ = dense_setup(L, torch, W=W, nobs=1)
+    kname, pass_bytes, desc = dominant_pass(progs)
+    d = desc[0]
+    _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << int(d["b_in"]), 7, sh), "fill")
+    launches = max(4, args.roofline_launches // 4)
+    pass_s = _time_launches(torch, stream, lambda: _lib.check(
+        L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "tdp_run_pass"),
+        launches)
+    del T0, T1
+    torch.cuda.empty_cache()
+    return {"workload": "widest pass of the same banded instance at W=%d, observable 0: 2^%d -> 2^%d entries, "
+                        "D=%d" % (W, int(d["b_in"]), int(d["b_out"]), int(d["D"])),
+            "kernel": kname, "alg_bytes_per_launch": pass_bytes, "launch_seconds": pass_s,
+            "achieved": pass_bytes / pass_s / 1e9, "frac": pass_bytes / pass_s / HBM_PEAK, "launches": launches}
 
 
 def bench_batched(args, scan):
@@ -385,7 +410,8 @@ def profile_pass(args):
     from tropidist_amd import _lib
     torch = _lib.torch()
     L = _lib.lib()
-    p, plans, progs, peak, T0, T1, _, _ = dense_setup(L, torch)
+    p, plans, progs, peak, T0, T1, _, _ = dense_setup(L, torch, W=args.profile_width,
+                                                      nobs=None if args.profile_width == 25 else 1)
     kname, alg_bytes, desc = dominant_pass(progs)
     d = desc[0]
     b_in = int(d["b_in"])
@@ -430,6 +456,9 @@ def main():
     ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
                     help="only replay the dominant fused pass N times and a device copy of the same input size N "
                          "times (the reference kernel for scaling hardware byte counters), then exit")
+    ap.add_argument("--profile-width", type=int, default=25,
+                    help="band window of the instance whose widest pass --profile-pass replays (25: the headline, "
+                         "2^26 tables; 27: 2^28 tables, larger than the MALL)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (the sharding design is written up, "
