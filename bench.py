@@ -350,7 +350,7 @@ def bench_sparse(args):
 
 
 def _serve_run(serve, reqs, rate, seed):
-    srv = serve.Server(memory_bytes=96 << 30, dense_lanes=3, sparse_lanes=4)
+    srv = serve.Server(memory_bytes=96 << 30, dense_lanes=3, sparse_arena_bytes=16 << 30)
     try:
         if rate is None:                        # closed loop: every request waiting at t = 0
             t0 = time.perf_counter()

@@ -219,6 +219,47 @@ int tdp_sp_join(const void *d_f, uint64_t nf, const void *d_bents, const uint32_
 /* d_hist[v] = number of states of value v (256 device counters).  Replaces sp_hist (cdist/sparse_dp.c:238-242). */
 int tdp_sp_hist(const void *d_ents, uint64_t n, uint64_t *d_hist, void *stream);
 
+/* ------------------------------------------------------------------ many sparse queries at once
+ * A handle holding many meet-in-the-middle queries resident on one device (the query of
+ * cdist/sparse_dp.py:985-1095 with a uniform split, its column steps and joins from cdist/sparse_dp.c:116-236).
+ * tdp_spb_step advances every active query by one column in a few launches and one read-back, so queries admitted
+ * between steps share the device (continuous batching).  No reference counterpart: the reference runs one query
+ * at a time from Python (cdist/sparse_dp.py:621-820).  Every table stays on the device, with its index, until the
+ * query is released; memory comes from the handle's arena of arena_bytes.  Calls on one handle are serialised;
+ * all device work goes to `stream`. */
+#define TDP_SPB_RUNNING 0
+#define TDP_SPB_DONE 1        /* finished: value (or none) known */
+#define TDP_SPB_STATES 2      /* a table or changed list exceeded max_states (BudgetExceeded "states") */
+#define TDP_SPB_NOMEM 3       /* the arena could not hold the query's tables */
+
+/* NULL on failure (tdp_last_error). */
+void *tdp_spb_create(uint64_t arena_bytes, int max_jobs, void *stream);
+int tdp_spb_free(void *h);
+
+/* A query of N columns: prm holds per column p ten words -- the column's row-slot mask, the forward closing mask
+ * and bits, the backward closing mask and bits, each as (low 64 bits, high 56 bits) -- and tmask the target mask
+ * per cut (N + 1 pairs).  The backward sweep keeps weights <= K - split, the forward frontier weights
+ * <= split - 1; max_states 0 means no limit.  Writes the query's id. */
+int tdp_spb_add(void *h, const uint64_t *prm, const uint64_t *tmask, int32_t N, int32_t K, int32_t split,
+                uint64_t max_states, int64_t *id);
+
+/* One column of every active query; *active = queries not finished. */
+int tdp_spb_step(void *h, int32_t *active);
+
+/* Ids of the queries that finished (done or failed) since the last call, oldest first, at most cap of them. */
+int tdp_spb_poll(void *h, int64_t *ids, int32_t cap, int32_t *n);
+
+/* Status (TDP_SPB_*), the least weight <= K (-1 if none or not finished) and, if stats is not NULL, three counters:
+ * states summed over cuts, the largest table, the join cut. */
+int tdp_spb_result(void *h, int64_t id, int32_t *status, int32_t *value, uint64_t *stats);
+
+/* Column positions (indices into the query's column order) of a solution of the query's value, traced back through
+ * its kept tables on the device. */
+int tdp_spb_witness(void *h, int64_t id, int32_t *positions, int32_t cap, int32_t *n);
+
+/* Frees a finished query's tables and id. */
+int tdp_spb_release(void *h, int64_t id);
+
 /* ------------------------------------------------------------------ meet-in-the-middle lookup tables
  * Open addressing over 2^log2slots device slots; all subsets of size <= k of n columns keyed by the 128-bit
  * XOR-linear syndrome hash (zh, zl) with their logical label lg.  The table owns its device memory behind the
@@ -264,6 +305,19 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
 /* r[i] = GF(2) rank of the first i columns of `order` (i = 0..n; out has n + 1 entries); columns as h_words, ncols
  * bitsets of rw uint64 words.  Replaces the rank profile of cdist/trellis.py:20-36 (cut ranks, best_order). */
 int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *order, int n, int32_t *out);
+
+/* A column order with few open rows at every cut: greedy over the columns sharing a row with those already taken
+ * (fewest newly opened rows net of closed ones, then most open rows touched, then lowest index).  No reference
+ * counterpart: the reference's orders come from cdist/ordering.py:36-179; any order gives the same exact value. */
+int tdp_greedy_order(const uint64_t *h_words, int ncols, int nrows, int start, int32_t *order);
+
+/* The sparse trellis of nq queries in the word layout of tdp_spb_add (prm: nq x ncols x 10 words, tm:
+ * nq x (ncols + 1) x 2): rows in minimal span form, the reduced target row, slots by interval colouring, per
+ * position the slot mask and closing masks / bits of both directions, per cut the target mask.  Replaces the
+ * construction of cdist/sparse_dp.py:63-145 (Trellis.__init__).  feasible[q] = 0 for a target that reduces to
+ * zero; nslots[q] > max_slots leaves the query's words unwritten. */
+int tdp_sp_trellis(const uint64_t *h_words, int ncols, int nrows, const int32_t *order, const uint8_t *targets,
+                   int nq, int max_slots, uint64_t *prm, uint64_t *tm, int32_t *nslots, int32_t *feasible);
 
 #ifdef __cplusplus
 }

@@ -32,7 +32,7 @@ def _alone(kind, p, K):
 def test_results_alone_equal_results_in_the_mix():
     print("case serve mix", flush=True)
     reqs = _mix(42, 9100)
-    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=2, sparse_lanes=2, max_batch=8)
+    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=2, max_batch=8, sparse_arena_bytes=1 << 30)
     tickets = [srv.submit_dense(p) if kind == "dense" else srv.submit_sparse(p, K) for kind, p, K in reqs]
     recs = [tk.wait(timeout=300) for tk in tickets]
     srv.close()
@@ -48,7 +48,7 @@ def test_results_alone_equal_results_in_the_mix():
 def test_poisson_stream_completes_with_latencies():
     print("case serve poisson", flush=True)
     reqs = [("dense", (p,)) if kind == "dense" else ("sparse", (p, K)) for kind, p, K in _mix(30, 9300)]
-    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=2, sparse_lanes=1)
+    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=2, sparse_arena_bytes=1 << 30)
     tickets, wall = serve.run_poisson(srv, reqs, rate=50.0, seed=1)
     srv.close()
     lat = [tk.latency for tk in tickets]

@@ -65,3 +65,62 @@ def test_native_cut_profile_equals_python():
         assert trellis.cut_profile_native(p.h, order, p.nrows) == trellis.cut_profile(p.h, order)
         rev = order[::-1]
         assert trellis.cut_profile_native(p.h, rev, p.nrows) == trellis.cut_profile(p.h, rev)
+
+
+def _sparse_cases():
+    from sparse_helpers import random_problem as sparse_random_problem
+    from tropidist_amd import problem
+    rng = random.Random(11)
+    out = [sparse_random_problem(rng) for _ in range(300)]
+    out += [problem.quasi_cyclic(l, m, seed=s) for l, m, s in
+            [(6, 6, 3001), (6, 8, 3003), (9, 8, 3004), (12, 12, 3000), (15, 9, 5012), (6, 15, 5010)]]
+    return rng, out
+
+
+def test_greedy_order_native_equals_python():
+    """tdp_greedy_order gives greedy_straddle_order's column order exactly (random problems, QC codes, any start)."""
+    from tropidist_amd import ordering
+    rng, probs = _sparse_cases()
+    for p in probs:
+        start = rng.randrange(p.ncols) if p.ncols else 0
+        assert ordering.greedy_order_native(p.h, p.nrows, start=start) == \
+            ordering.greedy_straddle_order(p.h, p.nrows, start=start)
+
+
+def test_sparse_trellis_words_native_equal_python():
+    """tdp_sp_trellis gives, per observable, the feasibility, slot count and engine words of sparse_dp.Trellis
+    (random problems in random orders and QC codes in the greedy order)."""
+    import numpy as np
+
+    from tropidist_amd import ordering, sparse_batch, sparse_dp
+    rng, probs = _sparse_cases()
+    checked = 0
+    for p in probs:
+        order = list(range(p.ncols))
+        if p.ncols > 100:
+            order = ordering.greedy_straddle_order(p.h, p.nrows)
+        else:
+            rng.shuffle(order)
+        want = []
+        too_wide = None
+        for q in range(p.nobs):
+            try:
+                tr = sparse_dp.Trellis.for_query(p, q, order)
+            except ValueError as exc:
+                too_wide = str(exc)
+                break
+            if tr.feasible:
+                want.append((q,) + sparse_batch.trellis_words(tr))
+        if too_wide is not None:
+            try:
+                sparse_batch.query_words(p, order)
+            except ValueError as exc:
+                assert str(exc) == too_wide
+                continue
+            raise AssertionError("native planner accepted a trellis past the slot limit")
+        got = sparse_batch.query_words(p, order)
+        assert [g[0] for g in got] == [w[0] for w in want]
+        for (q, prm, tm), (_, wprm, wtm) in zip(got, want):
+            assert np.array_equal(prm[:p.ncols], wprm[:p.ncols]) and np.array_equal(tm, wtm), q
+            checked += 1
+    assert checked > 200

@@ -66,6 +66,16 @@ SIGNATURES = {
     "tdp_batch_run": (ci, [vp, vp, ci, vp, ci, vp]),
     "tdp_batch_item_bytes": (ci, []),
     "tdp_plan_build": (ci, [vp, ci, ci, vp, vp, ci, vp, vp, i64, vp, vp, vp, vp, vp]),
+    "tdp_greedy_order": (ci, [vp, ci, ci, ci, vp]),
+    "tdp_sp_trellis": (ci, [vp, ci, ci, vp, vp, ci, ci, vp, vp, vp, vp]),
+    "tdp_spb_create": (vp, [u64, ci, vp]),
+    "tdp_spb_free": (ci, [vp]),
+    "tdp_spb_add": (ci, [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, u64, ctypes.POINTER(i64)]),
+    "tdp_spb_step": (ci, [vp, ctypes.POINTER(ctypes.c_int32)]),
+    "tdp_spb_poll": (ci, [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "tdp_spb_result": (ci, [vp, i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), vp]),
+    "tdp_spb_witness": (ci, [vp, i64, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "tdp_spb_release": (ci, [vp, i64]),
 }
 
 

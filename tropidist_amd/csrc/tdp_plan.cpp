@@ -6,6 +6,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
+#include <queue>
 #include <vector>
 
 #include "../../include/tdp.h"
@@ -108,6 +110,44 @@ Row reduce_target_row(Row lrow, const std::vector<Row> &hrows, int npos) {
         lrow.xor_with(hrows[q]);
         clear_start(lrow);
     }
+    return lrow;
+}
+
+// Rows of the columns over positions of the order (trellis.rows_from_columns, cdist/trellis.py:47-57), trimmed.
+int rows_over_positions(const uint64_t *h_words, int ncols, int nrows, const int32_t *order, std::vector<Row> &rows) {
+    int rw = (nrows + 63) / 64, pw = (ncols + 63) / 64;
+    rows.assign(nrows, Row());
+    for (Row &r : rows) r.w.assign(pw > 0 ? pw : 1, 0);
+    for (int pos = 0; pos < ncols; pos++) {
+        int j = order[pos];
+        if (j < 0 || j >= ncols) {
+            tdp::set_error("order entry %d out of range", j);
+            return TDP_EINVAL;
+        }
+        const uint64_t *c = h_words + (int64_t)j * rw;
+        for (int k = 0; k < rw; k++)
+            for (uint64_t x = c[k]; x; x &= x - 1) {
+                int r = k * 64 + __builtin_ctzll(x);
+                if (r < nrows) rows[r].w[pos >> 6] |= 1ull << (pos & 63);
+            }
+    }
+    for (Row &r : rows) {
+        r.lo = 0;
+        r.hi = pw;
+        r.trim();
+    }
+    return TDP_OK;
+}
+
+Row target_row(const uint8_t *targets, const int32_t *order, int ncols) {
+    Row lrow;
+    int pw = (ncols + 63) / 64;
+    lrow.w.assign(pw > 0 ? pw : 1, 0);
+    for (int pos = 0; pos < ncols; pos++)
+        if (targets[order[pos]]) lrow.w[pos >> 6] |= 1ull << (pos & 63);
+    lrow.lo = 0;
+    lrow.hi = pw;
+    lrow.trim();
     return lrow;
 }
 
@@ -263,6 +303,158 @@ int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *
             v.xor_with(piv[top]);
         }
         out[i + 1] = rank;
+    }
+    return TDP_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+// The greedy straddle order of tropidist_amd/ordering.py: grow the processed set one column at a time, taking from
+// the columns that share a row with a processed one the column that opens the fewest new rows net of the rows it
+// closes, then the one touching the most open rows, then the lowest index; with no such column, the lowest
+// unprocessed index.  Columns are ncols bitsets of ceil(nrows / 64) words.
+int tdp_greedy_order(const uint64_t *h_words, int ncols, int nrows, int start, int32_t *order) {
+    if (!h_words || !order || ncols < 0 || nrows < 0) {
+        tdp::set_error("tdp_greedy_order: bad arguments");
+        return TDP_EINVAL;
+    }
+    if (ncols == 0) return TDP_OK;
+    const int rw = (nrows + 63) / 64 > 0 ? (nrows + 63) / 64 : 1;
+    std::vector<std::vector<int>> col_rows(ncols), row_cols(nrows);
+    for (int j = 0; j < ncols; j++)
+        for (int k = 0; k < rw; k++)
+            for (uint64_t x = h_words[(int64_t)j * rw + k]; x; x &= x - 1) {
+                int r = k * 64 + __builtin_ctzll(x);
+                if (r < nrows) {
+                    col_rows[j].push_back(r);
+                    row_cols[r].push_back(j);
+                }
+            }
+    std::vector<int> remaining(nrows), in_front(ncols, 0), front;
+    for (int r = 0; r < nrows; r++) remaining[r] = (int)row_cols[r].size();
+    std::vector<char> opened(nrows, 0), done(ncols, 0);
+    int current = (start >= 0 && start < ncols) ? start : 0, n = 0, next_free = 0;
+    while (true) {
+        order[n++] = current;
+        done[current] = 1;
+        for (int r : col_rows[current]) {
+            opened[r] = 1;
+            remaining[r]--;
+            for (int j : row_cols[r])
+                if (!done[j] && !in_front[j]) {
+                    in_front[j] = 1;
+                    front.push_back(j);
+                }
+        }
+        if (in_front[current]) {
+            in_front[current] = 0;
+            front.erase(std::find(front.begin(), front.end(), current));
+        }
+        if (n == ncols) break;
+        if (!front.empty()) {
+            int best = -1, bnet = 0, btouch = 0;
+            for (int j : front) {
+                int nw = 0, closes = 0, touched = 0;
+                for (int r : col_rows[j]) {
+                    if (!opened[r]) nw += remaining[r] > 1;
+                    else touched++;
+                    if (remaining[r] == 1 && opened[r]) closes++;
+                }
+                int net = nw - closes;
+                if (best < 0 || net < bnet || (net == bnet && (touched > btouch || (touched == btouch && j < best)))) {
+                    best = j;
+                    bnet = net;
+                    btouch = touched;
+                }
+            }
+            current = best;
+        } else {
+            while (done[next_free]) next_free++;
+            current = next_free;
+        }
+    }
+    return TDP_OK;
+}
+
+// The sparse trellis of each query (tropidist_amd/sparse_dp.Trellis, cdist/sparse_dp.py:63-145) in the word layout
+// of the batched engine: minimal span form of the rows over positions, the reduced target row, an interval colouring
+// of the closed row spans onto slots (rows by start; a slot frees once its row has ended; the lowest free slot first),
+// then per position the column's slot mask and the closing masks and bits of both directions, and per cut the
+// target row's slot while the row is active.  Query q's target column bits are targets[q * ncols + j].
+// feasible[q] = 0 when the target reduces to zero; nslots[q] is the slot count, and the words are written only when
+// it is at most max_slots (at most 120).  prm: nq x ncols x 10 words, tm: nq x (ncols + 1) x 2 words.
+int tdp_sp_trellis(const uint64_t *h_words, int ncols, int nrows, const int32_t *order, const uint8_t *targets,
+                   int nq, int max_slots, uint64_t *prm, uint64_t *tm, int32_t *nslots, int32_t *feasible) {
+    if (!h_words || !order || !targets || !prm || !tm || !nslots || !feasible || ncols < 0 || nrows < 0 || nq < 0 ||
+        max_slots < 1 || max_slots > 120) {
+        tdp::set_error("tdp_sp_trellis: bad arguments");
+        return TDP_EINVAL;
+    }
+    std::vector<Row> rows;
+    int rc = rows_over_positions(h_words, ncols, nrows, order, rows);
+    if (rc) return rc;
+    std::vector<Row> hrows = minimal_span_form(rows, ncols);
+    const int N = ncols;
+    for (int q = 0; q < nq; q++) {
+        Row lrow = reduce_target_row(target_row(targets + (int64_t)q * ncols, order, ncols), hrows, ncols);
+        nslots[q] = 0;
+        if (lrow.zero()) {
+            feasible[q] = 0;
+            continue;
+        }
+        feasible[q] = 1;
+        std::vector<const Row *> all;
+        for (const Row &r : hrows) all.push_back(&r);
+        all.push_back(&lrow);
+        const int R = (int)all.size(), L = R - 1;
+        std::vector<int> st(R), en(R), slot(R), idx(R);
+        for (int r = 0; r < R; r++) {
+            st[r] = all[r]->lowbit();
+            en[r] = all[r]->bitlen() - 1;
+            idx[r] = r;
+        }
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return st[a] < st[b]; });
+        std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, std::greater<std::pair<int, int>>>
+            busy;
+        std::priority_queue<int, std::vector<int>, std::greater<int>> freeq;
+        int ns = 0;
+        for (int r : idx) {
+            while (!busy.empty() && busy.top().first < st[r]) {
+                freeq.push(busy.top().second);
+                busy.pop();
+            }
+            int sl;
+            if (!freeq.empty()) {
+                sl = freeq.top();
+                freeq.pop();
+            } else {
+                sl = ns++;
+            }
+            slot[r] = sl;
+            busy.push({en[r], sl});
+        }
+        nslots[q] = ns;
+        if (ns > max_slots) continue;
+        uint64_t *P = prm + (int64_t)q * N * 10;
+        uint64_t *T = tm + (int64_t)q * (N + 1) * 2;
+        memset(P, 0, sizeof(uint64_t) * 10 * (size_t)N);
+        memset(T, 0, sizeof(uint64_t) * 2 * (size_t)(N + 1));
+        for (int r = 0; r < R; r++) {
+            const int w = slot[r] >> 6;                 // word of the slot bit: 0 = low, 1 = high
+            const uint64_t b = 1ull << (slot[r] & 63);
+            const Row &row = *all[r];
+            for (int k = row.lo; k < row.hi; k++)
+                for (uint64_t x = row.w[k]; x; x &= x - 1) P[(int64_t)(k * 64 + __builtin_ctzll(x)) * 10 + w] |= b;
+            P[(int64_t)en[r] * 10 + 2 + w] |= b;       // forward closing mask
+            P[(int64_t)st[r] * 10 + 6 + w] |= b;       // backward closing mask
+            if (r == L) {
+                P[(int64_t)en[r] * 10 + 4 + w] |= b;   // forward closing bits (the target row must be 1)
+                P[(int64_t)st[r] * 10 + 8 + w] |= b;   // backward closing bits
+                for (int c = st[r] + 1; c <= en[r]; c++) T[2 * c + w] |= b;
+            }
+        }
     }
     return TDP_OK;
 }

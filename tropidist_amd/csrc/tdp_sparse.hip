@@ -11,89 +11,9 @@
 //   key k^upd  (only when the partner is absent, v below the cap, closing bits match): a new state of value v + 1.
 // Every output key arises exactly once (XOR by upd is a bijection and the filter fixes the cleared bits), so the
 // outputs need no deduplication: they are appended with one atomic per wave and indexed afterwards.
-#include "tdp_common.h"
+#include "tdp_sparse.h"
 
 namespace tdp {
-
-constexpr int kSpBlock = 256;
-constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr uint64_t kKeyMask = (1ull << 56) - 1;
-
-struct Ent {
-    uint64_t lo, hv;
-};
-
-struct Key128 {
-    uint64_t lo, hi;
-};
-
-__device__ __forceinline__ uint64_t mix64(uint64_t x) {
-    x ^= x >> 33;
-    x *= 0xff51afd7ed558ccdull;
-    x ^= x >> 33;
-    x *= 0xc4ceb9fe1a85ec53ull;
-    x ^= x >> 33;
-    return x;
-}
-
-__device__ __forceinline__ uint64_t hash_key(uint64_t lo, uint64_t hi) { return mix64(lo ^ mix64(hi + 0x9e3779b97f4a7c15ull)); }
-
-// value of key (lo, hi) in the table, 255 if absent
-__device__ __forceinline__ uint32_t sp_find(const Ent *__restrict__ ents, const uint32_t *__restrict__ slots,
-                                            uint64_t capmask, uint64_t lo, uint64_t hi) {
-    uint64_t h = hash_key(lo, hi) & capmask;
-    for (;;) {
-        uint32_t e = slots[h];
-        if (e == kEmpty) return 255u;
-        Ent x = ents[e];
-        if (x.lo == lo && (x.hv & kKeyMask) == hi) return (uint32_t)(x.hv >> 56);
-        h = (h + 1) & capmask;
-    }
-}
-
-// Lookahead bound (cdist/sparse_dp.c:25-44): ceil((extra + sum_c w_c |(key ^ tmask) & mask_c|) / Q).
-struct BoundArgs {
-    const uint64_t *mask;     // 2 words per class (lo, hi)
-    const int64_t *w;
-    int ncls;
-    int on;
-    int64_t extra;
-    uint64_t tm_lo, tm_hi;
-    int K;
-};
-
-__device__ __forceinline__ bool keep_state(const BoundArgs &b, uint64_t lo, uint64_t hi, int v) {
-    if (!b.on) return true;
-    uint64_t ulo = lo ^ b.tm_lo, uhi = hi ^ b.tm_hi;
-    int64_t tot = b.extra;
-    for (int c = 0; c < b.ncls; c++)
-        tot += b.w[c] * (int64_t)(__popcll(ulo & b.mask[2 * c]) + __popcll(uhi & b.mask[2 * c + 1]));
-    const int64_t Q = 720720;
-    return v + (int)((tot + Q - 1) / Q) <= b.K;
-}
-
-// Wave-aggregated append: returns the slot of this lane's item in `out` (only meaningful where pred), one atomic
-// per wave.  Must be reached by every lane of the wave.
-__device__ __forceinline__ uint64_t wave_append(unsigned long long *counter, bool pred) {
-    uint64_t ballot = __ballot(pred);
-    uint32_t lane = __lane_id();
-    uint64_t below = ballot & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-    uint32_t rank = (uint32_t)__popcll(below);
-    uint32_t total = (uint32_t)__popcll(ballot);
-    int leader = ballot ? __ffsll((long long)ballot) - 1 : 0;
-    unsigned long long base = 0;
-    if (pred && (int)lane == leader) base = atomicAdd(counter, (unsigned long long)total);
-    base = __shfl(base, leader, 64);
-    return base + rank;
-}
-
-struct StepArgs {
-    uint64_t upd_lo, upd_hi;
-    uint64_t sm_lo, sm_hi, sb_lo, sb_hi;      // closing rows: mask and required bits
-    int cap, kb;                              // candidates from values < cap; table keeps values < kb
-    int want_changed;
-    uint64_t out_cap, ch_cap;
-};
 
 // counters: [0] out count, [1] changed count, [2] overflow flag
 __global__ __launch_bounds__(kSpBlock) void k_sp_step(const Ent *__restrict__ old, uint64_t n,
@@ -101,50 +21,8 @@ __global__ __launch_bounds__(kSpBlock) void k_sp_step(const Ent *__restrict__ ol
                                                       StepArgs a, BoundArgs b, Ent *__restrict__ out,
                                                       Ent *__restrict__ ch, unsigned long long *counters) {
     uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    bool live = i < n;
-    bool o1 = false, o2 = false, c1 = false, c2 = false;
-    Ent e1 = {0, 0}, e2 = {0, 0}, f1 = {0, 0}, f2 = {0, 0};
-    if (live) {
-        Ent x = old[i];
-        uint64_t klo = x.lo, khi = x.hv & kKeyMask;
-        int v = (int)(x.hv >> 56);
-        uint64_t qlo = klo ^ a.upd_lo, qhi = khi ^ a.upd_hi;
-        bool has_upd = (a.upd_lo | a.upd_hi) != 0;
-        uint32_t vq = has_upd ? sp_find(old, slots, capmask, qlo, qhi) : 255u;
-        uint64_t clo = ~a.sm_lo, chi = ~a.sm_hi & kKeyMask;
-        // key k
-        if ((klo & a.sm_lo) == a.sb_lo && (khi & a.sm_hi) == a.sb_hi) {
-            uint64_t alo = klo & clo, ahi = khi & chi;
-            bool cand = has_upd && vq != 255u && (int)vq < a.cap && keep_state(b, alo, ahi, (int)vq + 1);
-            int vc = (int)vq + 1;
-            if (cand && vc < v) {
-                Ent c = {alo, ahi | ((uint64_t)vc << 56)};
-                if (a.want_changed) { c1 = true; f1 = c; }
-                if (vc < a.kb) { o1 = true; e1 = c; }
-            } else if (v < a.kb && keep_state(b, alo, ahi, v)) {
-                o1 = true;
-                e1 = Ent{alo, ahi | ((uint64_t)v << 56)};
-            }
-        }
-        // key k ^ upd, a new state when the partner is absent
-        if (has_upd && vq == 255u && v < a.cap && (qlo & a.sm_lo) == a.sb_lo && (qhi & a.sm_hi) == a.sb_hi) {
-            uint64_t alo = qlo & clo, ahi = qhi & chi;
-            if (keep_state(b, alo, ahi, v + 1)) {
-                Ent c = {alo, ahi | ((uint64_t)(v + 1) << 56)};
-                if (a.want_changed) { c2 = true; f2 = c; }
-                if (v + 1 < a.kb) { o2 = true; e2 = c; }
-            }
-        }
-    }
-    uint64_t p;
-    p = wave_append(&counters[0], o1);
-    if (o1) { if (p < a.out_cap) out[p] = e1; else counters[2] = 1; }
-    p = wave_append(&counters[0], o2);
-    if (o2) { if (p < a.out_cap) out[p] = e2; else counters[2] = 1; }
-    p = wave_append(&counters[1], c1);
-    if (c1) { if (p < a.ch_cap) ch[p] = f1; else counters[2] = 1; }
-    p = wave_append(&counters[1], c2);
-    if (c2) { if (p < a.ch_cap) ch[p] = f2; else counters[2] = 1; }
+    StepOut r = sp_step_entry(old, i, n, slots, capmask, a, b);
+    sp_step_emit(r, out, a.out_cap, ch, a.ch_cap, counters);
 }
 
 __global__ __launch_bounds__(kSpBlock) void k_fill_slots(uint32_t *slots, uint64_t cap) {
@@ -156,13 +34,7 @@ __global__ __launch_bounds__(kSpBlock) void k_fill_slots(uint32_t *slots, uint64
 __global__ __launch_bounds__(kSpBlock) void k_index(const Ent *__restrict__ ents, uint64_t n, uint32_t *slots,
                                                     uint64_t capmask) {
     uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    if (i >= n) return;
-    Ent x = ents[i];
-    uint64_t h = hash_key(x.lo, x.hv & kKeyMask) & capmask;
-    for (;;) {
-        if (atomicCAS(&slots[h], kEmpty, (uint32_t)i) == kEmpty) return;
-        h = (h + 1) & capmask;
-    }
+    if (i < n) sp_insert(ents, i, slots, capmask);
 }
 
 __global__ __launch_bounds__(kSpBlock) void k_lookup(const Ent *__restrict__ ents, const uint32_t *__restrict__ slots,

@@ -67,6 +67,24 @@ def greedy_straddle_order(cols, nrows, start=0):
     return order
 
 
+def greedy_order_native(cols, nrows, words=None, start=0):
+    """greedy_straddle_order by libtdp's C++ planner (tdp_greedy_order): the same order, without the Python loop.
+    words: trellis.column_words(cols, nrows) when the caller has them."""
+    import ctypes
+
+    import numpy as np
+
+    from . import _lib
+    n = len(cols)
+    if n == 0:
+        return []
+    hw = words if words is not None else trellis.column_words(cols, nrows)
+    out = np.zeros(n, dtype=np.int32)
+    _lib.check(_lib.lib().tdp_greedy_order(hw.ctypes.data_as(ctypes.c_void_p), n, nrows, start,
+                                           out.ctypes.data_as(ctypes.c_void_p)), "tdp_greedy_order")
+    return out.tolist()
+
+
 def order_width(cols, order):
     """Largest cut rank of an order."""
     return max(trellis.cut_profile(cols, order))

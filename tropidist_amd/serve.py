@@ -1,6 +1,6 @@
 """Serving loop for exact distance queries on one GPU: continuous batching with admission by device footprint.
 
-    srv = Server(memory_bytes=..., dense_lanes=2, sparse_lanes=2)
+    srv = Server(memory_bytes=..., dense_lanes=2, sparse_arena_bytes=4 << 30)
     ticket = srv.submit_dense(problem, order=None)          # certificate of the dense DP
     ticket = srv.submit_sparse(problem, K, order=None)      # least weight <= K by the sparse MITM DP, or None
     ticket.wait() -> record;  srv.close()
@@ -9,19 +9,25 @@ Dense requests wait in a FIFO queue.  Each dense lane is a thread with its own H
 every waiting request whose plans fit in the free device budget (in arrival order, up to max_batch), runs them as
 one batched launch (batch.certify_planned: one persistent kernel for the values, one for the witnesses) and
 completes them; requests that arrive meanwhile are admitted by the next free lane, so batches of different lanes
-overlap on the device.  Sparse requests run on sparse lanes (threads with their own streams) through
-sparse_dp.query, one request per lane at a time, admitted against the same budget with a reservation of
-`sparse_bytes`.  Host planning of a request happens on its lane before admission.
+overlap on the device.  Host planning of a request happens on its lane before admission.
 
-The result of a request does not depend on what it was batched with: every plan runs alone in its workgroup on
-its own tables (tests/test_gpu_serve.py checks each result alone and in the mix)."""
+Sparse requests go to one engine thread with its own stream and a resident sparse_batch.Engine, whose arena
+(sparse_arena_bytes, held for the server's life) is set apart from the dense budget.  Between any two engine steps
+the thread admits every waiting sparse request (every observable a query of the engine) and completes those whose
+queries have finished, so sparse requests that arrive while others run join them on the device, one column of all of
+them per step.  A request the busy arena cannot hold is retried alone, and one that does not fit alone is answered by
+the one-at-a-time path (sparse_certify).
+
+The result of a request does not depend on what it was batched with: every dense plan runs alone in its workgroup on
+its own tables, every sparse query on its own tables and counters (tests/test_gpu_serve.py checks each result alone
+and in the mix)."""
 from __future__ import annotations
 
 import collections
 import threading
 import time
 
-from . import _lib, auto, batch, sparse_dp
+from . import _lib, auto, batch, sparse_batch, sparse_dp
 
 
 class Ticket:
@@ -64,11 +70,14 @@ def _dense_footprint(plans):
 
 
 class Server:
-    def __init__(self, memory_bytes=64 << 30, dense_lanes=2, sparse_lanes=2, max_batch=1024, sparse_bytes=1 << 30):
-        self.budget = memory_bytes
+    def __init__(self, memory_bytes=64 << 30, dense_lanes=2, max_batch=1024, sparse_arena_bytes=4 << 30,
+                 sparse_max_states=30_000_000):
+        if sparse_arena_bytes >= memory_bytes:
+            raise ValueError("the sparse arena (%d bytes) must leave room for dense requests in %d bytes"
+                             % (sparse_arena_bytes, memory_bytes))
+        self.budget = memory_bytes - sparse_arena_bytes
         self.in_use = 0
         self.max_batch = max_batch
-        self.sparse_bytes = sparse_bytes
         self.dense_q = collections.deque()
         self.sparse_q = collections.deque()
         self.cv = threading.Condition()
@@ -81,10 +90,12 @@ class Server:
             th = threading.Thread(target=self._dense_lane, args=(t.cuda.Stream(),), daemon=True)
             th.start()
             self.threads.append(th)
-        for k in range(sparse_lanes):
-            th = threading.Thread(target=self._sparse_lane, args=(t.cuda.Stream(),), daemon=True)
-            th.start()
-            self.threads.append(th)
+        sst = t.cuda.Stream()
+        self.sparse_engine = sparse_batch.Engine(arena_bytes=sparse_arena_bytes, stream=sst,
+                                                 max_states=sparse_max_states)
+        th = threading.Thread(target=self._sparse_engine_loop, args=(sst,), daemon=True)
+        th.start()
+        self.threads.append(th)
 
     # ------------------------------------------------------------------ submission
     def submit_dense(self, problem, order=None):
@@ -107,6 +118,7 @@ class Server:
             self.cv.notify_all()
         for th in self.threads:
             th.join()
+        self.sparse_engine.close()
 
     # ------------------------------------------------------------------ lanes
     def _finish(self, tickets, results=None, error=None):
@@ -175,26 +187,69 @@ class Server:
                 except Exception as exc:          # delivered to every waiter of the failed batch
                     self._finish(take, error=exc)
 
-    def _sparse_lane(self, stream):
+    def _sparse_engine_loop(self, stream):
         t = _lib.torch()
-        while True:
-            with self.cv:
-                while (not self.sparse_q or self.in_use + self.sparse_bytes > self.budget) and not self.closed:
-                    self.cv.wait(0.01)
-                if not self.sparse_q and self.closed:
-                    return
-                if not self.sparse_q:
+        eng = self.sparse_engine
+        owner, retry = {}, collections.deque()
+
+        def complete(r):
+            if r.error is not None:
+                self._finish([r.tag], error=r.error)
+            else:
+                self._finish([r.tag], [r.record])
+
+        try:
+            while True:
+                with self.cv:
+                    while not self.sparse_q and not owner and not retry and not self.closed:
+                        self.cv.wait()
+                    if self.closed and not self.sparse_q and not owner and not retry:
+                        return
+                    new = list(self.sparse_q)
+                    self.sparse_q.clear()
+                now = time.perf_counter()
+                reqs = []
+                for tk in new:
+                    tk.t_start = now
+                    reqs.append(sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk))
+                if not owner and retry:             # alone now: the request the busy arena could not hold
+                    reqs.append(retry.popleft())
+                for r in reqs:
+                    try:
+                        if not r.trs:
+                            r.plan()
+                        if r.trs:
+                            r.admit(eng, owner)
+                            continue
+                        r.finish(eng)
+                    except Exception as exc:
+                        r.error = exc
+                    complete(r)
+                if not owner:
                     continue
-                tk = self.sparse_q.popleft()
-                tk.footprint = self.sparse_bytes
-                self.in_use += tk.footprint
-                tk.t_start = time.perf_counter()
-            try:
                 with t.cuda.stream(stream):
-                    rec = sparse_certify(tk.problem, tk.K, tk.order)
-                self._finish([tk], [rec])
-            except Exception as exc:
-                self._finish([tk], error=exc)
+                    eng.step()
+                for rid in eng.poll():
+                    r = owner.pop(rid)
+                    r.left -= 1
+                    if r.left:
+                        continue
+                    shared = bool(owner)
+                    r.finish(eng)
+                    if isinstance(r.error, MemoryError):
+                        r.error = None
+                        if shared:
+                            retry.append(r)
+                            continue
+                        try:                        # too large for the arena alone: checkpointed sweeps
+                            r.record = sparse_certify(r.problem, r.K, r.order)
+                        except Exception as exc:
+                            r.error = exc
+                    complete(r)
+        finally:
+            for r in list(owner.values()) + list(retry):
+                if r.tag.t_done is None:
+                    self._finish([r.tag], error=RuntimeError("server closed"))
 
     # ------------------------------------------------------------------ statistics
     def latencies(self, kind=None):
