@@ -48,7 +48,9 @@ def _gen_request(req):
     if kind == "sparse":
         return kind, problem.quasi_cyclic(a["l"], a["m"], seed=a["seed"]), a["K"]
     rng = np.random.default_rng(a["seed"])
-    return kind, workloads.banded_instance(rng, a["seed"], a["width"], a["weighted"]), None
+    p = workloads.banded_instance(rng, a["seed"], a["width"], a["weighted"])
+    p.band_width = a["width"]                   # the request's class (narrow below 21, wide from 21)
+    return kind, p, None
 
 
 def generate(args):
@@ -350,7 +352,9 @@ def bench_sparse(args):
 
 
 def _serve_run(serve, reqs, rate, seed):
-    srv = serve.Server(memory_bytes=96 << 30, dense_lanes=3, sparse_arena_bytes=16 << 30)
+    # 160 GiB of the 288 GiB HBM: 80 GiB for dense tables and witness arenas, 80 GiB for the resident sparse tables
+    # (all sparse requests of the closed-loop run hold about 65 GB at once)
+    srv = serve.Server(memory_bytes=160 << 30, dense_lanes=3, sparse_arena_bytes=80 << 30)
     try:
         if rate is None:                        # closed loop: every request waiting at t = 0
             t0 = time.perf_counter()
@@ -384,18 +388,23 @@ def bench_serving(args, requests):
     # plain GPU: a sample of the same requests answered one at a time, synchronously, by the single-problem paths
     from tropidist_amd import auto
     ns = min(args.serving_plain_sample, len(reqs))
+    per_class = {}
     t0 = time.perf_counter()
-    for kind, a in reqs[:ns]:
+    for (kind, a), (_, p, _) in zip(reqs[:ns], requests[:ns]):
+        t1 = time.perf_counter()
         try:
             auto.certify(a[0]) if kind == "dense" else serve.sparse_certify(a[0], a[1])
         except Exception:                       # answered with an error, as the server does
             pass
+        c = "sparse" if kind == "sparse" else ("dense_wide" if getattr(p, "band_width", 0) >= 21 else "dense_narrow")
+        per_class.setdefault(c, []).append(time.perf_counter() - t1)
     plain = ns / (time.perf_counter() - t0)
     return {"workload": "%d requests: 70%% dense width 10-20, 20%% width 21-25, 10%% sparse MITM K<=%d on QC codes "
                         "l,m 6-15; capacity with all submitted at once, then Poisson arrivals at %.0f/s (%.0f%% of "
                         "capacity)" % (len(reqs), args.serving_max_K, rate, 100 * rate / capacity),
             "capacity_certs_per_s": capacity, "capacity_wall_seconds": cap_wall, "rate": rate,
             "plain_gpu_certs_per_s": plain, "plain_sample": ns, "capacity_vs_plain": capacity / plain,
+            "plain_ms_per_request": {c: round(1e3 * sum(v) / len(v), 2) for c, v in sorted(per_class.items())},
             "certs_per_s": len(reqs) / wall, "wall_seconds": wall, "p50_s": serve.percentile(lat, 50),
             "p99_s": serve.percentile(lat, 99), "dense_p50_s": serve.percentile(dl, 50),
             "dense_p99_s": serve.percentile(dl, 99), "sparse_p50_s": serve.percentile(sl, 50),

@@ -249,8 +249,12 @@ int tdp_spb_step(void *h, int32_t *active);
 /* Ids of the queries that finished (done or failed) since the last call, oldest first, at most cap of them. */
 int tdp_spb_poll(void *h, int64_t *ids, int32_t cap, int32_t *n);
 
-/* Status (TDP_SPB_*), the least weight <= K (-1 if none or not finished) and, if stats is not NULL, three counters:
- * states summed over cuts, the largest table, the join cut. */
+/* Arena and progress counters: out[0] arena bytes, [1] bytes in use, [2] most bytes in use so far, [3] queries
+ * failed for lack of arena, [4] engine steps so far, [5] queries running. */
+int tdp_spb_stats(void *h, uint64_t *out);
+
+/* Status (TDP_SPB_*), the least weight <= K (-1 if none or not finished) and, if stats is not NULL, four counters:
+ * states summed over cuts, the largest table, the join cut, the most arena bytes the query held. */
 int tdp_spb_result(void *h, int64_t id, int32_t *status, int32_t *value, uint64_t *stats);
 
 /* Column positions (indices into the query's column order) of a solution of the query's value, traced back through

@@ -72,6 +72,7 @@ SIGNATURES = {
     "tdp_spb_free": (ci, [vp]),
     "tdp_spb_add": (ci, [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, u64, ctypes.POINTER(i64)]),
     "tdp_spb_step": (ci, [vp, ctypes.POINTER(ctypes.c_int32)]),
+    "tdp_spb_stats": (ci, [vp, vp]),
     "tdp_spb_poll": (ci, [vp, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tdp_spb_result": (ci, [vp, i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), vp]),
     "tdp_spb_witness": (ci, [vp, i64, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),

@@ -1,6 +1,6 @@
 """Front end of the dense exact evaluator on the GPU (the call surface of the reference's cdist/auto.py:43-182):
 
-    best_order(p)              the column order to plan with (natural or reversed, whichever is narrower)
+    best_order(p)              the column order to plan with (the natural order and its cut profile)
     estimate(p, ...)           plan cost without running anything: table bytes, work, predicted seconds
     dp_distance(p, order)      exact distance, one GPU plan per logical observable
     dp_witness(p, order, obs)  columns of a minimum-cost solution of one observable, by GPU traceback
@@ -29,25 +29,27 @@ def predicted_seconds(work):
 
 
 def best_order(p, seeds=0, time_limit=None):
-    """(order, (peak cut rank, log2 volume)) of the narrower of the natural and the reversed column order (the
-    first two candidates of cdist/auto.py:48-64)."""
-    best = None
-    words = p.words()
-    for c in (list(range(p.ncols)), list(reversed(range(p.ncols)))):
-        prof = trellis.cut_profile_native(p.h, c, p.nrows, words)
-        cost = (max(prof), math.log2(sum(2.0 ** x for x in prof)))
-        if best is None or cost < best[0]:
-            best = (cost, c)
-    return best[1], best[0]
+    """(order, (peak cut rank, log2 volume)) of the natural column order, the first candidate of
+    cdist/auto.py:48-64.  The second candidate there, the reversed order, has the reversed cut profile (its prefix
+    and suffix ranks trade places), hence the same peak and volume, and ties go to the first candidate: it never
+    wins, so it is not evaluated."""
+    c = list(range(p.ncols))
+    prof = trellis.cut_profile_native(p.h, c, p.nrows, p.words())
+    return c, (max(prof), math.log2(sum(2.0 ** x for x in prof)))
 
 
-def _plans(p, order):
-    """One plan per logical observable; the span form of H is shared (cdist/auto.py:67-76)."""
-    hrows = trellis.minimal_span_form(trellis.rows_from_columns(p.h, order, p.nrows))
-    pw = None if p.w is None else [p.w[j] for j in order]
-    for i in range(p.nobs):
-        lrow = trellis.reduce_target_row(trellis.target_row(p.l, 1 << i, order), hrows)
-        yield i, trellis.make_plan(hrows, lrow, len(order), pw)
+def _plans(p, order, observables=None):
+    """One plan per logical observable (or per listed observable); the span form of H is shared
+    (cdist/auto.py:67-76).  Built by the C++ planner, which gives trellis.make_plan's plans exactly."""
+    obs = list(range(p.nobs)) if observables is None else list(observables)
+    try:
+        plans = trellis.build_plans_native(p.h, p.l, [1 << i for i in obs], order, p.nrows, p.w, words=p.words())
+    except ValueError:                  # more than 63 active rows: plans too wide for any table, made in Python
+        hrows = trellis.minimal_span_form(trellis.rows_from_columns(p.h, order, p.nrows))
+        pw = None if p.w is None else [p.w[j] for j in order]
+        plans = [trellis.make_plan(hrows, trellis.reduce_target_row(trellis.target_row(p.l, 1 << i, order), hrows),
+                                   len(order), pw) for i in obs]
+    return list(zip(obs, plans))
 
 
 def estimate(p, order=None, max_table_bytes=None):
@@ -141,12 +143,9 @@ def witness_plan(plan, order, max_bytes=None, stream=None, executor="fused"):
 def dp_witness(p, order, obs, max_bytes=None):
     """Columns of a minimum-cost solution of min{cost(e) : H e = 0, (L e)_obs = 1} (cdist/auto.py:126-182); None if
     no logical exists or the traceback workspace exceeds max_bytes; OverflowError at 255."""
-    hrows = trellis.minimal_span_form(trellis.rows_from_columns(p.h, order, p.nrows))
-    lrow = trellis.reduce_target_row(trellis.target_row(p.l, 1 << obs, order), hrows)
-    if lrow == 0:
+    plan = _plans(p, order, [obs])[0][1]
+    if not plan.feasible:
         return None
-    pw = None if p.w is None else [p.w[j] for j in order]
-    plan = trellis.make_plan(hrows, lrow, len(order), pw)
     r = witness_plan(plan, order, max_bytes=max_bytes)
     return None if r is None else r[1]
 

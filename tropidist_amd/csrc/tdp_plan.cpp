@@ -269,13 +269,16 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
 }
 
 
-// r[i] = GF(2) rank of the first i columns of the order, i = 0..n (trellis.rank_profile, cdist/trellis.py:20-36).
-// Columns are ncols bitsets of rw words (h_words, column-major); pivots are kept by top bit with their word span.
-int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *order, int n, int32_t *out) {
-    if (!h_words || ncols < 0 || rw < 1 || !order || n < 0 || !out) {
-        tdp::set_error("tdp_rank_profile: bad arguments");
-        return TDP_EINVAL;
-    }
+// r[i] = GF(2) rank of the first i columns of the order (trellis.rank_profile, cdist/trellis.py:20-36).
+// Columns are ncols bitsets of rw words (h_words, column-major); pivots are kept by one end bit with their word span.
+// The rank does not depend on which end the pivots use, but the work does: a column reduced against pivots of
+// earlier columns gains bits away from its pivot end, so the pivot end is chosen to point where the order is going
+// (the lowest row when the columns' rows rise along the order, as in a banded matrix taken left to right, the highest
+// when they fall), which keeps the cascade of reductions short.
+}  // extern "C"
+
+template <bool LOW>
+static int rank_profile_by(const uint64_t *h_words, int ncols, int rw, const int32_t *order, int n, int32_t *out) {
     std::vector<Row> piv(64 * (size_t)rw);
     std::vector<char> has(64 * (size_t)rw, 0);
     int rank = 0;
@@ -293,18 +296,44 @@ int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *
         v.hi = rw;
         v.trim();
         while (!v.zero()) {
-            const int top = v.bitlen() - 1;
-            if (!has[top]) {
-                piv[top] = v;
-                has[top] = 1;
+            const int key = LOW ? v.lowbit() : v.bitlen() - 1;
+            if (!has[key]) {
+                piv[key] = v;
+                has[key] = 1;
                 rank++;
                 break;
             }
-            v.xor_with(piv[top]);
+            v.xor_with(piv[key]);
         }
         out[i + 1] = rank;
     }
     return TDP_OK;
+}
+
+extern "C" {
+
+int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *order, int n, int32_t *out) {
+    if (!h_words || ncols < 0 || rw < 1 || !order || n < 0 || !out) {
+        tdp::set_error("tdp_rank_profile: bad arguments");
+        return TDP_EINVAL;
+    }
+    // direction of the order: mean lowest row of its first and last eighth of columns
+    auto lowrow = [&](int j) -> double {
+        if (j < 0 || j >= ncols) return 0.0;
+        for (int k = 0; k < rw; k++) {
+            uint64_t x = h_words[(size_t)j * rw + k];
+            if (x) return k * 64 + __builtin_ctzll(x);
+        }
+        return 0.0;
+    };
+    const int m = std::max(1, n / 8);
+    double head = 0, tail = 0;
+    for (int i = 0; i < m && i < n; i++) {
+        head += lowrow(order[i]);
+        tail += lowrow(order[n - 1 - i]);
+    }
+    return head <= tail ? rank_profile_by<true>(h_words, ncols, rw, order, n, out)
+                        : rank_profile_by<false>(h_words, ncols, rw, order, n, out);
 }
 
 }  // extern "C"

@@ -210,6 +210,8 @@ public:
         free_.clear();
         free_[0] = s;
     }
+    uint64_t in_use = 0, peak = 0;
+
     uint8_t *alloc(uint64_t bytes) {
         bytes = align_up(bytes ? bytes : 1);
         for (auto it = free_.begin(); it != free_.end(); ++it) {
@@ -217,12 +219,15 @@ public:
             uint64_t off = it->first, left = it->second - bytes;
             free_.erase(it);
             if (left) free_[off + bytes] = left;
+            in_use += bytes;
+            peak = std::max(peak, in_use);
             return base + off;
         }
         return nullptr;
     }
     void release(uint8_t *p, uint64_t bytes) {
         bytes = align_up(bytes ? bytes : 1);
+        in_use -= bytes;
         uint64_t off = (uint64_t)(p - base);
         auto next = free_.lower_bound(off);
         if (next != free_.end() && off + bytes == next->first) {
@@ -280,6 +285,7 @@ struct JobRec {
     int best = -1, best_cut = -1, best_fl = 0;
     uint64_t best_lo = 0, best_hi = 0;
     uint64_t peak = 1, state_steps = 0;
+    uint64_t held = 0, peak_held = 0;  // arena bytes of the job's segments and changed buffers
 };
 
 }  // namespace
@@ -350,6 +356,8 @@ public:
         uint8_t *seg = arena.alloc(want);
         if (!seg && want > bytes) seg = arena.alloc(want = bytes);
         if (!seg) return nullptr;
+        j->held += align_up(want);
+        j->peak_held = std::max(j->peak_held, j->held);
         j->segs.push_back({seg, want});
         j->last_seg = want;
         j->bump = seg + bytes;
@@ -366,11 +374,15 @@ public:
             j->ch[k] = nullptr;
             j->chcap[k] = 0;
         }
+        j->held = 0;
         j->fw.clear();
         j->bw.clear();
     }
 
+    uint64_t nomem = 0;
+
     void fail(JobRec *j, int status) {
+        if (status == TDP_SPB_NOMEM) nomem++;
         j->status = status;
         j->phase = kDone;
         finished.push_back(j->id);
@@ -444,11 +456,16 @@ public:
             const int par = (int)(global_steps & 1);
             if (fwd) {
                 if (j->chcap[par] < 2 * std::max<uint64_t>(n, 1)) {
-                    if (j->ch[par]) arena.release((uint8_t *)j->ch[par], 16 * j->chcap[par]);
+                    if (j->ch[par]) {
+                        arena.release((uint8_t *)j->ch[par], 16 * j->chcap[par]);
+                        j->held -= align_up(16 * j->chcap[par]);
+                    }
                     uint64_t cap = 2 * std::max<uint64_t>(n, 64);
                     j->ch[par] = (Ent *)arena.alloc(16 * cap);
                     j->chcap[par] = j->ch[par] ? cap : 0;
                     if (!j->ch[par]) { fail(j, TDP_SPB_NOMEM); continue; }
+                    j->held += align_up(16 * cap);
+                    j->peak_held = std::max(j->peak_held, j->held);
                 }
                 ch = j->ch[par];
             }
@@ -711,6 +728,21 @@ int tdp_spb_step(void *h, int32_t *active) {
     return b->step(active);
 }
 
+int tdp_spb_stats(void *h, uint64_t *out) {
+    TDP_REQUIRE(h && out, "tdp_spb_stats: null argument");
+    SpBatch *b = static_cast<SpBatch *>(h);
+    std::lock_guard<std::mutex> g(b->mu);
+    int64_t active = 0;
+    for (JobRec *j : b->jobs) active += j->phase != kDone;
+    out[0] = b->arena.size;
+    out[1] = b->arena.in_use;
+    out[2] = b->arena.peak;
+    out[3] = b->nomem;
+    out[4] = b->global_steps;
+    out[5] = (uint64_t)active;
+    return TDP_OK;
+}
+
 int tdp_spb_poll(void *h, int64_t *ids, int32_t cap, int32_t *n) {
     TDP_REQUIRE(h && n && (ids || cap == 0) && cap >= 0, "tdp_spb_poll: bad arguments");
     SpBatch *b = static_cast<SpBatch *>(h);
@@ -734,6 +766,7 @@ int tdp_spb_result(void *h, int64_t id, int32_t *status, int32_t *value, uint64_
         stats[0] = j->state_steps;
         stats[1] = j->peak;
         stats[2] = (uint64_t)(j->best_cut < 0 ? 0 : j->best_cut);
+        stats[3] = j->peak_held;
     }
     return TDP_OK;
 }

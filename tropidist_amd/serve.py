@@ -188,7 +188,6 @@ class Server:
                     self._finish(take, error=exc)
 
     def _sparse_engine_loop(self, stream):
-        t = _lib.torch()
         eng = self.sparse_engine
         owner, retry = {}, collections.deque()
 
@@ -227,8 +226,7 @@ class Server:
                     complete(r)
                 if not owner:
                     continue
-                with t.cuda.stream(stream):
-                    eng.step()
+                eng.step()                          # on the engine's own stream
                 for rid in eng.poll():
                     r = owner.pop(rid)
                     r.left -= 1
@@ -260,9 +258,9 @@ def sparse_certify(p, K, order=None, max_states=30_000_000):
     """Least weight <= K of a solution over all observables by the sparse MITM DP (auto.sparse_distance's loop,
     cdist/auto.py:257-284, with the default split): {"distance", "witness", "observable", "witness_ok"}.  A table
     beyond max_states raises sparse_dp.BudgetExceeded (the request is answered with that error)."""
-    from .ordering import greedy_straddle_order
+    from .ordering import greedy_order_native
     from .problem import verify_witness
-    order = order if order is not None else greedy_straddle_order(p.h, p.nrows)
+    order = order if order is not None else greedy_order_native(p.h, p.nrows, p.words())
     best = None
     for i in range(p.nobs):
         tr = sparse_dp.Trellis.for_query(p, i, order)

@@ -139,13 +139,20 @@ class Engine:
                 return out
 
     def result(self, rid):
-        """(status, value or None, {"state_steps", "peak_states", "join_cut"})."""
+        """(status, value or None, {"state_steps", "peak_states", "join_cut", "peak_bytes"})."""
         st, v = ctypes.c_int32(0), ctypes.c_int32(0)
-        stats = (ctypes.c_uint64 * 3)()
+        stats = (ctypes.c_uint64 * 4)()
         _lib.check(self.L.tdp_spb_result(self.h, rid, ctypes.byref(st), ctypes.byref(v), stats), "tdp_spb_result")
         return st.value, (v.value if v.value >= 0 else None), {"state_steps": int(stats[0]),
                                                                 "peak_states": int(stats[1]),
-                                                                "join_cut": int(stats[2])}
+                                                                "join_cut": int(stats[2]),
+                                                                "peak_bytes": int(stats[3])}
+
+    def stats(self):
+        """{"arena_bytes", "in_use", "peak_in_use", "nomem", "steps", "running"}."""
+        out = (ctypes.c_uint64 * 6)()
+        _lib.check(self.L.tdp_spb_stats(self.h, out), "tdp_spb_stats")
+        return dict(zip(("arena_bytes", "in_use", "peak_in_use", "nomem", "steps", "running"), map(int, out)))
 
     def witness(self, rid, tr, value):
         """Sorted problem columns of a solution of weight `value` (the query's value); tr: the query's Trellis or
