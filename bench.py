@@ -10,7 +10,7 @@ observables' plans; host planning is timed separately).
 Beside it (each can be skipped):
   batched   512 banded scans (200-1200 detectors, 5-15 faults per detector, width 10-22, half with costs 1-3):
             certificates (distance + checked witness) per second from one persistent launch per pass, against the
-            same plans certified one at a time with per-op kernels on a sample ("plain GPU");
+            same plans certified one at a time by the single-plan fused executors on a sample ("plain GPU");
   sparse    the two-block quasi-cyclic code l = m = 12 (288 columns), meet-in-the-middle sweep at threshold K;
   tree      the tree-realization DP of a banded model (R=75 N=300 W=10) on a balanced tree: join pairs per second,
             against the reference's tree_dp.c joins on the host;
@@ -244,7 +244,7 @@ def bench_batched(args, scan):
     torch.cuda.synchronize()
     batched_s = time.perf_counter() - t1
     ok = all(r["witness_ok"] for r in recs if r["distance"] is not None)
-    # plain GPU: the first sample problems one at a time with per-op kernels
+    # plain GPU: the first sample problems one at a time by the single-plan executors (fused passes, fused witness)
     ns = min(args.plain_sample, len(scan))
     sub = [i for i, (pi, _) in enumerate(owner) if pi < ns]
     t2 = time.perf_counter()
