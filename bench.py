@@ -343,9 +343,10 @@ def bench_sparse(args):
         t0 = time.perf_counter()
         rr = sparse_dp.query(t2, 12, mode="mitm", use_h=False, backend=oracle.RefTables(t2))
         dr = time.perf_counter() - t0
-        cmp_["cpu_baseline"] = {"value": st2 / dr, "unit": "state-steps/s", "cores": oracle.rlib("viterbi").dp_num_threads(),
+        cmp_["cpu_baseline"] = {"value": st2 / dr, "unit": "state-steps/s", "cores": 1,
                                 "kind": "reference", "sample": "the same query through the reference's sparse_dp.c "
-                                "(sp_candidates / sp_merge / sp_join) under the same sweep logic", "seconds": dr,
+                                "(sp_candidates / sp_merge / sp_join, single-threaded C) under the same sweep logic",
+                                "seconds": dr,
                                 "matches_gpu": rr["value"] == rh["value"]}
     out["reference_sized"] = cmp_
     return out
