@@ -100,8 +100,10 @@ int tdp_witness_step_bytes(void);
 /* Many programs at once: each record is a value run (segments + passes on d_T0/d_T1 of 2^peak_bits bytes, as
  * tdp_run_program) or, when steps is not NULL, a witness run (as tdp_run_program_witness, on d_arena with room
  * for used_ops in the host array).  One launch per round runs the next pass of every active program, so programs
- * whose passes are too small to fill the device alone share it; per-op segments and steps are launched between the
- * rounds.  d_work: tdp_run_programs_workspace(runs) bytes; host_staging (optional, NULL): pinned host memory of at
+ * whose passes are too small to fill the device alone share it; per-op work on tables of at most 64 entries is a
+ * one-thread job of a round, larger per-op work is launched between the rounds.  Every run is validated on the host
+ * before any device work (TDP_EINVAL).  d_work: at least tdp_run_programs_workspace(runs) bytes (a bound computed
+ * from the runs' counts alone); host_staging (optional, NULL): pinned host memory of at
  * least as many bytes, through which the schedule is uploaded.  Outputs: value (the final entry, 255 = cost >= 255),
  * nused / used_ops for witness runs.  Tables of passes must hold at most 2^32 entries.  No reference counterpart
  * (the reference runs one plan at a time, cdist/auto.py:105-123). */

@@ -37,8 +37,9 @@ def test_version_string_without_a_device():
 
 
 def test_multi_program_records_and_validation_without_a_device():
-    """tdp_run_programs_workspace plans the rounds on the host only: record layout, and a descriptor addressing
-    outside its table is refused before any device work."""
+    """Record layout; tdp_run_programs_workspace sizes the workspace from counts only; tdp_run_programs validates
+    the whole schedule on the host first, so a descriptor addressing outside its table is refused before any device
+    work."""
     import ctypes
     import numpy as np
     from tropidist_amd import _lib, fused, problem, trellis
@@ -58,4 +59,5 @@ def test_multi_program_records_and_validation_without_a_device():
     assert L.tdp_run_programs_workspace(runs, 1, ctypes.byref(nb)) == 0 and nb.value > 0
     arr[0]["runs"][0, 0] = 0 | 1 << 8 | 38 << 16
     arr[0]["nruns"][0] = max(1, int(arr[0]["nruns"][0]))
-    assert L.tdp_run_programs_workspace(runs, 1, ctypes.byref(nb)) == 1
+    # fake device pointers: the call must fail in host validation, before it touches them
+    assert L.tdp_run_programs(runs, 1, 1 << 22, nb.value, None, 0, None) == 1

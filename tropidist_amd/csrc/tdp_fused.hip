@@ -432,6 +432,67 @@ struct Job {
     uint32_t desc, first_block;
 };
 
+// Per-op work on a table of at most kFoldEntries entries (the first and last ops of a plan, where the table is
+// too small for a pass), run as a job of a round by one thread instead of a launch of its own.  A job whose
+// desc is kOpJob has one block, and its `in` points to an OpJob.  Value runs: ops on `cur` (selects write `other`
+// and swap).  Witness steps (witness = 1): one select from cur (the step's input slot) to other (its output
+// slot), or a copy of cur to other followed by an expand (fill), expand + update, or update on other.
+constexpr uint32_t kOpJob = 0xFFFFFFFFu;
+constexpr uint64_t kFoldEntries = 64;
+struct OpJob {
+    uint8_t *cur, *other;
+    uint64_t n;
+    const uint64_t *ops;
+    int64_t nops;
+    int32_t witness, pad;
+};
+
+// The ops of cdist/viterbi.c on a tiny table, one thread: expand = the new half at 255 (an update that follows
+// then fills it, which is what the fused expand-update computes), update = the pairwise saturating min-plus,
+// select = the compaction of the kept half.
+__device__ void tiny_ops(const OpJob &J) {
+    uint8_t *cur = J.cur, *oth = J.other;
+    uint64_t n = J.n;
+    int64_t i = 0;
+    if (J.witness) {
+        if (J.ops[0] == 2) {
+            const int pos = (int)J.ops[1];
+            const uint64_t tgt = J.ops[2], low = (1ull << pos) - 1;
+            for (uint64_t k = 0; k < n / 2; k++) oth[k] = cur[((k & ~low) << 1) | (tgt << pos) | (k & low)];
+            return;
+        }
+        for (uint64_t k = 0; k < n; k++) oth[k] = cur[k];
+        cur = oth;
+    }
+    for (; i < J.nops; i++) {
+        const uint64_t code = J.ops[3 * i], a = J.ops[3 * i + 1], b = J.ops[3 * i + 2];
+        if (code == 0) {
+            for (uint64_t k = 0; k < n; k++) cur[n + k] = 255;
+            n <<= 1;
+        } else if (code == 1) {
+            if (a == 0 || n < 2) continue;                  // no pair to update, as the per-op kernels
+            const uint32_t w = b > 255 ? 255u : (uint32_t)b;
+            const uint64_t hb = 1ull << (63 - __builtin_clzll(a));
+            for (uint64_t s2 = 0; s2 < n; s2++) {
+                if (s2 & hb) continue;
+                const uint64_t t = s2 ^ a;
+                const uint32_t x = cur[s2], y = cur[t];
+                const uint32_t xw = x + w > 255u ? 255u : x + w, yw = y + w > 255u ? 255u : y + w;
+                cur[s2] = (uint8_t)(yw < x ? yw : x);
+                cur[t] = (uint8_t)(xw < y ? xw : y);
+            }
+        } else {
+            const int pos = (int)a;
+            const uint64_t low = (1ull << pos) - 1;
+            for (uint64_t k = 0; k < n / 2; k++) oth[k] = cur[((k & ~low) << 1) | (b << pos) | (k & low)];
+            uint8_t *t = cur;
+            cur = oth;
+            oth = t;
+            n >>= 1;
+        }
+    }
+}
+
 __global__ __launch_bounds__(kPassBlock) void k_pass_multi(const PassArgs *__restrict__ descs,
                                                            const Job *__restrict__ jobs, uint32_t njobs) {
     __shared__ uint32_t s_count;
@@ -444,6 +505,10 @@ __global__ __launch_bounds__(kPassBlock) void k_pass_multi(const PassArgs *__res
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_count, c);
     __syncthreads();
     const Job &J = jobs[s_count - 1];
+    if (J.desc == kOpJob) {
+        if (threadIdx.x == 0) tiny_ops(*reinterpret_cast<const OpJob *>(J.in));
+        return;
+    }
     const uint64_t local = (uint64_t)(blockIdx.x - J.first_block);
     if (local * kPassBlock + threadIdx.x >= J.nthreads) return;
     const PassArgs &a = descs[J.desc];
@@ -720,14 +785,28 @@ struct Action {
 };
 
 struct Schedule {
-    std::vector<PassArgs> descs;
-    std::vector<Job> jobs;
+    std::vector<uint32_t> desc_base;      // per run: index of its first pass descriptor in the upload
+    uint64_t ndescs = 0;                  // descriptors are copied from the callers' arrays at upload
+    std::vector<Job> jobs;                // op jobs hold their OpJob index in `in` until the upload
+    std::vector<OpJob> opjobs;            // `ops` holds the index of the first op word until the upload
+    std::vector<uint64_t> opwords;        // the op triples of the op jobs
     std::vector<Action> actions;
     std::vector<const uint8_t *> finals;  // value runs: the final one-entry table
     std::vector<int> value_runs, witness_runs;
     std::vector<WitnessLayout> layouts;   // per run (witness runs only)
-    uint64_t off_jobs, off_finals, off_results, off_trace, total;
+    uint64_t off_jobs, off_opjobs, off_opwords, off_finals, off_results, off_trace, total;
 };
+
+// Largest table size along ops [first, last) from n entries (the ops were checked by dry_ops_range).
+static uint64_t max_entries(const uint64_t *ops, int64_t first, int64_t last, uint64_t n) {
+    uint64_t m = n;
+    for (int64_t i = first; i < last; i++) {
+        if (ops[3 * i] == 0) n <<= 1;
+        else if (ops[3 * i] == 2) n >>= 1;
+        m = n > m ? n : m;
+    }
+    return m;
+}
 
 static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
     TDP_REQUIRE(nruns >= 1 && runs, "no programs");
@@ -736,19 +815,21 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
         int64_t a, b;      // op range / step index / (pass index, step index)
     };
     std::vector<std::vector<Ev>> evs(nruns);
-    std::vector<uint32_t> desc_base(nruns);
+    std::vector<uint32_t> &desc_base = S.desc_base;
+    desc_base.assign(nruns, 0);
     S.layouts.resize(nruns);
     for (int r = 0; r < nruns; r++) {
         const tdp_run_t &R = runs[r];
         const PassArgs *P = static_cast<const PassArgs *>(R.passes);
         TDP_REQUIRE(R.ops && R.nops >= 0 && (R.npasses == 0 || P), "run %d: bad op stream or passes", r);
-        desc_base[r] = (uint32_t)S.descs.size();
+        TDP_REQUIRE(S.ndescs + (uint64_t)R.npasses < (1ull << 31), "more than 2^31 pass descriptors");
+        desc_base[r] = (uint32_t)S.ndescs;
+        S.ndescs += (uint64_t)R.npasses;
         for (int64_t k = 0; k < R.npasses; k++) {
             int rc = check_pass(P[k]);
             if (rc) return rc;
             TDP_REQUIRE(P[k].b_in <= 32 && P[k].b_out <= 32, "run %d: multi-program passes are limited to 2^32 "
                         "entries", r);
-            S.descs.push_back(P[k]);
         }
         if (R.steps) {
             const WStep *W = static_cast<const WStep *>(R.steps);
@@ -797,7 +878,50 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
         uint64_t nblocks = 0;
         for (int r = 0; r < nruns; r++) {
             const tdp_run_t &R = runs[r];
-            while (cur[r] < evs[r].size() && evs[r][cur[r]].kind != 2) {
+            // per-op work on a tiny table is this round's job of the program (its next pass waits a round)
+            auto foldable = [&](const Ev &e) {
+                if (e.kind == 0) return max_entries(R.ops, e.a, e.b, nn[r]) <= kFoldEntries;
+                if (e.kind == 1) return static_cast<const WStep *>(R.steps)[e.a].n_in <= kFoldEntries / 2;
+                return false;
+            };
+            if (cur[r] < evs[r].size() && foldable(evs[r][cur[r]])) {
+                const Ev &e = evs[r][cur[r]++];
+                OpJob oj{};
+                int64_t op0, nop;
+                if (e.kind == 0) {
+                    oj.cur = tc[r];
+                    oj.other = to[r];
+                    oj.n = nn[r];
+                    op0 = e.a;
+                    nop = e.b - e.a;
+                    bool sw = false;
+                    int rc = dry_ops_range(R.ops, e.a, e.b, nn[r], R.peak_bits, sw);
+                    if (rc) return rc;
+                    if (sw) std::swap(tc[r], to[r]);
+                } else {
+                    const WStep &w = static_cast<const WStep *>(R.steps)[e.a];
+                    oj.cur = R.d_arena + w.in_off;
+                    oj.other = R.d_arena + w.out_off;
+                    oj.n = w.n_in;
+                    oj.witness = 1;
+                    op0 = w.op0;
+                    nop = w.nop;
+                }
+                oj.ops = reinterpret_cast<const uint64_t *>((uintptr_t)S.opwords.size());
+                oj.nops = nop;
+                S.opwords.insert(S.opwords.end(), R.ops + 3 * op0, R.ops + 3 * (op0 + nop));
+                Job J;
+                J.in = reinterpret_cast<const uint8_t *>((uintptr_t)S.opjobs.size());
+                J.out = nullptr;
+                J.nthreads = 1;
+                J.desc = kOpJob;
+                J.first_block = (uint32_t)nblocks;
+                nblocks += 1;
+                S.opjobs.push_back(oj);
+                S.jobs.push_back(J);
+                continue;
+            }
+            while (cur[r] < evs[r].size() && evs[r][cur[r]].kind != 2 && !foldable(evs[r][cur[r]])) {
                 const Ev &e = evs[r][cur[r]++];
                 Action act{};
                 act.run = r;
@@ -818,12 +942,13 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
                 }
                 S.actions.push_back(act);
             }
+            if (cur[r] < evs[r].size() && evs[r][cur[r]].kind != 2) continue;   // folded in a later round
             if (cur[r] >= evs[r].size()) {
                 if (!R.steps && !S.finals[r]) S.finals[r] = tc[r];
                 continue;
             }
             const Ev &e = evs[r][cur[r]++];
-            const PassArgs &p = S.descs[desc_base[r] + (uint32_t)e.a];
+            const PassArgs &p = static_cast<const PassArgs *>(R.passes)[e.a];
             Job J;
             if (R.steps) {
                 const WStep &w = static_cast<const WStep *>(R.steps)[e.b];
@@ -852,8 +977,10 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
         S.actions.push_back(act);
     }
     auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
-    S.off_jobs = up(S.descs.size() * sizeof(PassArgs));
-    S.off_finals = S.off_jobs + up(S.jobs.size() * sizeof(Job));
+    S.off_jobs = up(S.ndescs * sizeof(PassArgs));
+    S.off_opjobs = S.off_jobs + up(S.jobs.size() * sizeof(Job));
+    S.off_opwords = S.off_opjobs + up(S.opjobs.size() * sizeof(OpJob));
+    S.off_finals = S.off_opwords + up(S.opwords.size() * 8);
     S.off_results = S.off_finals + up(S.value_runs.size() * 8);
     S.off_trace = S.off_results + up(S.value_runs.size());
     S.total = S.off_trace + up(S.witness_runs.size() * sizeof(TraceJob)) + 256;
@@ -866,11 +993,26 @@ extern "C" {
 
 int tdp_run_record_bytes(void) { return (int)sizeof(tdp_run_t); }
 
+// A bound on the workspace from counts alone (no schedule): every pass a job, every segment or step an op job with
+// all of the run's ops, and the layout's alignment slack.  tdp_run_programs checks the exact size.
 int tdp_run_programs_workspace(const tdp_run_t *runs, int nruns, uint64_t *bytes) {
-    Schedule S;
-    int rc = build_schedule(runs, nruns, S);
-    if (rc) return rc;
-    *bytes = S.total;
+    TDP_REQUIRE(runs && nruns >= 1 && bytes, "no programs");
+    uint64_t passes = 0, events = 0, ops = 0, values = 0, witnesses = 0;
+    for (int r = 0; r < nruns; r++) {
+        const tdp_run_t &R = runs[r];
+        TDP_REQUIRE(R.npasses >= 0 && R.nops >= 0 && R.nseg >= 0 && R.nsteps >= 0, "run %d: negative counts", r);
+        passes += (uint64_t)R.npasses;
+        ops += (uint64_t)R.nops;
+        if (R.steps) {
+            events += (uint64_t)R.nsteps;
+            witnesses++;
+        } else {
+            events += (uint64_t)R.nseg;
+            values++;
+        }
+    }
+    *bytes = passes * sizeof(PassArgs) + (passes + events) * sizeof(Job) + events * sizeof(OpJob) + 24 * ops +
+             9 * values + witnesses * sizeof(TraceJob) + 8 * 256;
     return TDP_OK;
 }
 
@@ -894,8 +1036,18 @@ int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_
         uint8_t *p;
         uint8_t *data() { return p; }
     } host{hbuf};
-    memcpy(host.data(), S.descs.data(), S.descs.size() * sizeof(PassArgs));
+    for (int r = 0; r < nruns; r++)       // the validated descriptors, straight from the callers' arrays
+        if (runs[r].npasses)
+            memcpy(host.data() + (uint64_t)S.desc_base[r] * sizeof(PassArgs), runs[r].passes,
+                   (size_t)runs[r].npasses * sizeof(PassArgs));
+    for (Job &J : S.jobs)                 // op jobs: their record's device address
+        if (J.desc == kOpJob)
+            J.in = d_work + S.off_opjobs + (uintptr_t)J.in * sizeof(OpJob);
+    for (OpJob &oj : S.opjobs)
+        oj.ops = reinterpret_cast<const uint64_t *>(d_work + S.off_opwords + 8 * (uintptr_t)oj.ops);
     memcpy(host.data() + S.off_jobs, S.jobs.data(), S.jobs.size() * sizeof(Job));
+    memcpy(host.data() + S.off_opjobs, S.opjobs.data(), S.opjobs.size() * sizeof(OpJob));
+    memcpy(host.data() + S.off_opwords, S.opwords.data(), S.opwords.size() * 8);
     for (size_t k = 0; k < S.value_runs.size(); k++) {
         const uint8_t *f = S.finals[S.value_runs[k]];
         memcpy(host.data() + S.off_finals + 8 * k, &f, 8);
