@@ -97,19 +97,24 @@ def dense_setup(L, torch, W=25, nobs=None):
     plans = [pl for pl in trellis.build_plans_native(p.h, p.l, obs, order, p.nrows, p.w) if pl.feasible]
     plan_seconds = time.time() - t0
     t0 = time.time()
-    progs = [fused.program_for(pl) for pl in plans]
+    progs = [fused.program_for(pl, min_bits=2) for pl in plans]      # the programs fused.run_many runs
     compile_seconds = time.time() - t0
     peak = max(pl.peak_bits for pl in plans)
     T0, T1 = _lib.device_empty(1 << peak), _lib.device_empty(1 << peak)
     return p, plans, progs, peak, T0, T1, plan_seconds, compile_seconds
 
 
-def dominant_pass(progs):
-    """(kernel name, algorithmic bytes, descriptor record) of the pass with the most bytes moved in the first
-    program (the widest table)."""
+def dominant_passes(progs):
+    """(kernel name, algorithmic bytes, descriptor records) of the dominant kernel: every pass of the first
+    program (observable 0) that moves the most bytes (the widest table) and runs the most common kernel among
+    those."""
+    import collections
     prog = progs[0]
-    k = max(range(prog.npasses), key=prog.pass_bytes)
-    return prog.pass_kernel(k), prog.pass_bytes(k), prog.arr[k:k + 1].copy()
+    top = max(prog.pass_bytes(i) for i in range(prog.npasses))
+    widest = [i for i in range(prog.npasses) if prog.pass_bytes(i) == top]
+    kname = collections.Counter(prog.pass_kernel(i) for i in widest).most_common(1)[0][0]
+    idx = [i for i in widest if prog.pass_kernel(i) == kname]
+    return kname, top, prog.arr[idx].copy()
 
 
 def measured_traffic(kname, alg_bytes):
@@ -136,17 +141,12 @@ def bench_dense(args, L, torch):
     stream = torch.cuda.current_stream()
     sh = _lib.stream_handle(stream)
 
+    from tropidist_amd import fused
+
     def step():
-        vals = []
-        for pl, ops, prog in zip(plans, encoded, progs):
-            segs, arr = prog.segs, prog.arr
-            v = ctypes.c_int(-1)
-            _lib.check(L.tdp_run_program(ops.ctypes.data_as(ctypes.c_void_p), len(ops),
-                                         segs.ctypes.data_as(ctypes.c_void_p), len(segs),
-                                         arr.ctypes.data_as(ctypes.c_void_p), len(arr), pl.peak_bits,
-                                         _lib.dptr(T0), _lib.dptr(T1), ctypes.byref(v), sh), "tdp_run_program")
-            vals.append(kernel.final_value(v.value))
-        return vals
+        # the four observables' fused programs advance together, one pass of each per round (tdp_run_programs);
+        # the host schedule of the four is built inside the timed step
+        return fused.run_many(plans, stream=stream)
 
     for _ in range(args.warmup):
         values = step()
@@ -167,14 +167,16 @@ def bench_dense(args, L, torch):
         per_op_values.append(kernel.final_value(v.value))
     per_op_seconds = time.perf_counter() - t2
 
-    # roofline of the dominant kernel: the widest fused pass, replayed on its own input (T0 -> T1); algorithmic
-    # bytes = every input entry read once + every output entry written once
-    kname, pass_bytes, desc = dominant_pass(progs)
-    d = desc[0]
+    # roofline of the dominant kernel: the widest fused passes of observable 0 (all of them that run the dominant
+    # kernel), each replayed on its own input (T0 -> T1), the average launch; algorithmic bytes = every input entry
+    # read once + every output entry written once
+    kname, pass_bytes, descs = dominant_passes(progs)
+    d = descs[0]
     _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << int(d["b_in"]), 7, sh), "fill")
-    pass_s = _time_launches(torch, stream, lambda: _lib.check(
-        L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "tdp_run_pass"),
-        args.roofline_launches)
+    per = max(1, args.roofline_launches // len(descs))
+    pass_s = sum(_time_launches(torch, stream, lambda k=k: _lib.check(
+        L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), descs[k:k + 1].ctypes.data_as(ctypes.c_void_p), sh),
+        "tdp_run_pass"), per) for k in range(len(descs))) / len(descs)
     # the per-op streaming update at the peak table, for reference
     n = 1 << peak
     masks = [int(m) for c, m, _ in encoded[0].tolist() if c == 1 and (n >> 1) <= m < n][:1] or [n >> 1]
@@ -190,18 +192,20 @@ def bench_dense(args, L, torch):
                    "distances": values, "distance": min(values), "host_planning_seconds": round(plan_seconds, 3),
                    "pass_compile_seconds": round(compile_seconds, 3),
                    "passes_per_step": sum(pr.npasses for pr in progs),
-                   "executor": "fused passes (tdp_run_program), per-op kernels below 2^12 entries",
+                   "executor": "fused passes, the four programs advanced together one pass each per round "
+                               "(tdp_run_programs)",
                    "per_op_executor": {"states_per_s": total_work / per_op_seconds,
                                        "ms_per_step": 1e3 * per_op_seconds, "distances": per_op_values,
                                        "identical": per_op_values == values}},
         "roofline": {"bound": "hbm", "achieved": pass_bytes / pass_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": pass_bytes / pass_s / HBM_PEAK, "traffic": measured_traffic(kname, pass_bytes),
-                     "traffic_source": "profiles/pass_d5_banded_w25.json (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled "
-                                       "by a device copy of 2^26 bytes)",
+                     "traffic_source": "profiles/pass_d5_banded_w25.json (rocprofv3 FETCH_SIZE / WRITE_SIZE over the "
+                                       "same 59 passes, scaled by a device copy of 2^26 bytes)",
                      "kernel": kname,
                      "launch_seconds": pass_s, "alg_bytes_per_launch": pass_bytes,
-                     "pass": {"b_in": int(d["b_in"]), "b_out": int(d["b_out"]),
-                              "updates": int(d["D"]) + int(d["nlow"]), "threads": 1 << int(d["nfree"])},
+                     "passes": {"count": len(descs), "b_in": int(d["b_in"]), "b_out": int(d["b_out"]),
+                                "updates": [int(x["D"]) + int(x["nlow"]) for x in descs[:1]][0],
+                                "threads": 1 << int(d["nfree"]), "launches_each": per},
                      "per_op_update": {"kernel": "tdp::k_update_hi", "table_entries": n, "launch_seconds": upd_s,
                                        "achieved": 2 * n / upd_s / 1e9, "frac": 2 * n / upd_s / HBM_PEAK},
                      "dram": dram},
@@ -214,17 +218,18 @@ def dram_pass(args, L, torch, stream, sh, W=27):
     import ctypes
     from tropidist_amd import _lib
     _, _, progs, _, T0, T1, _, _ = dense_setup(L, torch, W=W, nobs=1)
-    kname, pass_bytes, desc = dominant_pass(progs)
-    d = desc[0]
+    kname, pass_bytes, descs = dominant_passes(progs)
+    d = descs[0]
     _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << int(d["b_in"]), 7, sh), "fill")
-    launches = max(4, args.roofline_launches // 4)
-    pass_s = _time_launches(torch, stream, lambda: _lib.check(
-        L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "tdp_run_pass"),
-        launches)
+    per = max(1, args.roofline_launches // (4 * len(descs)))
+    pass_s = sum(_time_launches(torch, stream, lambda k=k: _lib.check(
+        L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), descs[k:k + 1].ctypes.data_as(ctypes.c_void_p), sh),
+        "tdp_run_pass"), per) for k in range(len(descs))) / len(descs)
+    launches = per * len(descs)
     del T0, T1
     torch.cuda.empty_cache()
-    return {"workload": "widest pass of the same banded instance at W=%d, observable 0: 2^%d -> 2^%d entries, "
-                        "D=%d" % (W, int(d["b_in"]), int(d["b_out"]), int(d["D"])),
+    return {"workload": "the %d widest passes of the same banded instance at W=%d, observable 0: 2^%d -> 2^%d "
+                        "entries, D=%d, averaged" % (len(descs), W, int(d["b_in"]), int(d["b_out"]), int(d["D"])),
             "kernel": kname, "alg_bytes_per_launch": pass_bytes, "launch_seconds": pass_s,
             "achieved": pass_bytes / pass_s / 1e9, "frac": pass_bytes / pass_s / HBM_PEAK, "launches": launches}
 
@@ -438,22 +443,24 @@ def profile_pass(args):
     L = _lib.lib()
     p, plans, progs, peak, T0, T1, _, _ = dense_setup(L, torch, W=args.profile_width,
                                                       nobs=None if args.profile_width == 25 else 1)
-    kname, alg_bytes, desc = dominant_pass(progs)
-    d = desc[0]
+    kname, alg_bytes, descs = dominant_passes(progs)
+    d = descs[0]
     b_in = int(d["b_in"])
     stream = torch.cuda.current_stream()
     sh = _lib.stream_handle(stream)
     _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << b_in, 7, sh), "fill")
-    for _ in range(args.profile_pass):
-        _lib.check(L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), desc.ctypes.data_as(ctypes.c_void_p), sh), "pass")
+    for i in range(args.profile_pass):                  # the dominant passes round-robin
+        k = i % len(descs)
+        _lib.check(L.tdp_run_pass(_lib.dptr(T0), _lib.dptr(T1), descs[k:k + 1].ctypes.data_as(ctypes.c_void_p), sh),
+                   "pass")
     src = T0[:1 << b_in]
     dst = T1[:1 << b_in]
     for _ in range(args.profile_pass):
         dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"pass": {"kernel": kname, "b_in": b_in, "b_out": int(d["b_out"]), "D": int(d["D"]),
-                               "threads": 1 << int(d["nfree"]), "alg_bytes": alg_bytes}, "copy_bytes": 1 << b_in,
-                      "launches": args.profile_pass}))
+                               "threads": 1 << int(d["nfree"]), "alg_bytes": alg_bytes, "passes": len(descs)},
+                      "copy_bytes": 1 << b_in, "launches": args.profile_pass}))
 
 
 def main():
@@ -462,7 +469,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--roofline-launches", type=int, default=60)
+    ap.add_argument("--roofline-launches", type=int, default=240,
+                    help="launches timed for the roofline, spread over the dominant passes")
     ap.add_argument("--gen-workers", type=int, default=16, help="input generator processes (0: in process)")
     ap.add_argument("--skip-dense", action="store_true",
                     help="leave out the headline section (value null; for profiling the other sections alone)")

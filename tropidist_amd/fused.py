@@ -851,10 +851,11 @@ def _staging(nbytes):
     return buf
 
 
-def run_many(plans, witness=None, stream=None):
+def run_many(plans, witness=None, stream=None, min_bits=2):
     """Run many feasible plans together by tdp_run_programs (one launch per round runs the next pass of every
     program).  witness[i] true: a witness run, result (value, sorted used op indices); else the value.  Values of
-    255 or more raise OverflowError.  ValueError if a plan's column exceeds the pass limits (run it alone)."""
+    255 or more raise OverflowError.  ValueError if a plan's column exceeds the pass limits (run it alone).
+    min_bits: value runs use passes from 2^min_bits entries up (program_for); smaller tables run op by op."""
     import ctypes
     from . import _lib, kernel
     from .trellis import as_plan
@@ -884,8 +885,8 @@ def run_many(plans, witness=None, stream=None):
             used_of[r] = used
             keep.append(arena)
         else:
-            # passes from 2^2 entries up: narrow heads and tails are rounds too, not one launch per op
-            prog = program_for(plan, min_bits=2)
+            # by default passes from 2^2 entries up: narrow heads and tails are rounds too, not one launch per op
+            prog = program_for(plan, min_bits=min_bits)
             T0 = _lib.device_empty(1 << plan.peak_bits)
             T1 = _lib.device_empty(1 << plan.peak_bits)
             R.segments, R.nseg = prog.segs.ctypes.data, len(prog.segs)
