@@ -4,18 +4,22 @@
 value = dense Rank DP states/s (sum of Plan.work per second) on the single large dense instance: banded random
 detector error model, seed 2000: 600 detectors, 6000 fault columns each flipping 2-6 detectors inside a 25-row
 sliding window, 4 logical rows of density 0.05, integer costs uniform in 1..3, uint8 saturating tables; the natural
-column order has rank width 25 and 2^26-entry tables.  One step = the exact distance of the instance (all four
-observables' plans; host planning is timed separately).
+column order has rank width 25 and 2^26-entry tables.  One step = the exact distance of the instance: the four
+observables' fused programs run together, one pass of each per round (host planning and pass compilation are
+timed separately).  roofline: the widest passes of observable 0 (the dominant kernel), each replayed by HIP events,
+averaged; roofline.dram: the same for the W = 27 instance, whose 2^28-entry tables do not fit the MALL.
 
 Beside it (each can be skipped):
   batched   512 banded scans (200-1200 detectors, 5-15 faults per detector, width 10-22, half with costs 1-3):
-            certificates (distance + checked witness) per second from one persistent launch per pass, against the
-            same plans certified one at a time by the single-plan fused executors on a sample ("plain GPU");
-  sparse    the two-block quasi-cyclic code l = m = 12 (288 columns), meet-in-the-middle sweep at threshold K;
+            certificates (distance + checked witness) per second (the median of five passes over the scan), against
+            the same plans certified one at a time by the single-plan fused executors ("plain GPU", whole scan);
+  sparse    the two-block quasi-cyclic code l = m = 12 (288 columns), meet-in-the-middle sweep at threshold K, and
+            a reference-sized code (l = m = 9) against the reference's sparse_dp.c on the host;
   tree      the tree-realization DP of a banded model (R=75 N=300 W=10) on a balanced tree: join pairs per second,
             against the reference's tree_dp.c joins on the host;
   serving   the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse MITM on quasi-cyclic codes):
-            capacity with every request submitted at once, then a Poisson stream at half of it, p50 / p99 latency;
+            capacity with every request submitted at once, then a Poisson stream at half of it, p50 / p99 latency,
+            against a stratified plain-GPU sample;
   cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--skip-batched] [--skip-sparse]
