@@ -394,8 +394,12 @@ def bench_serving(args, requests):
     reqs = [("dense", (p,)) if kind == "dense" else ("sparse", (p, K)) for kind, p, K in requests]
     cap_tickets, cap_wall = _serve_run(serve, reqs, None, 0)
     capacity = len(reqs) / cap_wall
+    print("bench: serving capacity %.0f/s over %d requests (%.1f s)" % (capacity, len(reqs), time.time() - T_START),
+          file=sys.stderr, flush=True)
     rate = args.serving_rate if args.serving_rate else args.serving_load * capacity
     tickets, wall = _serve_run(serve, reqs, rate, 7)
+    print("bench: serving Poisson stream at %.0f/s done (%.1f s)" % (rate, time.time() - T_START), file=sys.stderr,
+          flush=True)
     lat = [tk.latency for tk in tickets]
     errs = [tk for tk in tickets if tk.error is not None]
     ok = all(tk.result["witness_ok"] for tk in tickets if tk.error is None and tk.result["distance"] is not None)

@@ -15,8 +15,9 @@ Sparse requests go to one engine thread with its own stream and a resident spars
 (sparse_arena_bytes, held for the server's life) is set apart from the dense budget.  Between any two engine steps
 the thread admits every waiting sparse request (every observable a query of the engine) and completes those whose
 queries have finished, so sparse requests that arrive while others run join them on the device, one column of all of
-them per step.  A request the busy arena cannot hold is retried alone, and one that does not fit alone is answered by
-the one-at-a-time path (sparse_certify).
+them per step.  Requests are admitted while the arena is less than `sparse_admit` full (one at least when nothing
+runs); a request that finds the arena full while others run waits for memory to free, and one that does not fit
+alone is answered by the one-at-a-time path (sparse_certify).
 
 The result of a request does not depend on what it was batched with: every dense plan runs alone in its workgroup on
 its own tables, every sparse query on its own tables and counters (tests/test_gpu_serve.py checks each result alone
@@ -71,11 +72,12 @@ def _dense_footprint(plans):
 
 class Server:
     def __init__(self, memory_bytes=64 << 30, dense_lanes=2, max_batch=1024, sparse_arena_bytes=4 << 30,
-                 sparse_max_states=30_000_000):
+                 sparse_max_states=30_000_000, sparse_admit=0.5):
         if sparse_arena_bytes >= memory_bytes:
             raise ValueError("the sparse arena (%d bytes) must leave room for dense requests in %d bytes"
                              % (sparse_arena_bytes, memory_bytes))
         self.budget = memory_bytes - sparse_arena_bytes
+        self.sparse_admit = sparse_admit        # admit sparse requests while the arena is less full than this
         self.in_use = 0
         self.max_batch = max_batch
         self.dense_q = collections.deque()
@@ -188,8 +190,13 @@ class Server:
                     self._finish(take, error=exc)
 
     def _sparse_engine_loop(self, stream):
+        """Admits waiting sparse requests while the engine's arena is less than SPARSE_ADMIT full (always one when
+        nothing runs), steps the engine and completes finished requests.  A request that finds the arena full while
+        others run waits at the head of the queue for memory to free; one that finds it full alone is answered by
+        the checkpointed one-at-a-time sweeps."""
         eng = self.sparse_engine
-        owner, retry = {}, collections.deque()
+        owner, waiting = {}, collections.deque()
+        running = set()
 
         def complete(r):
             if r.error is not None:
@@ -200,30 +207,31 @@ class Server:
         try:
             while True:
                 with self.cv:
-                    while not self.sparse_q and not owner and not retry and not self.closed:
+                    while not self.sparse_q and not owner and not waiting and not self.closed:
                         self.cv.wait()
-                    if self.closed and not self.sparse_q and not owner and not retry:
+                    if self.closed and not self.sparse_q and not owner and not waiting:
                         return
                     new = list(self.sparse_q)
                     self.sparse_q.clear()
                 now = time.perf_counter()
-                reqs = []
-                for tk in new:
+                for tk in new:                      # host planning (C++ trellis words) as the requests arrive
                     tk.t_start = now
-                    reqs.append(sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk))
-                if not owner and retry:             # alone now: the request the busy arena could not hold
-                    reqs.append(retry.popleft())
-                for r in reqs:
+                    r = sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk)
                     try:
-                        if not r.trs:
-                            r.plan()
+                        r.plan()
                         if r.trs:
-                            r.admit(eng, owner)
+                            waiting.append(r)
                             continue
                         r.finish(eng)
                     except Exception as exc:
                         r.error = exc
                     complete(r)
+                st = eng.stats()
+                while waiting and (not running or st["in_use"] < self.sparse_admit * st["arena_bytes"]):
+                    r = waiting.popleft()
+                    r.admit(eng, owner)
+                    running.add(r)
+                    st = eng.stats()
                 if not owner:
                     continue
                 eng.step()                          # on the engine's own stream
@@ -232,12 +240,13 @@ class Server:
                     r.left -= 1
                     if r.left:
                         continue
-                    shared = bool(owner)
+                    running.discard(r)
+                    alone = not running
                     r.finish(eng)
                     if isinstance(r.error, MemoryError):
                         r.error = None
-                        if shared:
-                            retry.append(r)
+                        if not alone:
+                            waiting.appendleft(r)   # again when memory has freed
                             continue
                         try:                        # too large for the arena alone: checkpointed sweeps
                             r.record = sparse_certify(r.problem, r.K, r.order)
@@ -245,7 +254,7 @@ class Server:
                             r.error = exc
                     complete(r)
         finally:
-            for r in list(owner.values()) + list(retry):
+            for r in list(owner.values()) + list(waiting):
                 if r.tag.t_done is None:
                     self._finish([r.tag], error=RuntimeError("server closed"))
 
