@@ -54,3 +54,20 @@ def test_poisson_stream_completes_with_latencies():
     lat = [tk.latency for tk in tickets]
     assert all(x is not None and x > 0 for x in lat) and all(tk.error is None for tk in tickets)
     assert serve.percentile(lat, 50) <= serve.percentile(lat, 99)
+
+
+def test_sparse_requests_beyond_the_engine_slots():
+    """Twelve sparse requests whose queries far exceed an engine of 6 job slots: admission waits for slots, and
+    every result equals the one-at-a-time answer."""
+    print("case serve sparse slots", flush=True)
+    probs = [(problem.quasi_cyclic(l, m, seed=9500 + k), K) for k, (l, m, K) in
+             enumerate([(6, 6, 6), (6, 8, 6), (6, 6, 7), (9, 6, 5), (6, 9, 6), (8, 6, 6)] * 2)]
+    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=1, sparse_arena_bytes=1 << 30, sparse_max_jobs=6)
+    tickets = [srv.submit_sparse(p, K) for p, K in probs]
+    recs = [tk.wait(timeout=300) for tk in tickets]
+    srv.close()
+    for (p, K), rec in zip(probs, recs):
+        want = serve.sparse_certify(p, K)
+        assert rec["distance"] == want["distance"] and rec["observable"] == want["observable"]
+        if rec["distance"] is not None:
+            assert rec["witness_ok"]

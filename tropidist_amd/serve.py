@@ -72,7 +72,7 @@ def _dense_footprint(plans):
 
 class Server:
     def __init__(self, memory_bytes=64 << 30, dense_lanes=2, max_batch=1024, sparse_arena_bytes=4 << 30,
-                 sparse_max_states=30_000_000, sparse_admit=0.5):
+                 sparse_max_states=30_000_000, sparse_admit=0.5, sparse_max_jobs=4096):
         if sparse_arena_bytes >= memory_bytes:
             raise ValueError("the sparse arena (%d bytes) must leave room for dense requests in %d bytes"
                              % (sparse_arena_bytes, memory_bytes))
@@ -94,7 +94,7 @@ class Server:
             self.threads.append(th)
         sst = t.cuda.Stream()
         self.sparse_engine = sparse_batch.Engine(arena_bytes=sparse_arena_bytes, stream=sst,
-                                                 max_states=sparse_max_states)
+                                                 max_states=sparse_max_states, max_jobs=sparse_max_jobs)
         th = threading.Thread(target=self._sparse_engine_loop, args=(sst,), daemon=True)
         th.start()
         self.threads.append(th)
@@ -219,24 +219,46 @@ class Server:
                     r = sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk)
                     try:
                         r.plan()
-                        if r.trs:
+                        if len(r.trs) > eng.max_jobs:   # more queries than the engine has slots: one at a time
+                            r.record = sparse_certify(r.problem, r.K, r.order)
+                        elif r.trs:
                             waiting.append(r)
                             continue
-                        r.finish(eng)
+                        else:
+                            r.finish(eng)
                     except Exception as exc:
                         r.error = exc
                     complete(r)
                 st = eng.stats()
-                while waiting and (not running or st["in_use"] < self.sparse_admit * st["arena_bytes"]):
+                while waiting and (not running or (st["in_use"] < self.sparse_admit * st["arena_bytes"] and
+                                                   len(owner) + len(waiting[0].trs) <= eng.max_jobs)):
                     r = waiting.popleft()
-                    r.admit(eng, owner)
+                    try:
+                        r.admit(eng, owner)
+                    except Exception as exc:        # answered with the error; queries already added drain first
+                        r.error = exc
+                        if not r.ids:
+                            complete(r)
+                            continue
                     running.add(r)
                     st = eng.stats()
                 if not owner:
                     continue
-                eng.step()                          # on the engine's own stream
+                try:
+                    eng.step()                      # on the engine's own stream
+                except Exception as exc:            # a failed step ends every running request with its error
+                    for r in list(running):
+                        for rid in r.ids:
+                            owner.pop(rid, None)
+                        r.error = exc
+                        complete(r)
+                    running.clear()
+                    continue
                 for rid in eng.poll():
-                    r = owner.pop(rid)
+                    r = owner.pop(rid, None)
+                    if r is None:                   # a query of a request already answered with an error
+                        eng.release(rid)
+                        continue
                     r.left -= 1
                     if r.left:
                         continue

@@ -84,6 +84,7 @@ class Engine:
         self.stream = stream
         self._sh = _lib.stream_handle(stream)
         self.max_states = max_states
+        self.max_jobs = max_jobs
         self.h = self.L.tdp_spb_create(arena_bytes, max_jobs, self._sh)
         if not self.h:
             raise MemoryError(self.L.tdp_last_error().decode(errors="replace"))
@@ -190,17 +191,26 @@ class Request:
         return self
 
     def admit(self, eng, owner=None):
-        """Adds the request's queries to the engine; owner (a dict id -> request) learns them."""
+        """Adds the request's queries to the engine; owner (a dict id -> request) learns them.  If an add fails, the
+        queries added so far stay registered (they drain like the others) and the exception propagates: the caller
+        records it as the request's error, which finish() keeps."""
         N = self.problem.ncols
-        self.ids = [eng.add_words(prm, tm, N, self.K) for _, prm, tm in self.trs]
-        self.left = len(self.ids)
-        if owner is not None:
-            for rid in self.ids:
-                owner[rid] = self
+        self.ids = []
+        try:
+            for _, prm, tm in self.trs:
+                self.ids.append(eng.add_words(prm, tm, N, self.K))
+        finally:
+            self.left = len(self.ids)
+            if owner is not None:
+                for rid in self.ids:
+                    owner[rid] = self
 
     def finish(self, eng):
-        """Builds the record (or the error) once every query has finished, and releases the queries."""
+        """Builds the record (or the error) once every query has finished, and releases the queries.  An error
+        recorded earlier (a failed admission) stands."""
         try:
+            if self.error is not None:
+                return self
             best = None
             for (i, _, _), rid in zip(self.trs, self.ids):
                 st, v, _ = eng.result(rid)
@@ -233,24 +243,43 @@ def certify_requests(requests, engine=None, arena_bytes=4 << 30, stream=None, fa
     A request whose tables do not fit the arena is answered by the one-at-a-time path (serve.sparse_certify) when
     `fallback` is set."""
     from . import serve
+    import collections
     eng = engine or Engine(arena_bytes=arena_bytes, stream=stream)
     try:
-        reqs, owner = [], {}
+        reqs, owner, waiting = [], {}, collections.deque()
         for item in requests:
             r = Request(*item)
             try:
                 r.plan()
-                if r.trs:
-                    r.admit(eng, owner)
+                if len(r.trs) > eng.max_jobs:       # more queries than the engine has slots
+                    if not fallback:
+                        raise ValueError("%d queries, the engine holds %d" % (len(r.trs), eng.max_jobs))
+                    r.record = serve.sparse_certify(r.problem, r.K, r.order)
+                elif r.trs:
+                    waiting.append(r)
                 else:
                     r.finish(eng)
             except Exception as exc:
                 r.error = exc
             reqs.append(r)
-        while owner:
+        while owner or waiting:
+            # admit in arrival order while the engine has job slots for a request's queries
+            while waiting and (not owner or len(owner) + len(waiting[0].trs) <= eng.max_jobs):
+                r = waiting.popleft()
+                try:
+                    r.admit(eng, owner)
+                except Exception as exc:            # queries already added drain; the request keeps the error
+                    r.error = exc
+                    if not r.ids:
+                        continue
+            if not owner:
+                continue
             eng.step()
             for rid in eng.poll():
-                r = owner.pop(rid)
+                r = owner.pop(rid, None)
+                if r is None:
+                    eng.release(rid)
+                    continue
                 r.left -= 1
                 if r.left:
                     continue
