@@ -370,14 +370,17 @@ def _serve_run(serve, reqs, rate, seed):
     # (all sparse requests of the closed-loop run hold about 65 GB at once)
     srv = serve.Server(memory_bytes=160 << 30, dense_lanes=3, sparse_arena_bytes=80 << 30)
     try:
+        def note(msg):
+            print("bench: serving %s (%.1f s)" % (msg, time.time() - T_START), file=sys.stderr, flush=True)
+
         if rate is None:                        # closed loop: every request waiting at t = 0
             t0 = time.perf_counter()
             tickets = [srv.submit_dense(*a) if k == "dense" else srv.submit_sparse(*a) for k, a in reqs]
-            for tk in tickets:
-                tk._event.wait()
+            serve.wait_all(tickets, lambda m: note("closed loop: " + m))
             wall = time.perf_counter() - t0
         else:
-            tickets, wall = serve.run_poisson(srv, reqs, rate=rate, seed=seed)
+            tickets, wall = serve.run_poisson(srv, reqs, rate=rate, seed=seed,
+                                              progress=lambda m: note("Poisson stream: " + m))
     finally:
         srv.close()
     return tickets, wall

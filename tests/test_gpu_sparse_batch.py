@@ -131,6 +131,11 @@ def test_requests_equal_sparse_certify():
         assert r.record["distance"] == want["distance"]
         assert r.record["observable"] == want["observable"]
         assert r.record["witness_ok"]
+        # both deepen through the threshold ladder: the least value equals one sweep of every observable at K
+        vals = [sparse_dp.query(tr, K, mode="mitm", use_h=False, backend="hip")["value"]
+                for tr in (sparse_dp.Trellis.for_query(p, i, r.order) for i in range(p.nobs)) if tr.feasible]
+        vals = [v for v in vals if v is not None]
+        assert r.record["distance"] == (min(vals) if vals else None)
 
 
 def test_state_budget_raises_like_one_query_path():

@@ -1,0 +1,106 @@
+"""Host logic of serving without a device: the threshold ladder of sparse requests, a request's deepening from one
+threshold to the next (against a stand-in engine that answers each query with its least weight when it is within
+the threshold), and the waiting helper's progress reports."""
+import threading
+import time
+
+from tropidist_amd import problem, serve, sparse_batch
+
+
+def test_threshold_ladder():
+    assert sparse_batch.threshold_ladder(4) == [4]
+    assert sparse_batch.threshold_ladder(3) == [3]
+    assert sparse_batch.threshold_ladder(5) == [5]
+    assert sparse_batch.threshold_ladder(7) == [5, 7]
+    assert sparse_batch.threshold_ladder(8) == [4, 6, 8]
+    assert sparse_batch.threshold_ladder(10) == [4, 6, 8, 10]
+    for K in range(1, 20):
+        ks = sparse_batch.threshold_ladder(K)
+        assert ks[-1] == K and ks == sorted(set(ks)) and (len(ks) == 1 or ks[0] >= 4)
+
+
+class _Engine:
+    """Answers a query with value `least[obs]` when it is <= the query's threshold, else with no value."""
+
+    def __init__(self, least, witness):
+        self.least, self.wit = least, witness
+        self.added, self.released, self.q = [], [], {}
+
+    def add_words(self, prm, tm, N, K):
+        rid = len(self.added)
+        self.added.append(K)
+        self.q[rid] = (prm, K)
+        return rid
+
+    def result(self, rid):
+        obs, K = self.q[rid]
+        v = self.least[obs]
+        return sparse_batch.DONE, (v if v is not None and v <= K else None), {}
+
+    def witness(self, rid, order, value):
+        return self.wit
+
+    def release(self, rid):
+        self.released.append(rid)
+
+
+def _run(req, eng):
+    """Admits and finishes the request until it is no longer pending; returns the thresholds swept."""
+    while True:
+        req.admit(eng)
+        req.finish(eng)
+        if not req.pending:
+            return eng.added
+
+
+def test_request_deepens_until_a_threshold_finds_a_solution():
+    p = problem.chain(6)                          # least weight 6: every column
+    witness = list(range(p.ncols))
+    # one query per observable; the stand-in takes the observable as the query's words
+    req = sparse_batch.Request(p, 8, order=list(range(p.ncols)))
+    req.trs = [(0, 0, None)]
+    eng = _Engine({0: 6}, witness)
+    assert _run(req, eng) == [4, 6]               # 4 finds nothing, 6 finds the solution; 8 is never swept
+    assert req.record["distance"] == 6 and req.record["observable"] == 0 and req.record["witness_ok"]
+    assert sorted(eng.released) == [0, 1]
+
+
+def test_request_without_a_solution_sweeps_every_threshold():
+    p = problem.chain(6)
+    req = sparse_batch.Request(p, 10, order=list(range(p.ncols)))
+    req.trs = [(0, 0, None)]
+    eng = _Engine({0: None}, None)
+    assert _run(req, eng) == [4, 6, 8, 10]
+    assert req.record["distance"] is None and req.record["K"] == 10
+
+
+def test_request_picks_the_first_observable_with_the_least_value():
+    p = problem.chain(6)
+    req = sparse_batch.Request(p, 8, order=list(range(p.ncols)))
+    req.trs = [(0, 0, None), (1, 1, None), (2, 2, None)]
+    eng = _Engine({0: 7, 1: 6, 2: 6}, list(range(p.ncols)))
+    assert _run(req, eng) == [4, 4, 4, 6, 6, 6]
+    assert req.record["distance"] == 6 and req.record["observable"] == 1
+
+
+def test_wait_all_reports_only_progress():
+    class T:
+        def __init__(self, kind):
+            self.kind, self._event = kind, threading.Event()
+
+    tickets = [T("dense" if i % 3 else "sparse") for i in range(30)]
+    msgs = []
+
+    def answer():
+        for tk in reversed(tickets):
+            time.sleep(0.02)
+            tk._event.set()
+
+    th = threading.Thread(target=answer)
+    th.start()
+    serve.wait_all(tickets, msgs.append, every=0.1)
+    th.join()
+    assert all(tk._event.is_set() for tk in tickets)
+    assert msgs and all("answered" in m for m in msgs)
+    counts = [int(m.split()[0]) for m in msgs]
+    assert counts == sorted(set(counts))          # strictly increasing: no report without new answers

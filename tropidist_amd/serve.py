@@ -16,8 +16,11 @@ Sparse requests go to one engine thread with its own stream and a resident spars
 the thread admits every waiting sparse request (every observable a query of the engine) and completes those whose
 queries have finished, so sparse requests that arrive while others run join them on the device, one column of all of
 them per step.  Requests are admitted while the arena is less than `sparse_admit` full (one at least when nothing
-runs); a request that finds the arena full while others run waits for memory to free, and one that does not fit
-alone is answered by the one-at-a-time path (sparse_certify).
+runs); a request that finds the arena full while others run waits for memory to free.  A request that does not
+fit alone, or has more queries than the engine has slots, goes to a lane of its own with its own stream, where the
+one-at-a-time path (sparse_certify, checkpointed sweeps) answers it while the engine keeps stepping the others.
+Every sparse request deepens through sparse_batch.threshold_ladder(K) and every table is held to
+`sparse_max_states` states on both paths; a request past that budget is answered with sparse_dp.BudgetExceeded.
 
 The result of a request does not depend on what it was batched with: every dense plan runs alone in its workgroup on
 its own tables, every sparse query on its own tables and counters (tests/test_gpu_serve.py checks each result alone
@@ -93,11 +96,18 @@ class Server:
             th.start()
             self.threads.append(th)
         sst = t.cuda.Stream()
+        self.sparse_max_states = sparse_max_states
         self.sparse_engine = sparse_batch.Engine(arena_bytes=sparse_arena_bytes, stream=sst,
                                                  max_states=sparse_max_states, max_jobs=sparse_max_jobs)
-        th = threading.Thread(target=self._sparse_engine_loop, args=(sst,), daemon=True)
-        th.start()
-        self.threads.append(th)
+        self.engine_thread = threading.Thread(target=self._sparse_engine_loop, args=(sst,), daemon=True)
+        self.engine_thread.start()
+        # requests the engine cannot hold (more queries than its slots, or tables beyond its arena alone) are
+        # answered one query at a time on a lane of their own, so that the engine keeps stepping the others
+        self.single_q = collections.deque()
+        self.single_closed = False
+        self.single_thread = threading.Thread(target=self._sparse_single_lane, args=(t.cuda.Stream(),),
+                                              daemon=True)
+        self.single_thread.start()
 
     # ------------------------------------------------------------------ submission
     def submit_dense(self, problem, order=None):
@@ -120,6 +130,11 @@ class Server:
             self.cv.notify_all()
         for th in self.threads:
             th.join()
+        self.engine_thread.join()               # may still hand requests to the single-query lane
+        with self.cv:
+            self.single_closed = True
+            self.cv.notify_all()
+        self.single_thread.join()
         self.sparse_engine.close()
 
     # ------------------------------------------------------------------ lanes
@@ -220,7 +235,8 @@ class Server:
                     try:
                         r.plan()
                         if len(r.trs) > eng.max_jobs:   # more queries than the engine has slots: one at a time
-                            r.record = sparse_certify(r.problem, r.K, r.order)
+                            self._to_single(r)
+                            continue
                         elif r.trs:
                             waiting.append(r)
                             continue
@@ -265,20 +281,45 @@ class Server:
                     running.discard(r)
                     alone = not running
                     r.finish(eng)
+                    if r.pending:                   # no solution at this threshold: admitted again for the next
+                        waiting.appendleft(r)
+                        continue
                     if isinstance(r.error, MemoryError):
                         r.error = None
                         if not alone:
                             waiting.appendleft(r)   # again when memory has freed
                             continue
-                        try:                        # too large for the arena alone: checkpointed sweeps
-                            r.record = sparse_certify(r.problem, r.K, r.order)
-                        except Exception as exc:
-                            r.error = exc
+                        self._to_single(r)          # too large for the arena alone: checkpointed sweeps
+                        continue
                     complete(r)
         finally:
             for r in list(owner.values()) + list(waiting):
                 if r.tag.t_done is None:
                     self._finish([r.tag], error=RuntimeError("server closed"))
+
+    def _to_single(self, r):
+        with self.cv:
+            self.single_q.append(r)
+            self.cv.notify_all()
+
+    def _sparse_single_lane(self, stream):
+        """Answers the requests handed over by the engine thread with sparse_certify (checkpointed single-query
+        sweeps at the server's state budget), one at a time on this lane's stream."""
+        t = _lib.torch()
+        while True:
+            with self.cv:
+                while not self.single_q and not self.single_closed:
+                    self.cv.wait()
+                if not self.single_q:
+                    return
+                r = self.single_q.popleft()
+            try:
+                with t.cuda.stream(stream):
+                    rec = sparse_certify(r.problem, r.K, r.order, max_states=self.sparse_max_states)
+                    stream.synchronize()
+                self._finish([r.tag], [rec])
+            except Exception as exc:
+                self._finish([r.tag], error=exc)
 
     # ------------------------------------------------------------------ statistics
     def latencies(self, kind=None):
@@ -286,28 +327,36 @@ class Server:
 
 
 def sparse_certify(p, K, order=None, max_states=30_000_000):
-    """Least weight <= K of a solution over all observables by the sparse MITM DP (auto.sparse_distance's loop,
-    cdist/auto.py:257-284, with the default split): {"distance", "witness", "observable", "witness_ok"}.  A table
-    beyond max_states raises sparse_dp.BudgetExceeded (the request is answered with that error)."""
+    """Least weight <= K of a solution over all observables by the sparse MITM DP: {"distance", "witness",
+    "observable", "witness_ok"}.  One query at a time on the single-query path.  The thresholds deepen through
+    sparse_batch.threshold_ladder(K), as the engine's requests do; at each one the observables are swept in turn,
+    after a solution of weight v the others at v - 1 (auto.sparse_distance's loop, cdist/auto.py:257-284, with the
+    default split).  A table beyond max_states raises sparse_dp.BudgetExceeded (the request is answered with that
+    error)."""
     from .ordering import greedy_order_native
     from .problem import verify_witness
     order = order if order is not None else greedy_order_native(p.h, p.nrows, p.words())
-    best = None
-    for i in range(p.nobs):
-        tr = sparse_dp.Trellis.for_query(p, i, order)
-        if not tr.feasible:
-            continue
-        k = K if best is None else best[0] - 1
-        if k < 1:
+    trs = [sparse_dp.Trellis.for_query(p, i, order) for i in range(p.nobs)]
+    best, peak = None, 0
+    for kt in sparse_batch.threshold_ladder(K):
+        for i, tr in enumerate(trs):
+            if not tr.feasible:
+                continue
+            k = kt if best is None else best[0] - 1
+            if k < 1:
+                break
+            r = sparse_dp.query(tr, k, mode="mitm", use_h=False, witness=True, backend="hip", max_states=max_states)
+            peak = max(peak, r["peak_states"])
+            if r["value"] is not None:
+                best = (r["value"], r["witness"], i)
+        if best is not None:
             break
-        r = sparse_dp.query(tr, k, mode="mitm", use_h=False, witness=True, backend="hip", max_states=max_states)
-        if r["value"] is not None:
-            best = (r["value"], r["witness"], i)
     if best is None:
-        return {"distance": None, "witness": None, "observable": None, "witness_ok": True, "K": K}
+        return {"distance": None, "witness": None, "observable": None, "witness_ok": True, "K": K, "path": "single",
+                "peak_states": peak}
     v, cols, obs = best
     return {"distance": v, "witness": cols, "observable": obs, "K": K,
-            "witness_ok": verify_witness(p, cols, v, obs)}
+            "witness_ok": verify_witness(p, cols, v, obs), "path": "single", "peak_states": peak}
 
 
 def percentile(xs, q):
@@ -318,9 +367,9 @@ def percentile(xs, q):
     return xs[i]
 
 
-def run_poisson(server, requests, rate, seed=0):
+def run_poisson(server, requests, rate, seed=0, progress=None):
     """Submit requests at Poisson arrival times of the given rate (per second); wait for all; returns
-    (tickets, wall seconds)."""
+    (tickets, wall seconds).  progress(msg) reports every 1000th submission and, while waiting, answered requests."""
     import numpy as np
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=len(requests))
@@ -336,6 +385,27 @@ def run_poisson(server, requests, rate, seed=0):
             tickets.append(server.submit_dense(*args))
         else:
             tickets.append(server.submit_sparse(*args))
-    for tk in tickets:
-        tk._event.wait()
+        if progress is not None and len(tickets) % 1000 == 0:
+            progress("%d submitted, %d answered" % (len(tickets), sum(tk._event.is_set() for tk in tickets)))
+    wait_all(tickets, progress)
     return tickets, time.perf_counter() - t0
+
+
+def wait_all(tickets, progress=None, every=20.0):
+    """Wait until every ticket is answered.  progress(msg) is called about every `every` seconds, and only when more
+    tickets have been answered since its last call."""
+    i, last, t_last = 0, -1, time.perf_counter()
+    while i < len(tickets):
+        if tickets[i]._event.wait(timeout=1.0):
+            i += 1
+        if progress is not None and time.perf_counter() - t_last >= every:
+            t_last = time.perf_counter()
+            done = sum(tk._event.is_set() for tk in tickets)
+            if done > last:
+                per = {}
+                for tk in tickets:
+                    a, n = per.get(tk.kind, (0, 0))
+                    per[tk.kind] = (a + tk._event.is_set(), n + 1)
+                progress("%d of %d answered (%s)" % (done, len(tickets), ", ".join(
+                    "%s %d/%d" % (k, a, n) for k, (a, n) in sorted(per.items()))))
+                last = done

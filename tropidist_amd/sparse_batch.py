@@ -168,17 +168,38 @@ class Engine:
         _lib.check(self.L.tdp_spb_release(self.h, rid), "tdp_spb_release")
 
 
+def threshold_ladder(K, first=4, step=2):
+    """Thresholds of the deepening sweeps of a request with threshold K: K, K - step, ... down to `first`, in
+    ascending order ([K] when K <= first).  A sweep at k that finds no solution for any observable proves the least
+    weight above k, so the first sweep that finds one gives the least weight <= K (cdist/sparse_dp.py:1110-1151
+    deepens one unit at a time).  A meet-in-the-middle sweep grows about n-fold per two units of threshold, so the
+    lower sweeps cost a small fraction of the last one and end early the requests whose least weight is small."""
+    ks = [K]
+    while ks[-1] - step >= first:
+        ks.append(ks[-1] - step)
+    return ks[::-1]
+
+
 class Request:
     """A whole sparse request: the least weight <= K over every observable of a problem (serve.sparse_certify's
-    record).  All feasible observables run at threshold K at once; the distance is their least value and the
-    observable the first reaching it -- what the sequential loop with thresholds best - 1 finds."""
-    __slots__ = ("problem", "K", "order", "trs", "ids", "record", "error", "tag", "left")
+    record).  The request deepens through threshold_ladder(K); at each threshold all feasible observables run at
+    once, and the first threshold with a solution ends it: the distance is the least value and the observable the
+    first reaching it -- what the sequential loop with thresholds best - 1 finds.  After a threshold without a
+    solution, finish() sets `pending` and the caller admits the request again for the next one."""
+    __slots__ = ("problem", "K", "order", "trs", "ids", "record", "error", "tag", "left", "ladder", "stage",
+                 "pending")
 
     def __init__(self, problem, K, order=None, tag=None):
         self.problem, self.K, self.order, self.tag = problem, K, order, tag
         self.trs, self.ids = [], []
         self.record = self.error = None
         self.left = 0
+        self.ladder, self.stage, self.pending = threshold_ladder(K), 0, False
+
+    @property
+    def k(self):
+        """The threshold of the current sweep."""
+        return self.ladder[self.stage]
 
     def plan(self):
         """Column order (the greedy straddle order by default) and the words of each feasible observable's trellis,
@@ -196,9 +217,10 @@ class Request:
         records it as the request's error, which finish() keeps."""
         N = self.problem.ncols
         self.ids = []
+        self.pending = False
         try:
             for _, prm, tm in self.trs:
-                self.ids.append(eng.add_words(prm, tm, N, self.K))
+                self.ids.append(eng.add_words(prm, tm, N, self.k))
         finally:
             self.left = len(self.ids)
             if owner is not None:
@@ -207,13 +229,15 @@ class Request:
 
     def finish(self, eng):
         """Builds the record (or the error) once every query has finished, and releases the queries.  An error
-        recorded earlier (a failed admission) stands."""
+        recorded earlier (a failed admission) stands.  A sweep below K without a solution sets `pending` and moves
+        to the next threshold instead."""
         try:
             if self.error is not None:
                 return self
-            best = None
+            best, peak = None, 0
             for (i, _, _), rid in zip(self.trs, self.ids):
-                st, v, _ = eng.result(rid)
+                st, v, info = eng.result(rid)
+                peak = max(peak, info.get("peak_states", 0))
                 if st == STATES:
                     raise sparse_dp.BudgetExceeded("a table of observable %d exceeded %d states"
                                                    % (i, eng.max_states), "states")
@@ -221,14 +245,18 @@ class Request:
                     raise MemoryError("sparse batch arena full")
                 if v is not None and (best is None or v < best[0]):
                     best = (v, i, rid)
-            if best is None:
+            if best is None and self.stage + 1 < len(self.ladder):
+                self.stage += 1
+                self.pending = True
+            elif best is None:
                 self.record = {"distance": None, "witness": None, "observable": None, "witness_ok": True,
-                               "K": self.K}
+                               "K": self.K, "path": "engine", "peak_states": peak}
             else:
                 v, i, rid = best
                 cols = eng.witness(rid, self.order, v)
                 self.record = {"distance": v, "witness": cols, "observable": i, "K": self.K,
-                               "witness_ok": verify_witness(self.problem, cols, v, i)}
+                               "witness_ok": verify_witness(self.problem, cols, v, i), "path": "engine",
+                               "peak_states": peak}
         except Exception as exc:
             self.error = exc
         finally:
@@ -284,6 +312,9 @@ def certify_requests(requests, engine=None, arena_bytes=4 << 30, stream=None, fa
                 if r.left:
                     continue
                 r.finish(eng)
+                if r.pending:                       # no solution at this threshold: the next one
+                    waiting.appendleft(r)
+                    continue
                 if fallback and isinstance(r.error, MemoryError):
                     try:
                         r.error, r.record = None, serve.sparse_certify(r.problem, r.K, r.order)
