@@ -365,10 +365,11 @@ def bench_sparse(args):
     return out
 
 
-def _serve_run(serve, reqs, rate, seed):
+def _serve_run(serve, reqs, rate, seed, max_states):
     # 160 GiB of the 288 GiB HBM: 80 GiB for dense tables and witness arenas, 80 GiB for the resident sparse tables
     # (all sparse requests of the closed-loop run hold about 65 GB at once)
-    srv = serve.Server(memory_bytes=160 << 30, dense_lanes=3, sparse_arena_bytes=80 << 30)
+    srv = serve.Server(memory_bytes=160 << 30, dense_lanes=3, sparse_arena_bytes=80 << 30,
+                       sparse_max_states=max_states)
     try:
         def note(msg):
             print("bench: serving %s (%.1f s)" % (msg, time.time() - T_START), file=sys.stderr, flush=True)
@@ -395,12 +396,12 @@ def bench_serving(args, requests):
     t.cuda.synchronize()
     t.cuda.empty_cache()
     reqs = [("dense", (p,)) if kind == "dense" else ("sparse", (p, K)) for kind, p, K in requests]
-    cap_tickets, cap_wall = _serve_run(serve, reqs, None, 0)
+    cap_tickets, cap_wall = _serve_run(serve, reqs, None, 0, args.serving_max_states)
     capacity = len(reqs) / cap_wall
     print("bench: serving capacity %.0f/s over %d requests (%.1f s)" % (capacity, len(reqs), time.time() - T_START),
           file=sys.stderr, flush=True)
     rate = args.serving_rate if args.serving_rate else args.serving_load * capacity
-    tickets, wall = _serve_run(serve, reqs, rate, 7)
+    tickets, wall = _serve_run(serve, reqs, rate, 7, args.serving_max_states)
     print("bench: serving Poisson stream at %.0f/s done (%.1f s)" % (rate, time.time() - T_START), file=sys.stderr,
           flush=True)
     lat = [tk.latency for tk in tickets]
@@ -425,15 +426,20 @@ def bench_serving(args, requests):
             kind, a = reqs[idx]
             t1 = time.perf_counter()
             try:
-                auto.certify(a[0]) if kind == "dense" else serve.sparse_certify(a[0], a[1])
+                if kind == "dense":
+                    auto.certify(a[0])
+                else:
+                    serve.sparse_certify(a[0], a[1], max_states=args.serving_max_states)
             except Exception:                   # answered with an error, as the server does
                 pass
             per_class.setdefault(c, []).append(time.perf_counter() - t1)
     ns = sum(len(v) for v in per_class.values())
     plain = 1.0 / sum(len(members[c]) / len(reqs) * (sum(v) / len(v)) for c, v in per_class.items())
-    return {"workload": "%d requests: 70%% dense width 10-20, 20%% width 21-25, 10%% sparse MITM K<=%d on QC codes "
-                        "l,m 6-15; capacity with all submitted at once, then Poisson arrivals at %.0f/s (%.0f%% of "
-                        "capacity)" % (len(reqs), args.serving_max_K, rate, 100 * rate / capacity),
+    return {"workload": "%d requests: 70%% dense width 10-20, 20%% width 21-25, 10%% sparse MITM K 4-%d on QC codes "
+                        "l,m 6-15 (tables of at most %d states); capacity with all submitted at once, then Poisson "
+                        "arrivals at %.0f/s (%.0f%% of capacity)" % (len(reqs), args.serving_max_K,
+                                                                     args.serving_max_states, rate,
+                                                                     100 * rate / capacity),
             "capacity_certs_per_s": capacity, "capacity_wall_seconds": cap_wall, "rate": rate,
             "plain_gpu_certs_per_s": plain, "plain_sample": ns, "capacity_vs_plain": capacity / plain,
             "plain_ms_per_request": {c: round(1e3 * sum(v) / len(v), 2) for c, v in sorted(per_class.items())},
@@ -442,6 +448,7 @@ def bench_serving(args, requests):
             "p99_s": serve.percentile(lat, 99), "dense_p50_s": serve.percentile(dl, 50),
             "dense_p99_s": serve.percentile(dl, 99), "sparse_p50_s": serve.percentile(sl, 50),
             "sparse_p99_s": serve.percentile(sl, 99), "errors": len(errs),
+            "certified_per_s": (len(reqs) - len(errs)) / wall, "sparse_max_states": args.serving_max_states,
             "error_kinds": sorted({type(tk.error).__name__ + ": " + str(tk.error)[:80] for tk in errs}),
             "witnesses_ok": ok, "same_distances_both_runs": same}
 
@@ -502,6 +509,9 @@ def main():
                          "smaller batches of trickling arrivals fall behind)")
     ap.add_argument("--serving-plain-sample", type=int, default=60)
     ap.add_argument("--serving-max-K", type=int, default=8)
+    ap.add_argument("--serving-max-states", type=int, default=30_000_000,
+                    help="state budget of every sparse table, on the served and the one-at-a-time path alike (a "
+                         "request past it is answered BudgetExceeded, not certified)")
     ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
                     help="only replay the dominant fused pass N times and a device copy of the same input size N "
                          "times (the reference kernel for scaling hardware byte counters), then exit")

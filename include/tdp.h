@@ -233,6 +233,7 @@ int tdp_sp_hist(const void *d_ents, uint64_t n, uint64_t *d_hist, void *stream);
 #define TDP_SPB_DONE 1        /* finished: value (or none) known */
 #define TDP_SPB_STATES 2      /* a table or changed list exceeded max_states (BudgetExceeded "states") */
 #define TDP_SPB_NOMEM 3       /* the arena could not hold the query's tables */
+#define TDP_SPB_CANCELLED 4   /* stopped by tdp_spb_cancel */
 
 /* NULL on failure (tdp_last_error). */
 void *tdp_spb_create(uint64_t arena_bytes, int max_jobs, void *stream);
@@ -252,7 +253,8 @@ int tdp_spb_step(void *h, int32_t *active);
 int tdp_spb_poll(void *h, int64_t *ids, int32_t cap, int32_t *n);
 
 /* Arena and progress counters: out[0] arena bytes, [1] bytes in use, [2] most bytes in use so far, [3] queries
- * failed for lack of arena, [4] engine steps so far, [5] queries running. */
+ * that found the arena full (TDP_SPB_NOMEM), [4] engine steps so far, [5] queries running, [6] queries that found
+ * the arena full at the last step. */
 int tdp_spb_stats(void *h, uint64_t *out);
 
 /* Status (TDP_SPB_*), the least weight <= K (-1 if none or not finished) and, if stats is not NULL, four counters:
@@ -265,6 +267,11 @@ int tdp_spb_witness(void *h, int64_t id, int32_t *positions, int32_t cap, int32_
 
 /* Frees a finished query's tables and id. */
 int tdp_spb_release(void *h, int64_t id);
+
+/* Stops a running query between steps (status TDP_SPB_CANCELLED, its tables freed; it is reported by the next poll);
+ * a finished query is left as it is.  For callers that give a request up, e.g. when one of its queries gave its
+ * memory up. */
+int tdp_spb_cancel(void *h, int64_t id);
 
 /* ------------------------------------------------------------------ meet-in-the-middle lookup tables
  * Open addressing over 2^log2slots device slots; all subsets of size <= k of n columns keyed by the 128-bit

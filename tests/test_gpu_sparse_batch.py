@@ -138,6 +138,26 @@ def test_requests_equal_sparse_certify():
         assert r.record["distance"] == (min(vals) if vals else None)
 
 
+def test_small_arena_queries_wait_for_memory():
+    print("case sparse batch requests in an arena of 6 MiB", flush=True)
+    reqs = [(problem.quasi_cyclic(l, m, seed=s), K) for l, m, s, K in
+            [(6, 6, 3001, 6), (6, 8, 3003, 8), (9, 8, 3004, 6), (8, 9, 3005, 6), (6, 9, 3006, 7), (9, 9, 3007, 8)]]
+    eng = sparse_batch.Engine(arena_bytes=6 << 20)
+    try:
+        got = sparse_batch.certify_requests(reqs, engine=eng)
+        st = eng.stats()
+        print("arena peak %d of %d bytes, %d queries gave their memory up" % (st["peak_in_use"], st["arena_bytes"],
+                                                                            st["nomem"]), flush=True)
+        assert st["running"] == 0 and st["in_use"] == 0
+    finally:
+        eng.close()
+    for (p, K), r in zip(reqs, got):
+        assert r.error is None, r.error
+        want = serve.sparse_certify(p, K)
+        assert r.record["distance"] == want["distance"] and r.record["observable"] == want["observable"]
+        assert r.record["witness_ok"]
+
+
 def test_state_budget_raises_like_one_query_path():
     print("case sparse batch state budget", flush=True)
     p = problem.quasi_cyclic(6, 6, seed=3001)

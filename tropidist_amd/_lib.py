@@ -77,6 +77,7 @@ SIGNATURES = {
     "tdp_spb_result": (ci, [vp, i64, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), vp]),
     "tdp_spb_witness": (ci, [vp, i64, vp, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tdp_spb_release": (ci, [vp, i64]),
+    "tdp_spb_cancel": (ci, [vp, i64]),
 }
 
 

@@ -274,6 +274,7 @@ struct JobRec {
     bool stepped = false;
     Ent *out = nullptr;
     uint64_t out_bytes = 0;
+    uint32_t *out_slots = nullptr;    // the new table's index, carved from the step's reservation
     int par = 0;                      // changed buffer of the step
     uint64_t nout = 0, nch = 0;
     // the join in flight (read at the next global step)
@@ -380,6 +381,7 @@ public:
     }
 
     uint64_t nomem = 0;
+    uint64_t nomem_last = 0;          // jobs that found the arena full at the last step
 
     void fail(JobRec *j, int status) {
         if (status == TDP_SPB_NOMEM) nomem++;
@@ -441,6 +443,7 @@ public:
     }
 
     int step(int *active) {
+        const uint64_t nomem_before = nomem;
         // ---- 1. column steps of every stepping job
         uint32_t nstep = 0;
         uint64_t maxn = 1;
@@ -450,12 +453,11 @@ public:
             const bool fwd = j->phase == kFw;
             const Tab &cur = fwd ? j->fw[j->p] : j->bw[j->p + 1];
             const uint64_t n = cur.n;
-            uint8_t *out = jalloc(j, 32 * std::max<uint64_t>(n, 1));
-            if (!out) { fail(j, TDP_SPB_NOMEM); continue; }
+            const uint64_t n1 = std::max<uint64_t>(n, 1);
             Ent *ch = nullptr;
             const int par = (int)(global_steps & 1);
             if (fwd) {
-                if (j->chcap[par] < 2 * std::max<uint64_t>(n, 1)) {
+                if (j->chcap[par] < 2 * n1) {
                     if (j->ch[par]) {
                         arena.release((uint8_t *)j->ch[par], 16 * j->chcap[par]);
                         j->held -= align_up(16 * j->chcap[par]);
@@ -469,6 +471,10 @@ public:
                 }
                 ch = j->ch[par];
             }
+            // one reservation for the step's output (up to 2n entries) and the new table's index
+            const uint64_t out_part = align_up(32 * n1);
+            uint8_t *out = jalloc(j, out_part + 4 * index_cap(2 * n1));
+            if (!out) { fail(j, TDP_SPB_NOMEM); continue; }
             const uint64_t *q = j->prm.data() + (size_t)j->p * kPrmWords;
             SpbStep &S = h_step[nstep++];
             S.old = cur.ents;
@@ -489,7 +495,7 @@ public:
             S.pad0 = S.pad1 = 0;
             j->stepped = true;
             j->out = (Ent *)out;
-            j->out_bytes = align_up(32 * std::max<uint64_t>(n, 1));
+            j->out_bytes = align_up(out_part + 4 * index_cap(2 * n1));
             j->par = par;
             maxn = std::max(maxn, n);
         }
@@ -536,8 +542,13 @@ public:
             j->state_steps += j->nout;
             j->peak = std::max<uint64_t>(j->peak, std::max<uint64_t>(j->nout, j->nch));
             if (std::max<uint64_t>(j->nout, j->nch) > j->max_states) { fail(j, TDP_SPB_STATES); continue; }
-            // give back the unused tail of the output reservation (the last bump allocation of the job)
-            if ((uint8_t *)j->out + j->out_bytes == j->bump) j->bump = (uint8_t *)j->out + align_up(16 * std::max<uint64_t>(j->nout, 1));
+            // the table's index follows its entries inside the reservation (16 nout <= 32 n and
+            // index_cap(nout) <= index_cap(2 n)); the unused tail is given back when the reservation is the job's
+            // last bump allocation
+            uint8_t *slots = (uint8_t *)j->out + align_up(16 * std::max<uint64_t>(j->nout, 1));
+            j->out_slots = (uint32_t *)slots;
+            uint8_t *end = slots + align_up(4 * index_cap(j->nout));
+            if ((uint8_t *)j->out + j->out_bytes == j->bump) j->bump = end;
         }
 
         // ---- 3. new tables: index, and the forward joins
@@ -549,19 +560,23 @@ public:
             if (j->phase == kDone) continue;
             Tab t;
             int cut;
+            uint64_t cap = 64;
             if (start) {
-                t.ents = (Ent *)jalloc(j, 16);
+                uint8_t *m = jalloc(j, align_up(16) + 4 * index_cap(1));
+                if (!m) { fail(j, TDP_SPB_NOMEM); continue; }
+                t.ents = (Ent *)m;
+                t.slots = (uint32_t *)(m + align_up(16));
                 t.n = 1;
+                cap = index_cap(1);
                 cut = j->phase == kBwStart ? j->N : 0;
             } else {
                 t.ents = j->out;
                 t.n = j->nout;
+                cap = index_cap(t.n);
+                t.slots = j->out_slots;
                 cut = j->phase == kBw ? j->p : j->p + 1;
             }
-            uint64_t cap = index_cap(t.n);
-            t.slots = (uint32_t *)jalloc(j, 4 * cap);
             t.capmask = cap - 1;
-            if (!t.ents || !t.slots) { fail(j, TDP_SPB_NOMEM); continue; }
             SpbPost &P = h_post[npost++];
             memset(&P, 0, sizeof(P));
             P.ents = t.ents;
@@ -613,6 +628,7 @@ public:
             k_spb_post<<<(unsigned)(npost * perp), kSpBlock, 0, st>>>(d_post, (uint32_t)perp);
             TDP_LAUNCH_CHECK();
         }
+        nomem_last = nomem - nomem_before;
         int na = 0;
         for (JobRec *j : jobs) na += j->phase != kDone;
         *active = na;
@@ -740,6 +756,7 @@ int tdp_spb_stats(void *h, uint64_t *out) {
     out[3] = b->nomem;
     out[4] = b->global_steps;
     out[5] = (uint64_t)active;
+    out[6] = b->nomem_last;
     return TDP_OK;
 }
 
@@ -778,6 +795,16 @@ int tdp_spb_witness(void *h, int64_t id, int32_t *positions, int32_t cap, int32_
     JobRec *j = b->find(id);
     TDP_REQUIRE(j != nullptr, "no sparse batch job %lld", (long long)id);
     return b->witness(j, positions, cap, n);
+}
+
+int tdp_spb_cancel(void *h, int64_t id) {
+    TDP_REQUIRE(h, "tdp_spb_cancel: null handle");
+    SpBatch *b = static_cast<SpBatch *>(h);
+    std::lock_guard<std::mutex> g(b->mu);
+    JobRec *j = b->find(id);
+    TDP_REQUIRE(j != nullptr, "no sparse batch job %lld", (long long)id);
+    if (j->phase != kDone) b->fail(j, TDP_SPB_CANCELLED);
+    return TDP_OK;
 }
 
 int tdp_spb_release(void *h, int64_t id) {

@@ -15,8 +15,9 @@ Sparse requests go to one engine thread with its own stream and a resident spars
 (sparse_arena_bytes, held for the server's life) is set apart from the dense budget.  Between any two engine steps
 the thread admits every waiting sparse request (every observable a query of the engine) and completes those whose
 queries have finished, so sparse requests that arrive while others run join them on the device, one column of all of
-them per step.  Requests are admitted while the arena is less than `sparse_admit` full (one at least when nothing
-runs); a request that finds the arena full while others run waits for memory to free.  A request that does not
+them per step.  Requests are admitted while the arena is less than `sparse_admit` full and no query found it full
+at the last step (one at least when nothing runs).  A request one of whose queries finds the arena full stops its
+other queries and goes back to the head of the queue while others run.  A request that does not
 fit alone, or has more queries than the engine has slots, goes to a lane of its own with its own stream, where the
 one-at-a-time path (sparse_certify, checkpointed sweeps) answers it while the engine keeps stepping the others.
 Every sparse request deepens through sparse_batch.threshold_ladder(K) and every table is held to
@@ -247,6 +248,7 @@ class Server:
                     complete(r)
                 st = eng.stats()
                 while waiting and (not running or (st["in_use"] < self.sparse_admit * st["arena_bytes"] and
+                                                   not st["nomem_last"] and
                                                    len(owner) + len(waiting[0].trs) <= eng.max_jobs)):
                     r = waiting.popleft()
                     try:
@@ -275,6 +277,7 @@ class Server:
                     if r is None:                   # a query of a request already answered with an error
                         eng.release(rid)
                         continue
+                    r.collect(eng, rid)
                     r.left -= 1
                     if r.left:
                         continue

@@ -26,7 +26,7 @@ from . import _lib, sparse_dp
 from .problem import verify_witness
 
 M64 = (1 << 64) - 1
-RUNNING, DONE, STATES, NOMEM = 0, 1, 2, 3
+RUNNING, DONE, STATES, NOMEM, CANCELLED = 0, 1, 2, 3, 4
 
 
 def trellis_words(tr):
@@ -150,10 +150,12 @@ class Engine:
                                                                 "peak_bytes": int(stats[3])}
 
     def stats(self):
-        """{"arena_bytes", "in_use", "peak_in_use", "nomem", "steps", "running"}."""
-        out = (ctypes.c_uint64 * 6)()
+        """{"arena_bytes", "in_use", "peak_in_use", "nomem", "steps", "running", "nomem_last"}: "nomem" counts the
+        queries that found the arena full, "nomem_last" those of the last step."""
+        out = (ctypes.c_uint64 * 7)()
         _lib.check(self.L.tdp_spb_stats(self.h, out), "tdp_spb_stats")
-        return dict(zip(("arena_bytes", "in_use", "peak_in_use", "nomem", "steps", "running"), map(int, out)))
+        return dict(zip(("arena_bytes", "in_use", "peak_in_use", "nomem", "steps", "running", "nomem_last"),
+                        map(int, out)))
 
     def witness(self, rid, tr, value):
         """Sorted problem columns of a solution of weight `value` (the query's value); tr: the query's Trellis or
@@ -166,6 +168,10 @@ class Engine:
 
     def release(self, rid):
         _lib.check(self.L.tdp_spb_release(self.h, rid), "tdp_spb_release")
+
+    def cancel(self, rid):
+        """Stops a running query (status CANCELLED, reported by the next poll); no effect on a finished one."""
+        _lib.check(self.L.tdp_spb_cancel(self.h, rid), "tdp_spb_cancel")
 
 
 def threshold_ladder(K, first=4, step=2):
@@ -187,7 +193,7 @@ class Request:
     first reaching it -- what the sequential loop with thresholds best - 1 finds.  After a threshold without a
     solution, finish() sets `pending` and the caller admits the request again for the next one."""
     __slots__ = ("problem", "K", "order", "trs", "ids", "record", "error", "tag", "left", "ladder", "stage",
-                 "pending")
+                 "pending", "results")
 
     def __init__(self, problem, K, order=None, tag=None):
         self.problem, self.K, self.order, self.tag = problem, K, order, tag
@@ -195,6 +201,7 @@ class Request:
         self.record = self.error = None
         self.left = 0
         self.ladder, self.stage, self.pending = threshold_ladder(K), 0, False
+        self.results = {}
 
     @property
     def k(self):
@@ -217,6 +224,7 @@ class Request:
         records it as the request's error, which finish() keeps."""
         N = self.problem.ncols
         self.ids = []
+        self.results = {}
         self.pending = False
         try:
             for _, prm, tm in self.trs:
@@ -227,6 +235,19 @@ class Request:
                 for rid in self.ids:
                     owner[rid] = self
 
+    def collect(self, eng, rid):
+        """Takes the result of a finished query.  A query without a value (or stopped by a budget) needs no witness,
+        so its tables are released at once: the arena is free for the others while the request's last queries
+        run."""
+        res = eng.result(rid)
+        self.results[rid] = res
+        if res[1] is None:
+            eng.release(rid)
+        if res[0] in (NOMEM, STATES):             # the request fails: its queries still running stop now
+            for other in self.ids:
+                if other not in self.results:
+                    eng.cancel(other)
+
     def finish(self, eng):
         """Builds the record (or the error) once every query has finished, and releases the queries.  An error
         recorded earlier (a failed admission) stands.  A sweep below K without a solution sets `pending` and moves
@@ -235,14 +256,18 @@ class Request:
             if self.error is not None:
                 return self
             best, peak = None, 0
-            for (i, _, _), rid in zip(self.trs, self.ids):
-                st, v, info = eng.result(rid)
-                peak = max(peak, info.get("peak_states", 0))
+            res = [(i, rid) + tuple(self.results[rid] if rid in self.results else eng.result(rid))
+                   for (i, _, _), rid in zip(self.trs, self.ids)]
+            for i, rid, st, v, info in res:
                 if st == STATES:
                     raise sparse_dp.BudgetExceeded("a table of observable %d exceeded %d states"
                                                    % (i, eng.max_states), "states")
-                if st == NOMEM:
-                    raise MemoryError("sparse batch arena full")
+            if any(st == NOMEM for _, _, st, _, _ in res):
+                raise MemoryError("sparse batch arena full")
+            if any(st != DONE for _, _, st, _, _ in res):
+                raise RuntimeError("a query of the request was cancelled")
+            for i, rid, st, v, info in res:
+                peak = max(peak, info.get("peak_states", 0))
                 if v is not None and (best is None or v < best[0]):
                     best = (v, i, rid)
             if best is None and self.stage + 1 < len(self.ladder):
@@ -261,8 +286,10 @@ class Request:
             self.error = exc
         finally:
             for rid in self.ids:
-                eng.release(rid)
+                if rid not in self.results or self.results[rid][1] is not None:
+                    eng.release(rid)
             self.ids = []
+            self.results = {}
         return self
 
 
@@ -308,6 +335,7 @@ def certify_requests(requests, engine=None, arena_bytes=4 << 30, stream=None, fa
                 if r is None:
                     eng.release(rid)
                     continue
+                r.collect(eng, rid)
                 r.left -= 1
                 if r.left:
                     continue
