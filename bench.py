@@ -413,8 +413,8 @@ def bench_serving(args, requests):
     sl = [tk.latency for tk in tickets if tk.kind == "sparse"]
     # plain GPU: a sample of the same requests answered one at a time, synchronously, by the single-problem paths
     from tropidist_amd import auto
-    # the sample is stratified: the first requests of each class (narrow dense, wide dense, sparse), each class's
-    # mean time weighted by its share of the whole mix
+    # the sample is stratified: requests evenly spaced through each class (narrow dense, wide dense, sparse), each
+    # class's mean time weighted by its share of the whole mix
     members = {}
     for idx, ((kind, _), (_, p, _)) in enumerate(zip(reqs, requests)):
         c = "sparse" if kind == "sparse" else ("dense_wide" if getattr(p, "band_width", 0) >= 21 else "dense_narrow")
@@ -422,7 +422,8 @@ def bench_serving(args, requests):
     per_c = max(1, args.serving_plain_sample // max(1, len(members)))
     per_class = {}
     for c, idxs in members.items():
-        for idx in idxs[:per_c]:
+        k = min(per_c, len(idxs))
+        for idx in [idxs[(j * len(idxs)) // k] for j in range(k)]:
             kind, a = reqs[idx]
             t1 = time.perf_counter()
             try:
@@ -433,6 +434,8 @@ def bench_serving(args, requests):
             except Exception:                   # answered with an error, as the server does
                 pass
             per_class.setdefault(c, []).append(time.perf_counter() - t1)
+        print("bench: serving plain sample of %s: %d requests, %.1f ms each (%.1f s)"
+              % (c, k, 1e3 * sum(per_class[c]) / k, time.time() - T_START), file=sys.stderr, flush=True)
     ns = sum(len(v) for v in per_class.values())
     plain = 1.0 / sum(len(members[c]) / len(reqs) * (sum(v) / len(v)) for c, v in per_class.items())
     return {"workload": "%d requests: 70%% dense width 10-20, 20%% width 21-25, 10%% sparse MITM K 4-%d on QC codes "

@@ -131,7 +131,9 @@ def test_requests_equal_sparse_certify():
         assert r.record["distance"] == want["distance"]
         assert r.record["observable"] == want["observable"]
         assert r.record["witness_ok"]
-        # both deepen through the threshold ladder: the least value equals one sweep of every observable at K
+        deep = serve.sparse_certify(p, K, deepen=True)
+        assert deep["distance"] == want["distance"] and deep["observable"] == want["observable"]
+        # the engine deepens through the threshold ladder: the least value equals one sweep of every observable at K
         vals = [sparse_dp.query(tr, K, mode="mitm", use_h=False, backend="hip")["value"]
                 for tr in (sparse_dp.Trellis.for_query(p, i, r.order) for i in range(p.nobs)) if tr.feasible]
         vals = [v for v in vals if v is not None]

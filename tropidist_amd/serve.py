@@ -106,8 +106,7 @@ class Server:
         # answered one query at a time on a lane of their own, so that the engine keeps stepping the others
         self.single_q = collections.deque()
         self.single_closed = False
-        self.single_thread = threading.Thread(target=self._sparse_single_lane, args=(t.cuda.Stream(),),
-                                              daemon=True)
+        self.single_thread = threading.Thread(target=self._sparse_single_lane, daemon=True)
         self.single_thread.start()
 
     # ------------------------------------------------------------------ submission
@@ -305,10 +304,12 @@ class Server:
             self.single_q.append(r)
             self.cv.notify_all()
 
-    def _sparse_single_lane(self, stream):
+    def _sparse_single_lane(self):
         """Answers the requests handed over by the engine thread with sparse_certify (checkpointed single-query
-        sweeps at the server's state budget), one at a time on this lane's stream."""
+        sweeps at the server's state budget), one at a time on this lane's stream.  The stream is made at the first
+        request: most mixes never need it, and every stream beyond the device's hardware queues shares one."""
         t = _lib.torch()
+        stream = None
         while True:
             with self.cv:
                 while not self.single_q and not self.single_closed:
@@ -317,6 +318,8 @@ class Server:
                     return
                 r = self.single_q.popleft()
             try:
+                if stream is None:
+                    stream = t.cuda.Stream()
                 with t.cuda.stream(stream):
                     rec = sparse_certify(r.problem, r.K, r.order, max_states=self.sparse_max_states)
                     stream.synchronize()
@@ -329,19 +332,20 @@ class Server:
         return [tk.latency for tk in self.done if tk.latency is not None and (kind is None or tk.kind == kind)]
 
 
-def sparse_certify(p, K, order=None, max_states=30_000_000):
+def sparse_certify(p, K, order=None, max_states=30_000_000, deepen=False):
     """Least weight <= K of a solution over all observables by the sparse MITM DP: {"distance", "witness",
-    "observable", "witness_ok"}.  One query at a time on the single-query path.  The thresholds deepen through
-    sparse_batch.threshold_ladder(K), as the engine's requests do; at each one the observables are swept in turn,
+    "observable", "witness_ok"}.  One query at a time on the single-query path: the observables are swept in turn,
     after a solution of weight v the others at v - 1 (auto.sparse_distance's loop, cdist/auto.py:257-284, with the
-    default split).  A table beyond max_states raises sparse_dp.BudgetExceeded (the request is answered with that
-    error)."""
+    default split).  deepen: first through the lower thresholds of sparse_batch.threshold_ladder(K), as the
+    engine's requests do (the same result; one query at a time it is slower on the serving mix, whose codes mostly
+    have distances above K).  A table beyond max_states raises sparse_dp.BudgetExceeded (the request is answered
+    with that error)."""
     from .ordering import greedy_order_native
     from .problem import verify_witness
     order = order if order is not None else greedy_order_native(p.h, p.nrows, p.words())
     trs = [sparse_dp.Trellis.for_query(p, i, order) for i in range(p.nobs)]
     best, peak = None, 0
-    for kt in sparse_batch.threshold_ladder(K):
+    for kt in (sparse_batch.threshold_ladder(K) if deepen else [K]):
         for i, tr in enumerate(trs):
             if not tr.feasible:
                 continue
