@@ -143,19 +143,55 @@ __device__ __forceinline__ StepOut sp_step_entry(const Ent *__restrict__ old, ui
     return r;
 }
 
-// Appends a step's outputs (every lane of the wave must call it).  counters: [0] table count, [1] changed count,
-// [2] overflow flag (an output past out_cap / ch_cap).
+// Appends a step's outputs with one atomic per block and counter: every thread of the block must call it, the same
+// number of times (it holds three block barriers).  Each wave counts its outputs by ballot, thread 0 and 1 reserve
+// the block's table and changed ranges, and each lane writes at its wave's offset plus its rank.  One reservation
+// per block instead of four per wave keeps the two counters, which every block of the launch shares, from
+// serialising the step.  counters: [0] table count, [1] changed count, [2] overflow flag (an output past out_cap /
+// ch_cap).
 __device__ __forceinline__ void sp_step_emit(const StepOut &r, Ent *__restrict__ out, uint64_t out_cap,
                                              Ent *__restrict__ ch, uint64_t ch_cap, unsigned long long *counters) {
-    uint64_t p;
-    p = wave_append(&counters[0], r.o1);
-    if (r.o1) { if (p < out_cap) out[p] = r.e1; else counters[2] = 1; }
-    p = wave_append(&counters[0], r.o2);
-    if (r.o2) { if (p < out_cap) out[p] = r.e2; else counters[2] = 1; }
-    p = wave_append(&counters[1], r.c1);
-    if (r.c1) { if (p < ch_cap) ch[p] = r.f1; else counters[2] = 1; }
-    p = wave_append(&counters[1], r.c2);
-    if (r.c2) { if (p < ch_cap) ch[p] = r.f2; else counters[2] = 1; }
+    constexpr int kWaves = kSpBlock / 64;
+    __shared__ uint32_t s_cnt[2][kWaves];
+    __shared__ unsigned long long s_base[2];
+    const uint32_t lane = __lane_id();
+    const uint32_t w = threadIdx.x >> 6;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const uint64_t bo1 = __ballot(r.o1), bo2 = __ballot(r.o2), bc1 = __ballot(r.c1), bc2 = __ballot(r.c2);
+    const uint32_t no1 = (uint32_t)__popcll(bo1), nc1 = (uint32_t)__popcll(bc1);
+    if (lane == 0) {
+        s_cnt[0][w] = no1 + (uint32_t)__popcll(bo2);
+        s_cnt[1][w] = nc1 + (uint32_t)__popcll(bc2);
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        uint32_t tot = 0;
+        for (int k = 0; k < kWaves; k++) tot += s_cnt[threadIdx.x][k];
+        s_base[threadIdx.x] = tot ? atomicAdd(&counters[threadIdx.x], (unsigned long long)tot) : 0ull;
+    }
+    __syncthreads();
+    uint64_t bo = s_base[0], bc = s_base[1];
+    for (uint32_t k = 0; k < w; k++) {
+        bo += s_cnt[0][k];
+        bc += s_cnt[1][k];
+    }
+    __syncthreads();                          // the next call rewrites s_cnt and s_base
+    if (r.o1) {
+        const uint64_t p = bo + (uint64_t)__popcll(bo1 & lt);
+        if (p < out_cap) out[p] = r.e1; else counters[2] = 1;
+    }
+    if (r.o2) {
+        const uint64_t p = bo + no1 + (uint64_t)__popcll(bo2 & lt);
+        if (p < out_cap) out[p] = r.e2; else counters[2] = 1;
+    }
+    if (r.c1) {
+        const uint64_t p = bc + (uint64_t)__popcll(bc1 & lt);
+        if (p < ch_cap) ch[p] = r.f1; else counters[2] = 1;
+    }
+    if (r.c2) {
+        const uint64_t p = bc + nc1 + (uint64_t)__popcll(bc2 & lt);
+        if (p < ch_cap) ch[p] = r.f2; else counters[2] = 1;
+    }
 }
 
 }  // namespace tdp

@@ -10,7 +10,7 @@
 //   key k      (if its closing bits match): old value v against the candidate partner + 1 (partner below the cap);
 //   key k^upd  (only when the partner is absent, v below the cap, closing bits match): a new state of value v + 1.
 // Every output key arises exactly once (XOR by upd is a bijection and the filter fixes the cleared bits), so the
-// outputs need no deduplication: they are appended with one atomic per wave and indexed afterwards.
+// outputs need no deduplication: they are appended with one atomic per block and counter and indexed afterwards.
 #include "tdp_sparse.h"
 
 namespace tdp {
