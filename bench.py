@@ -17,13 +17,16 @@ Beside it (each can be skipped):
             a reference-sized code (l = m = 9) against the reference's sparse_dp.c on the host;
   tree      the tree-realization DP of a banded model (R=75 N=300 W=10) on a balanced tree: join pairs per second,
             against the reference's tree_dp.c joins on the host;
+  tables    the meet-in-the-middle lookup table of every column subset of size <= 3 of the l = m = 12 code: build
+            time per entry and batched lookups per second, against the reference's gmitm.c build on the host;
   serving   the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse MITM on quasi-cyclic codes):
             capacity with every request submitted at once, then a Poisson stream at half of it, p50 / p99 latency,
             against a stratified plain-GPU sample;
   cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--skip-batched] [--skip-sparse]
-                    [--skip-serving] [--serving-requests 600] [--batch-instances 512] [--sparse-K 12]
+                    [--skip-tree] [--skip-tables] [--skip-serving] [--serving-requests 600] [--serving-max-K 8]
+                    [--serving-max-states 30000000] [--batch-instances 512] [--sparse-K 12]
 """
 import argparse
 import json
@@ -285,6 +288,63 @@ def _balanced(lo, hi):
     return (_balanced(lo, mid), _balanced(mid, hi))
 
 
+def bench_tables(args):
+    """Meet-in-the-middle lookup tables (H rows): every column subset of size <= 3 of the QC code l = m = 12 (288
+    columns: 3,981,553 subsets in 2^23 slots at load 0.5) built on the device, best of three builds; then 2^22
+    batched lookups of random 3-subsets with a flipped label, timed by HIP events on the kernel's stream.  Beside
+    it, the reference's gmitm.c tab_build (OpenMP) on the host for the CPU figure."""
+    import ctypes
+    import os
+    import numpy as np
+    import oracle
+    from tropidist_amd import _lib, tables, workloads
+    t = _lib.torch()
+    L = _lib.lib()
+    p = workloads.sparse_qc()
+    tables.Table(p, 2).free()                   # warm-up
+    builds = []
+    for _ in range(3):
+        tab = tables.Table(p, 3)
+        builds.append(tab.build_seconds)
+        if len(builds) < 3:
+            tab.free()
+    entries = int(tab.entries)
+    rng = np.random.default_rng(7)
+    m = 1 << 22
+    idx = np.sort(np.stack([rng.choice(p.ncols, 3, replace=False) for _ in range(m >> 12)]), axis=1)
+    idx = np.tile(idx, (4096, 1))
+    zh = tab.zh[idx[:, 0]] ^ tab.zh[idx[:, 1]] ^ tab.zh[idx[:, 2]]
+    zl = tab.zl[idx[:, 0]] ^ tab.zl[idx[:, 1]] ^ tab.zl[idx[:, 2]]
+    lg = (tab.lg[idx[:, 0]] ^ tab.lg[idx[:, 1]] ^ tab.lg[idx[:, 2]]) ^ np.uint64(1)
+    R = np.full(m, 3, dtype=np.int32)
+    dev = [t.from_numpy(np.ascontiguousarray(a).view(np.int64 if a.dtype == np.uint64 else np.int32)).cuda()
+           for a in (zh, zl, lg, R)]
+    res = t.empty(m, dtype=t.int32, device="cuda")
+    stream = t.cuda.current_stream()
+
+    def launch():
+        _lib.check(L.tdp_tab_lookup_batch(tab.handle, dev[0].data_ptr(), dev[1].data_ptr(), dev[2].data_ptr(),
+                                          dev[3].data_ptr(), m, res.data_ptr(), _lib.stream_handle(stream)),
+                   "tdp_tab_lookup_batch")
+    look = _time_launches(t, stream, launch, 10)
+    hits = int((res >= 0).sum().item())
+    out = {"workload": "QC l=m=12 (288 columns), seed 3000: all column subsets of size <= 3 (%d entries, 2^%d slots); "
+                       "%d lookups of 3-subsets with a flipped label" % (entries, tab.log2slots, m),
+           "entries": entries, "build_seconds": min(builds), "ns_per_entry": 1e9 * min(builds) / entries,
+           "lookups_per_s": m / look, "lookup_hits": hits}
+    tab.free()
+    if not args.no_cpu and oracle.have_ref():
+        threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+        c0 = time.perf_counter()
+        h, cnt, free = oracle.tab_build(tab.zh, tab.zl, tab.lg, 3, tab.log2slots, 1 if tab.wide else 0, impl="ref")
+        cs = time.perf_counter() - c0
+        free(h)
+        out["cpu_baseline"] = {"value": cnt / cs, "unit": "entries/s", "cores": threads, "kind": "reference",
+                               "sample": "the same table built by the reference's gmitm.c tab_build (OpenMP)",
+                               "seconds": cs, "entries": int(cnt), "matches_gpu": int(cnt) == entries}
+    return out
+
+
 def bench_tree(args):
     """Tree-realization DP (T rows): banded R=75 N=300 W=10, one logical row, a balanced tree over the natural
     column order; join pairs per second on the GPU, and the reference's tree_dp.c joins (single-threaded, in the
@@ -498,6 +558,7 @@ def main():
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-sparse", action="store_true")
     ap.add_argument("--skip-tree", action="store_true")
+    ap.add_argument("--skip-tables", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-repeats", type=int, default=5, help="timed passes over the scan (median reported)")
@@ -556,6 +617,9 @@ def main():
     if not args.skip_tree:
         out["tree"] = bench_tree(args)
         progress("tree done")
+    if not args.skip_tables:
+        out["tables"] = bench_tables(args)
+        progress("tables done")
     if not args.skip_serving:
         out["serving"] = bench_serving(args, inputs["requests"])
         progress("serving done")

@@ -67,8 +67,10 @@ __device__ __forceinline__ uint8_t cost_load(const uint8_t *cost, uint64_t i) {
     return (uint8_t)((w >> ((a & 3) * 8)) & 0xFFu);
 }
 
-// returns false when a wide insert must be retried by a later launch
-__device__ bool tab_insert(const TabView &t, bool wide, uint64_t zh, uint64_t zl, uint64_t lg, uint8_t c) {
+// returns false when a wide insert must be retried by a later launch; `added` counts the keys this thread stored
+// (the caller adds its total to the table count once, not one atomic per key on a counter every thread shares)
+__device__ bool tab_insert(const TabView &t, bool wide, uint64_t zh, uint64_t zl, uint64_t lg, uint8_t c,
+                           uint32_t &added) {
     uint64_t i = zh & t.mask;
     if (!wide) {
         uint64_t key = (zl & kFpMask) | (lg & 0xFFFF) | (1ull << 63);
@@ -78,7 +80,7 @@ __device__ bool tab_insert(const TabView &t, bool wide, uint64_t zh, uint64_t zl
                 uint64_t seen = atomicCAS(reinterpret_cast<unsigned long long *>(&t.key[i]), 0ull,
                                           (unsigned long long)key);
                 if (seen == 0) {
-                    atomicAdd(t.count, 1ull);
+                    added++;
                     cur = key;
                 } else {
                     cur = seen;
@@ -98,7 +100,7 @@ __device__ bool tab_insert(const TabView &t, bool wide, uint64_t zh, uint64_t zl
             uint64_t seen = atomicCAS(reinterpret_cast<unsigned long long *>(&t.key[i]), 0ull,
                                       (unsigned long long)key);
             if (seen == 0) {
-                atomicAdd(t.count, 1ull);
+                added++;
                 __hip_atomic_store(&t.lgs[i], lg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __threadfence();
                 cost_min(t.cost, i, c);
@@ -120,8 +122,8 @@ __device__ bool tab_insert(const TabView &t, bool wide, uint64_t zh, uint64_t zl
 
 __device__ __forceinline__ void insert_or_defer(const TabView &t, bool wide, uint64_t zh, uint64_t zl, uint64_t lg,
                                                 uint8_t c, Retry *retry, unsigned long long *nretry,
-                                                uint64_t retry_cap) {
-    if (!tab_insert(t, wide, zh, zl, lg, c)) {
+                                                uint64_t retry_cap, uint32_t &added) {
+    if (!tab_insert(t, wide, zh, zl, lg, c, added)) {
         unsigned long long r = atomicAdd(nretry, 1ull);
         if (r < retry_cap) retry[r] = Retry{zh, zl, lg, c};
     }
@@ -140,8 +142,10 @@ __global__ void k_tab_small(TabView t, int wide, int n, const uint64_t *zh, cons
                             Retry *retry, unsigned long long *nretry, uint64_t retry_cap) {
     int j = (int)(blockIdx.x * kTabBlock + threadIdx.x) - 1;
     if (j >= n) return;
-    if (j < 0) insert_or_defer(t, wide, 0, 0, 0, 0, retry, nretry, retry_cap);
-    else insert_or_defer(t, wide, zh[j], zl[j], lg[j], 1, retry, nretry, retry_cap);
+    uint32_t added = 0;
+    if (j < 0) insert_or_defer(t, wide, 0, 0, 0, 0, retry, nretry, retry_cap, added);
+    else insert_or_defer(t, wide, zh[j], zl[j], lg[j], 1, retry, nretry, retry_cap, added);
+    if (added) atomicAdd(t.count, (unsigned long long)added);
 }
 
 // subsets of size 2..k with leading pair (i, j), i < j, remaining indices ascending (depth first)
@@ -178,18 +182,18 @@ __global__ void k_tab_pairs(TabView t, int wide, int n, int k, const uint64_t *z
     h[2] = zh[i] ^ zh[j];
     l[2] = zl[i] ^ zl[j];
     g[2] = lg[i] ^ lg[j];
-    insert_or_defer(t, wide, h[2], l[2], g[2], 2, retry, nretry, retry_cap);
-    if (k <= 2) return;
+    uint32_t added = 0;
+    insert_or_defer(t, wide, h[2], l[2], g[2], 2, retry, nretry, retry_cap, added);
     int depth = 2;           // subset size of h[depth]
     at[depth] = j;           // last index used
     int next = j + 1;
-    for (;;) {
+    while (k > 2) {
         if (next < n) {
             int d1 = depth + 1;
             h[d1] = h[depth] ^ zh[next];
             l[d1] = l[depth] ^ zl[next];
             g[d1] = g[depth] ^ lg[next];
-            insert_or_defer(t, wide, h[d1], l[d1], g[d1], (uint8_t)d1, retry, nretry, retry_cap);
+            insert_or_defer(t, wide, h[d1], l[d1], g[d1], (uint8_t)d1, retry, nretry, retry_cap, added);
             if (d1 < k) {
                 at[d1] = next;
                 depth = d1;
@@ -203,6 +207,7 @@ __global__ void k_tab_pairs(TabView t, int wide, int n, int k, const uint64_t *z
             depth--;
         }
     }
+    if (added) atomicAdd(t.count, (unsigned long long)added);
 }
 
 __global__ void k_tab_retry(TabView t, const Retry *in, uint64_t m, Retry *out, unsigned long long *nout,
@@ -210,7 +215,9 @@ __global__ void k_tab_retry(TabView t, const Retry *in, uint64_t m, Retry *out, 
     uint64_t i = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
     if (i >= m) return;
     Retry r = in[i];
-    insert_or_defer(t, true, r.zh, r.zl, r.lg, (uint8_t)r.cost, out, nout, cap);
+    uint32_t added = 0;
+    insert_or_defer(t, true, r.zh, r.zl, r.lg, (uint8_t)r.cost, out, nout, cap, added);
+    if (added) atomicAdd(t.count, (unsigned long long)added);
 }
 
 // Least cost <= R of an entry with the query's fingerprint and a different label; -1 if none (gmitm.c:136-163).
