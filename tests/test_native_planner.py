@@ -124,3 +124,10 @@ def test_sparse_trellis_words_native_equal_python():
             assert np.array_equal(prm[:p.ncols], wprm[:p.ncols]) and np.array_equal(tm, wtm), q
             checked += 1
     assert checked > 200
+
+
+def test_planning_order_is_best_orders_choice():
+    from tropidist_amd import auto, problem
+    for seed in range(4):
+        p = problem.banded(R=60, N=240, kmin=2, kmax=4, W=8 + seed, nobs=2, rho=0.1, seed=seed)
+        assert auto.planning_order(p) == auto.best_order(p)[0]

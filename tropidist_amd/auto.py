@@ -1,6 +1,7 @@
 """Front end of the dense exact evaluator on the GPU (the call surface of the reference's cdist/auto.py:43-182):
 
     best_order(p)              the column order to plan with (the natural order and its cut profile)
+    planning_order(p)          the same order without its cut profile (what the planners use)
     estimate(p, ...)           plan cost without running anything: table bytes, work, predicted seconds
     dp_distance(p, order)      exact distance, one GPU plan per logical observable
     dp_witness(p, order, obs)  columns of a minimum-cost solution of one observable, by GPU traceback
@@ -36,6 +37,12 @@ def best_order(p, seeds=0, time_limit=None):
     c = list(range(p.ncols))
     prof = trellis.cut_profile_native(p.h, c, p.nrows, p.words())
     return c, (max(prof), math.log2(sum(2.0 ** x for x in prof)))
+
+
+def planning_order(p):
+    """best_order(p)[0] without computing the cut profile: the candidates of cdist/auto.py:48-64 are the natural
+    order and its reverse, which ties and loses the tie, so the natural order is always chosen."""
+    return list(range(p.ncols))
 
 
 def _plans(p, order, observables=None):
@@ -155,7 +162,7 @@ def certify(p, order=None, max_table_bytes=None):
     "witness_ok", "per_observable", "seconds"} (the record fields of bench/common.py:411-466).  The lower bound is
     the DP minimum over all observables; the upper bound is the cost of the witness, checked against H and L."""
     t0 = time.time()
-    order = best_order(p)[0] if order is None else order
+    order = planning_order(p) if order is None else order
     best, per = dp_distance(p, order, max_table_bytes=max_table_bytes)
     rec = {"distance": best, "per_observable": per, "lower_bound": best, "upper_bound": None, "witness": None,
            "observable": None, "witness_ok": False}

@@ -181,7 +181,7 @@ def plan_problems(problems, orders=None, compile_wide=True):
     each plan).  compile_wide: also compile the fused programs (value and witness) of plans with 2^WIDE_BITS
     entries or more, which the wide path runs (cached on the plans)."""
     from .trellis import build_plans_native
-    orders = orders or [auto.best_order(p)[0] for p in problems]
+    orders = orders or [auto.planning_order(p) for p in problems]
     flat, owner = [], []
     for pi, (p, o) in enumerate(zip(problems, orders)):
         plans = build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], o, p.nrows, p.w, words=p.words())

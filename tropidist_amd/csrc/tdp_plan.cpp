@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <functional>
 #include <queue>
+#include <utility>
 #include <vector>
 
 #include "../../include/tdp.h"
@@ -52,7 +53,7 @@ std::vector<Row> minimal_span_form(std::vector<Row> rows, int npos) {
             int s = r.lowbit();
             if (by_start[s] < 0) {
                 by_start[s] = (int)store.size();
-                store.push_back(r);
+                store.push_back(std::move(r));
                 break;
             }
             Row &held = store[by_start[s]];
@@ -67,22 +68,21 @@ std::vector<Row> minimal_span_form(std::vector<Row> rows, int npos) {
     std::vector<Row> out;
     out.reserve(store.size());
     for (Row *rp : stack) {
-        Row r = *rp;
+        Row r = std::move(*rp);                  // store is not read again
         while (true) {
             int e = r.bitlen();
             if (by_end[e] < 0) {
                 by_end[e] = (int)out.size();
-                out.push_back(r);
+                out.push_back(std::move(r));
                 break;
             }
             Row &held = out[by_end[e]];
             if (held.lowbit() > r.lowbit()) {
                 r.xor_with(held);
             } else {
-                Row next = held;
-                next.xor_with(r);
-                held = r;
-                r = next;
+                // held takes r, and r continues as held ^ r
+                std::swap(held, r);
+                r.xor_with(held);
             }
         }
     }
@@ -184,7 +184,7 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
         r.hi = pw;
         r.trim();
     }
-    std::vector<Row> hrows = minimal_span_form(rows, ncols);
+    std::vector<Row> hrows = minimal_span_form(std::move(rows), ncols);
     for (int q = 0; q < nq; q++) {
         uint64_t *qops = ops + (int64_t)q * ops_cap * 3;
         int32_t *qcols = columns + (int64_t)q * ops_cap;
@@ -424,7 +424,7 @@ int tdp_sp_trellis(const uint64_t *h_words, int ncols, int nrows, const int32_t 
     std::vector<Row> rows;
     int rc = rows_over_positions(h_words, ncols, nrows, order, rows);
     if (rc) return rc;
-    std::vector<Row> hrows = minimal_span_form(rows, ncols);
+    std::vector<Row> hrows = minimal_span_form(std::move(rows), ncols);
     const int N = ncols;
     for (int q = 0; q < nq; q++) {
         Row lrow = reduce_target_row(target_row(targets + (int64_t)q * ncols, order, ncols), hrows, ncols);

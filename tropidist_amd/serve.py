@@ -155,7 +155,7 @@ class Server:
     @staticmethod
     def _plan(tk):
         if tk.plans is None:
-            order = tk.order if tk.order is not None else auto.best_order(tk.problem)[0]
+            order = tk.order if tk.order is not None else auto.planning_order(tk.problem)
             tk.order = order
             orders, flat, owner = batch.plan_problems([tk.problem], [order])
             tk.plans = (flat, owner)
