@@ -21,6 +21,7 @@
 namespace tdp {
 
 constexpr int kPassBlock = 256;
+constexpr int kPassWaves = 6;       // minimum waves per SIMD asked of the pass kernels
 constexpr int kMaxRuns = 12;
 enum { MAP_IN = 0, MAP_OUT = 1, MAP_NEW = 2, MAP_SEL = 3 };
 
@@ -216,8 +217,11 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
     }
 }
 
+// Six waves per SIMD: the cube of a D = 5 pass then fits 80 VGPRs without spilling (the compiler's own choice, 88,
+// allows five), and the extra wave hides more of the loads' latency: the widest passes gain 1-4% (kernel trace and
+// HIP events).
 template <int D, bool BYTE, bool FREE_NEW, typename IDX>
-__global__ __launch_bounds__(kPassBlock) void k_pass(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
+__global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
                                                      uint64_t nthreads, const PassArgs a) {
     const uint64_t block = (uint64_t)blockIdx.y * gridDim.x + blockIdx.x;
     if (block * kPassBlock + threadIdx.x < nthreads)
@@ -493,7 +497,9 @@ __device__ void tiny_ops(const OpJob &J) {
     }
 }
 
-__global__ __launch_bounds__(kPassBlock) void k_pass_multi(const PassArgs *__restrict__ descs,
+// The same bound spills 16 bytes per lane here (job lookup state); the headline step of four programs is still 2%
+// faster with it than at five waves.
+__global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass_multi(const PassArgs *__restrict__ descs,
                                                            const Job *__restrict__ jobs, uint32_t njobs) {
     __shared__ uint32_t s_count;
     if (threadIdx.x == 0) s_count = 0;
