@@ -497,8 +497,8 @@ __device__ void tiny_ops(const OpJob &J) {
     }
 }
 
-// The same bound spills 16 bytes per lane here (job lookup state); the headline step of four programs is still 2%
-// faster with it than at five waves.
+// The same bound spills 16 bytes per lane here; the headline step of four programs is still 2% faster with it than
+// at five waves.
 __global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass_multi(const PassArgs *__restrict__ descs,
                                                            const Job *__restrict__ jobs, uint32_t njobs) {
     __shared__ uint32_t s_count;
