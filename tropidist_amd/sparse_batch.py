@@ -238,15 +238,19 @@ class Request:
     def collect(self, eng, rid):
         """Takes the result of a finished query.  A query without a value (or stopped by a budget) needs no witness,
         so its tables are released at once: the arena is free for the others while the request's last queries
-        run."""
-        res = eng.result(rid)
-        self.results[rid] = res
-        if res[1] is None:
-            eng.release(rid)
-        if res[0] in (NOMEM, STATES):             # the request fails: its queries still running stop now
-            for other in self.ids:
-                if other not in self.results:
-                    eng.cancel(other)
+        run.  An error here becomes the request's error (finish() keeps it)."""
+        try:
+            res = eng.result(rid)
+            self.results[rid] = res
+            if res[1] is None:
+                eng.release(rid)
+            if res[0] in (NOMEM, STATES):         # the request fails: its queries still running stop now
+                for other in self.ids:
+                    if other not in self.results:
+                        eng.cancel(other)
+        except Exception as exc:
+            if self.error is None:
+                self.error = exc
 
     def finish(self, eng):
         """Builds the record (or the error) once every query has finished, and releases the queries.  An error
@@ -287,7 +291,11 @@ class Request:
         finally:
             for rid in self.ids:
                 if rid not in self.results or self.results[rid][1] is not None:
-                    eng.release(rid)
+                    try:
+                        eng.release(rid)
+                    except Exception as exc:      # reported with the request; the other queries still go
+                        if self.error is None:
+                            self.error, self.record = exc, None
             self.ids = []
             self.results = {}
         return self
