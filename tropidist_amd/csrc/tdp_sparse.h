@@ -78,6 +78,16 @@ __device__ __forceinline__ uint64_t wave_append(unsigned long long *counter, boo
     return base + rank;
 }
 
+// Least value over the 64 lanes of the wave (every lane must call it).
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint32_t w = (uint32_t)__shfl_xor((int)v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
 struct StepArgs {
     uint64_t upd_lo, upd_hi;
     uint64_t sm_lo, sm_hi, sb_lo, sb_hi;      // closing rows: mask and required bits

@@ -62,11 +62,16 @@ __global__ __launch_bounds__(kSpBlock) void k_join_total(const Ent *__restrict__
                                                          const Ent *__restrict__ bents,
                                                          const uint32_t *__restrict__ bslots, uint64_t capmask,
                                                          uint64_t tm_lo, uint64_t tm_hi, unsigned long long *best) {
-    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    if (i >= nf) return;
-    Ent x = f[i];
-    uint32_t vb = sp_find(bents, bslots, capmask, x.lo ^ tm_lo, (x.hv & kKeyMask) ^ tm_hi);
-    if (vb != 255u) atomicMin(&best[0], (unsigned long long)((x.hv >> 56) + vb));
+    // one atomic per wave on best[0], which every thread of the launch would otherwise hit
+    const uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
+    uint32_t t = ~0u;
+    if (i < nf) {
+        Ent x = f[i];
+        uint32_t vb = sp_find(bents, bslots, capmask, x.lo ^ tm_lo, (x.hv & kKeyMask) ^ tm_hi);
+        if (vb != 255u) t = (uint32_t)(x.hv >> 56) + vb;
+    }
+    t = wave_min_u32(t);
+    if (__lane_id() == 0 && t != ~0u) atomicMin(&best[0], (unsigned long long)t);
 }
 
 __global__ __launch_bounds__(kSpBlock) void k_join_pick(const Ent *__restrict__ f, uint64_t nf,

@@ -129,10 +129,18 @@ __global__ __launch_bounds__(kSpBlock) void k_spb_post(const SpbPost *__restrict
     if (x == 0 && threadIdx.x < 3) J.cnt[threadIdx.x] = 0;
     const uint64_t stride = (uint64_t)per_job * kSpBlock;
     for (uint64_t i = (uint64_t)x * kSpBlock + threadIdx.x; i < J.n; i += stride) sp_insert(J.ents, i, J.slots, J.capmask);
-    for (uint64_t i = (uint64_t)x * kSpBlock + threadIdx.x; i < J.jn; i += stride) {
-        Ent f = J.jf[i];
-        uint32_t vb = sp_find(J.jb, J.jslots, J.jcapmask, f.lo ^ J.tm_lo, (f.hv & kKeyMask) ^ J.tm_hi);
-        if (vb != 255u) atomicMin(&J.cnt[3], (unsigned long long)((f.hv >> 56) + vb));
+    // the joins: one atomic per wave on the job's least total (the loop bound is uniform over the block, so every
+    // lane reaches the wave minimum)
+    for (uint64_t base = (uint64_t)x * kSpBlock; base < J.jn; base += stride) {
+        const uint64_t i = base + threadIdx.x;
+        uint32_t t = ~0u;
+        if (i < J.jn) {
+            Ent f = J.jf[i];
+            uint32_t vb = sp_find(J.jb, J.jslots, J.jcapmask, f.lo ^ J.tm_lo, (f.hv & kKeyMask) ^ J.tm_hi);
+            if (vb != 255u) t = (uint32_t)(f.hv >> 56) + vb;
+        }
+        t = wave_min_u32(t);
+        if (__lane_id() == 0 && t != ~0u) atomicMin(&J.cnt[3], (unsigned long long)t);
     }
 }
 
