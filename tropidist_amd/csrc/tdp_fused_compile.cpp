@@ -143,7 +143,7 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
     if (logical_out) {
         phys = lay;
     } else {
-        std::vector<char> alive(nvar, 0);
+        char alive[kMaxVars + 1] = {0};
         for (int v : lay) alive[v] = 1;
         phys.assign(b_out, -1);
         std::vector<int> movers, order;
@@ -159,7 +159,8 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
         for (int k = 0; k < b_out; k++)
             if (phys[k] < 0) phys[k] = order[o++];
     }
-    std::vector<int> out_pos(nvar, -1);
+    int out_pos[kMaxVars + 1];
+    std::fill(out_pos, out_pos + nvar, -1);
     for (int p = 0; p < b_out; p++) out_pos[phys[p]] = p;
     out_log.resize(lay.size());
     for (size_t p = 0; p < lay.size(); p++) out_log[p] = out_pos[lay[p]];
@@ -180,7 +181,7 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
     const int D = (int)dirs.size();
     std::vector<uint64_t> red;
     for (const Dir &d : dirs) red.push_back(d.mh);
-    std::vector<char> is_piv(nvar, 0);
+    char is_piv[kMaxVars + 1] = {0};
     for (int k = 0; k < D; k++) {
         uint64_t v = red[k];
         // pivot: the highest opened-row variable of the reduced vector, else its highest input bit
@@ -197,14 +198,17 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
         for (int k2 = 0; k2 < D; k2++)
             if (k2 != k && ((red[k2] >> pv) & 1)) red[k2] ^= v;
     }
-    std::vector<int> new_idx(nvar, -1), sel_idx(nvar, -1);
+    int new_idx[kMaxVars + 1], sel_idx[kMaxVars + 1];
+    std::fill(new_idx, new_idx + nvar, -1);
+    std::fill(sel_idx, sel_idx + nvar, -1);
     for (size_t k = 0; k < new_vars.size(); k++) new_idx[new_vars[k]] = (int)k;
     for (size_t k = 0; k < sel_vars.size(); k++) sel_idx[sel_vars[k]] = (int)k;
     auto deltas = [&](uint64_t mask, uint64_t &din, uint64_t &dout, uint32_t &dnew, uint32_t &dsel) {
         din = dout = 0;
         dnew = dsel = 0;
-        for (int v = 0; v < nvar; v++) {
-            if (!((mask >> v) & 1)) continue;
+        for (uint64_t x = mask; x; x &= x - 1) {          // the variables of the mask (free ones have one)
+            const int v = __builtin_ctzll(x);
+            if (v >= nvar) break;
             if (v < b_in) din |= 1ull << v;
             if (out_pos[v] >= 0) dout |= 1ull << out_pos[v];
             if (new_idx[v] >= 0) dnew |= 1u << new_idx[v];
