@@ -354,19 +354,24 @@ extern "C" int tdp_fused_compile(const uint64_t *ops, int64_t nops, int dmax, in
         int nvar = b;
         while (kk < ncol) {
             const int64_t c0 = cols[kk].first, c1 = cols[kk].second;
-            std::vector<int> lay2(lay);
+            int lay2[kMaxVars];                     // the layout after the column's expands (variables < 64)
+            int n2 = (int)std::min<size_t>(lay.size(), kMaxVars);
+            std::copy(lay.begin(), lay.begin() + n2, lay2);
             int nv2 = nvar;
             uint64_t m = 0;
             int ne = 0, nsl = 0;
-            bool over = false;
-            for (int64_t i = c0; i < c1; i++) {
+            bool over = (int)lay.size() > kMaxVars;
+            for (int64_t i = c0; i < c1 && !over; i++) {
                 uint64_t code = ops[3 * i], a = ops[3 * i + 1];
                 if (code == 0) {
-                    if (nv2 >= kMaxVars) over = true;
-                    lay2.push_back(nv2++);
+                    if (nv2 >= kMaxVars || n2 >= kMaxVars) {
+                        over = true;                // the column is not taken (checked below)
+                        break;
+                    }
+                    lay2[n2++] = nv2++;
                     ne++;
                 } else if (code == 1) {
-                    for (size_t p = 0; p < lay2.size() && p < 64; p++)
+                    for (int p = 0; p < n2; p++)
                         if ((a >> p) & 1) m |= 1ull << (lay2[p] & 63);
                 } else {
                     nsl++;
