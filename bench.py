@@ -20,7 +20,7 @@ Beside it (each can be skipped):
   tables    the meet-in-the-middle lookup table of every column subset of size <= 3 of the l = m = 12 code: build
             time per entry and batched lookups per second, against the reference's gmitm.c build on the host;
   serving   the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse MITM on quasi-cyclic codes):
-            capacity with every request submitted at once, then a Poisson stream at half of it, p50 / p99 latency,
+            capacity with every request submitted at once, then a Poisson stream at a quarter of it, p50 / p99 latency,
             against a stratified plain-GPU sample;
   cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
 
@@ -568,9 +568,10 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=600)
     ap.add_argument("--serving-rate", type=float, default=None,
                     help="Poisson arrival rate (default: --serving-load times the measured capacity)")
-    ap.add_argument("--serving-load", type=float, default=0.5,
-                    help="Poisson rate as a fraction of the closed-loop capacity (0.5 is sustained; at 0.8 the "
-                         "smaller batches of trickling arrivals fall behind)")
+    ap.add_argument("--serving-load", type=float, default=0.25,
+                    help="Poisson rate as a fraction of the closed-loop capacity (the closed loop runs batches of "
+                         "hundreds; trickling arrivals make small ones, and about a quarter of that rate is "
+                         "sustained)")
     ap.add_argument("--serving-plain-sample", type=int, default=60)
     ap.add_argument("--serving-max-K", type=int, default=8)
     ap.add_argument("--serving-max-states", type=int, default=30_000_000,

@@ -84,6 +84,8 @@ class Server:
         self.sparse_admit = sparse_admit        # admit sparse requests while the arena is less full than this
         self.in_use = 0
         self.max_batch = max_batch
+        self.idle_lanes = 0                     # dense lanes waiting for requests
+        self.share_above = 64                   # queue length from which a lane leaves a share to idle lanes
         self.dense_q = collections.deque()
         self.sparse_q = collections.deque()
         self.cv = threading.Condition()
@@ -166,11 +168,18 @@ class Server:
         t = _lib.torch()
         while True:
             with self.cv:
+                self.idle_lanes += 1
                 while not self.dense_q and not self.closed:
                     self.cv.wait()
+                self.idle_lanes -= 1
                 if not self.dense_q and self.closed:
                     return
-                claimed = [self.dense_q.popleft() for _ in range(min(self.max_batch, len(self.dense_q)))]
+                # a long queue is shared with the lanes that are idle too, so that they plan and run in parallel;
+                # a short one (trickling arrivals) is taken whole
+                n = len(self.dense_q)
+                if n >= self.share_above:
+                    n = -(-n // (1 + self.idle_lanes))
+                claimed = [self.dense_q.popleft() for _ in range(min(self.max_batch, n))]
             for tk in claimed:                    # host planning outside the lock
                 self._plan(tk)
             while claimed:
