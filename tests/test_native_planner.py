@@ -131,3 +131,14 @@ def test_planning_order_is_best_orders_choice():
     for seed in range(4):
         p = problem.banded(R=60, N=240, kmin=2, kmax=4, W=8 + seed, nobs=2, rho=0.1, seed=seed)
         assert auto.planning_order(p) == auto.best_order(p)[0]
+
+
+def test_more_than_63_active_rows_is_refused():
+    # a dense random 80 x 200 matrix: about 80 rows are active in the middle of any order
+    import pytest
+    rng = random.Random(5)
+    nrows, ncols = 80, 200
+    h = [rng.getrandbits(nrows) | 1 for _ in range(ncols)]
+    l = [rng.getrandbits(1) for _ in range(ncols)]
+    with pytest.raises(ValueError):
+        trellis.build_plans_native(h, l, [1], list(range(ncols)), nrows)

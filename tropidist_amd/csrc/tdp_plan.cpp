@@ -209,12 +209,26 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
         all.push_back(&lrow);
         std::vector<int> target(all.size(), 0);
         target.back() = 1;
-        std::vector<std::vector<int>> opening(ncols), closing(ncols);
+        // rows opening / closing at each column, as index lists in row order (counting sort, no per-column lists)
+        std::vector<int> open_at(ncols + 1, 0), close_at(ncols + 1, 0);
         for (size_t i = 0; i < all.size(); i++) {
-            opening[all[i]->lowbit()].push_back((int)i);
-            closing[all[i]->bitlen() - 1].push_back((int)i);
+            open_at[all[i]->lowbit() + 1]++;
+            close_at[all[i]->bitlen()]++;
         }
-        std::vector<int> active;
+        for (int j = 0; j < ncols; j++) {
+            open_at[j + 1] += open_at[j];
+            close_at[j + 1] += close_at[j];
+        }
+        std::vector<int> opening(all.size()), closing(all.size());
+        {
+            std::vector<int> fo(open_at.begin(), open_at.end() - 1), fc(close_at.begin(), close_at.end() - 1);
+            for (size_t i = 0; i < all.size(); i++) {
+                opening[fo[all[i]->lowbit()]++] = (int)i;
+                closing[fc[all[i]->bitlen() - 1]++] = (int)i;
+            }
+        }
+        std::vector<int> active, ending;
+        std::vector<const uint64_t *> active_w;   // the active rows' words, beside `active`
         int64_t n = 0, k = 0;
         int bits = 0, pk = 0;
         double wk = 0;
@@ -227,16 +241,22 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
             n++;
         };
         for (int j = 0; j < ncols && !overflow; j++) {
-            for (int idx : opening[j]) {
-                active.push_back(idx);
+            for (int t = open_at[j]; t < open_at[j + 1]; t++) {
+                active.push_back(opening[t]);
+                active_w.push_back(all[opening[t]]->w.data());
                 emit(0, 0, 0);
                 bits++;
             }
             pk = std::max(pk, (int)active.size());
             uint64_t mask = 0;
-            for (size_t p = 0; p < active.size(); p++)
-                if (all[active[p]]->test(j)) mask |= 1ull << p;
-            if (mask) {
+            const int jw = j >> 6, jb = j & 63;
+            uint64_t beyond = 0;                    // a row past slot 63 has this column (an error below)
+            for (size_t p = 0; p < active_w.size(); p++) {
+                const uint64_t bit = (active_w[p][jw] >> jb) & 1ull;
+                if (p < 64) mask |= bit << p;
+                else beyond |= bit;
+            }
+            if (mask || beyond) {
                 if (active.size() > 63) {
                     tdp::set_error("plan: more than 63 active rows");
                     return TDP_EINVAL;
@@ -245,8 +265,9 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
                 emit(1, mask, weights ? (uint64_t)weights[order[j]] : 1);
                 wk += (double)(1ull << bits);
             }
-            std::vector<int> ending;
-            for (int idx : closing[j]) {
+            ending.clear();
+            for (int t = close_at[j]; t < close_at[j + 1]; t++) {
+                const int idx = closing[t];
                 for (size_t p = 0; p < active.size(); p++)
                     if (active[p] == idx) ending.push_back((int)p);
             }
@@ -254,6 +275,7 @@ int tdp_plan_build(const uint64_t *h_words, int ncols, int nrows, const int32_t 
             for (int p : ending) {
                 emit(2, (uint64_t)p, (uint64_t)target[active[p]]);
                 active.erase(active.begin() + p);
+                active_w.erase(active_w.begin() + p);
                 bits--;
             }
         }
