@@ -14,6 +14,8 @@ from __future__ import annotations
 
 from dataclasses import dataclass, field
 
+import itertools
+
 import numpy as np
 
 OP_EXPAND, OP_UPDATE, OP_SELECT = 0, 1, 2
@@ -54,7 +56,10 @@ def column_words(cols, nrows):
     if not cols:
         return np.zeros((1, rw), dtype=np.uint64)
     nb = 8 * rw
-    raw = b"".join(int(c).to_bytes(nb, "little") for c in cols)
+    try:                                      # map calls int.to_bytes without a Python frame per column
+        raw = b"".join(map(int.to_bytes, cols, itertools.repeat(nb), itertools.repeat("little")))
+    except TypeError:                         # columns that are not Python ints (numpy scalars)
+        raw = b"".join(int(c).to_bytes(nb, "little") for c in cols)
     return np.frombuffer(raw, dtype=np.uint64).reshape(len(cols), rw).copy()
 
 
