@@ -330,8 +330,9 @@ def build_plans_native(h, l, obs_masks, order, nrows, weights=None, words=None):
     order_a = np.asarray(order, dtype=np.int32)
     w = None if weights is None else np.asarray(weights, dtype=np.int32)
     cap = ncols + 2 * (nrows + 1) + 4
-    ops = np.zeros((max(1, nq), cap, 3), dtype=np.uint64)
-    cols = np.zeros((max(1, nq), cap), dtype=np.int32)
+    # the planner writes every field of every query and the first nops[q] ops / columns it reports
+    ops = np.empty((max(1, nq), cap, 3), dtype=np.uint64)
+    cols = np.empty((max(1, nq), cap), dtype=np.int32)
     nops = np.zeros(max(1, nq), dtype=np.int64)
     peak = np.zeros(max(1, nq), dtype=np.int32)
     work = np.zeros(max(1, nq), dtype=np.float64)
