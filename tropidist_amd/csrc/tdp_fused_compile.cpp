@@ -112,11 +112,16 @@ int address_runs(const uint64_t *vals, int n, uint32_t *runs) {
 bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vector<int> &in_log, bool logical_out,
                 Pass &P, std::vector<int> &out_log) {
     const int b_in = (int)in_log.size();
-    std::vector<int> lay(in_log);
+    // scratch lists reused across calls (one set per thread: serving lanes compile concurrently)
+    static thread_local std::vector<int> lay, new_vars, sel_vars, phys, movers, order, free;
+    static thread_local std::vector<Update> upd;
+    static thread_local std::vector<uint64_t> red, basis;
+    lay.assign(in_log.begin(), in_log.end());
+    new_vars.clear();
+    sel_vars.clear();
+    upd.clear();
     int nvar = b_in;
-    std::vector<int> new_vars, sel_vars;
     int sel_t[kMaxVars];
-    std::vector<Update> upd;
     P.upd_ops.clear();
     for (int64_t i = first; i < last; i++) {
         uint64_t code = ops[3 * i], a = ops[3 * i + 1], b = ops[3 * i + 2];
@@ -139,14 +144,14 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
         }
     }
     const int b_out = (int)lay.size();
-    std::vector<int> phys;
     if (logical_out) {
-        phys = lay;
+        phys.assign(lay.begin(), lay.end());
     } else {
         char alive[kMaxVars + 1] = {0};
         for (int v : lay) alive[v] = 1;
         phys.assign(b_out, -1);
-        std::vector<int> movers, order;
+        movers.clear();
+        order.clear();
         for (int v = 0; v < b_in; v++)
             if (alive[v]) {
                 if (v < b_out) phys[v] = v;
@@ -170,8 +175,10 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
         uint32_t lam;
         uint64_t w;
     };
-    std::vector<Dir> dirs;
-    std::vector<std::pair<uint32_t, uint64_t>> lows;
+    static thread_local std::vector<Dir> dirs;
+    static thread_local std::vector<std::pair<uint32_t, uint64_t>> lows;
+    dirs.clear();
+    lows.clear();
     for (const Update &u : upd) {
         uint64_t mh = u.m & ~3ull;
         uint32_t lam = (uint32_t)(u.m & 3);
@@ -179,7 +186,7 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
         else if (u.m) lows.push_back({lam, u.w});
     }
     const int D = (int)dirs.size();
-    std::vector<uint64_t> red;
+    red.clear();
     for (const Dir &d : dirs) red.push_back(d.mh);
     char is_piv[kMaxVars + 1] = {0};
     for (int k = 0; k < D; k++) {
@@ -217,10 +224,10 @@ bool build_pass(const uint64_t *ops, int64_t first, int64_t last, const std::vec
     };
     PassArgs &A = P.args;
     memset(&A, 0, sizeof(A));
-    std::vector<uint64_t> basis = {1, 2};
+    basis.assign({1, 2});
     for (const Dir &d : dirs) basis.push_back(d.mh);
     for (size_t k = 0; k < basis.size(); k++) deltas(basis[k], A.din[k], A.dout[k], A.dnew[k], A.dsel[k]);
-    std::vector<int> free;
+    free.clear();
     for (int v = 2; v < nvar; v++)
         if (!is_piv[v]) free.push_back(v);
     const int nfree = (int)free.size();
