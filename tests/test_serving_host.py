@@ -106,3 +106,12 @@ def test_wait_all_reports_only_progress():
     assert msgs and all("answered" in m for m in msgs)
     counts = [int(m.split()[0]) for m in msgs]
     assert counts == sorted(set(counts))          # strictly increasing: no report without new answers
+
+
+def test_claim_count_shares_long_queues_only():
+    assert serve.claim_count(10, 2, 1024) == 10            # short queue: taken whole
+    assert serve.claim_count(540, 2, 1024) == 180          # long queue: a third with two idle lanes
+    assert serve.claim_count(540, 0, 1024) == 540          # no idle lane: all of it
+    assert serve.claim_count(5000, 0, 1024) == 1024        # never more than max_batch
+    assert serve.claim_count(64, 1, 1024, share_above=64) == 32
+    assert serve.claim_count(63, 1, 1024, share_above=64) == 63

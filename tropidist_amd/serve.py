@@ -5,11 +5,11 @@
     ticket = srv.submit_sparse(problem, K, order=None)      # least weight <= K by the sparse MITM DP, or None
     ticket.wait() -> record;  srv.close()
 
-Dense requests wait in a FIFO queue.  Each dense lane is a thread with its own HIP stream: when free, it takes
-every waiting request whose plans fit in the free device budget (in arrival order, up to max_batch), runs them as
-one batched launch (batch.certify_planned: one persistent kernel for the values, one for the witnesses) and
-completes them; requests that arrive meanwhile are admitted by the next free lane, so batches of different lanes
-overlap on the device.  Host planning of a request happens on its lane before admission.
+Dense requests wait in a FIFO queue.  Each dense lane is a thread with its own HIP stream: when free, it takes the
+waiting requests (a long queue is shared with the other idle lanes, see claim_count; up to max_batch), plans them,
+admits in arrival order those whose plans fit in the free device budget, runs them as one batched launch
+(batch.certify_planned: one persistent kernel for the values, one for the witnesses) and completes them; requests
+that arrive meanwhile are admitted by the next free lane, so batches of different lanes overlap on the device.
 
 Sparse requests go to one engine thread with its own stream and a resident sparse_batch.Engine, whose arena
 (sparse_arena_bytes, held for the server's life) is set apart from the dense budget.  Between any two engine steps
@@ -17,9 +17,9 @@ the thread admits every waiting sparse request (every observable a query of the 
 queries have finished, so sparse requests that arrive while others run join them on the device, one column of all of
 them per step.  Requests are admitted while the arena is less than `sparse_admit` full and no query found it full
 at the last step (one at least when nothing runs).  A request one of whose queries finds the arena full stops its
-other queries and goes back to the head of the queue while others run.  A request that does not
-fit alone, or has more queries than the engine has slots, goes to a lane of its own with its own stream, where the
-one-at-a-time path (sparse_certify, checkpointed sweeps) answers it while the engine keeps stepping the others.
+other queries and goes back to the head of the queue while others run.  A request that does not fit alone, or has
+more queries than the engine has slots, goes to a lane of its own with its own stream, where the one-at-a-time path
+(sparse_certify, checkpointed sweeps) answers it while the engine keeps stepping the others.
 Every sparse request deepens through sparse_batch.threshold_ladder(K) and every table is held to
 `sparse_max_states` states on both paths; a request past that budget is answered with sparse_dp.BudgetExceeded.
 
@@ -174,12 +174,8 @@ class Server:
                 self.idle_lanes -= 1
                 if not self.dense_q and self.closed:
                     return
-                # a long queue is shared with the lanes that are idle too, so that they plan and run in parallel;
-                # a short one (trickling arrivals) is taken whole
-                n = len(self.dense_q)
-                if n >= self.share_above:
-                    n = -(-n // (1 + self.idle_lanes))
-                claimed = [self.dense_q.popleft() for _ in range(min(self.max_batch, n))]
+                n = claim_count(len(self.dense_q), self.idle_lanes, self.max_batch, self.share_above)
+                claimed = [self.dense_q.popleft() for _ in range(n)]
             for tk in claimed:                    # host planning outside the lock
                 self._plan(tk)
             while claimed:
@@ -339,6 +335,16 @@ class Server:
     # ------------------------------------------------------------------ statistics
     def latencies(self, kind=None):
         return [tk.latency for tk in self.done if tk.latency is not None and (kind is None or tk.kind == kind)]
+
+
+def claim_count(waiting, idle_lanes, max_batch, share_above=64):
+    """Requests a waking dense lane takes from a queue of `waiting`: a long queue (share_above or more) is shared
+    with the lanes that are idle too, so that they plan and run in parallel; a short one (trickling arrivals) is
+    taken whole; never more than max_batch."""
+    n = waiting
+    if n >= share_above:
+        n = -(-n // (1 + idle_lanes))
+    return min(max_batch, n)
 
 
 def sparse_certify(p, K, order=None, max_states=30_000_000, deepen=False):
