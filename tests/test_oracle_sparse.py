@@ -145,3 +145,68 @@ def test_sparse_8bit_boundary():
     tr = sparse_dp.Trellis.for_query(problem.chain(510), 0, list(range(510)))
     with pytest.raises(ValueError):
         sparse_dp.query(tr, 510, mode="mitm", backend=oracle.OracleTables(tr), use_h=False, witness=False)
+
+
+class _OracleEngine:
+    """The engine's query interface over the oracle's CPU tables: query `rid` is sparse_dp.query at the threshold
+    it was added with (the words argument carries the Trellis)."""
+
+    def __init__(self):
+        self.q, self.released = {}, set()
+        self.max_states = None
+
+    def add_words(self, tr, tm, N, K):
+        rid = len(self.q)
+        self.q[rid] = sparse_dp.query(tr, K, mode="mitm", use_h=False, witness=True, backend=oracle.OracleTables(tr))
+        return rid
+
+    def result(self, rid):
+        return 1, self.q[rid]["value"], {"peak_states": self.q[rid]["peak_states"]}
+
+    def witness(self, rid, order, value):
+        return sorted(self.q[rid]["witness"])
+
+    def release(self, rid):
+        assert rid not in self.released
+        self.released.add(rid)
+
+    def cancel(self, rid):
+        pass
+
+
+def test_request_deepening_equals_brute_force():
+    """sparse_batch.Request deepens through the threshold ladder; with the oracle answering its queries, its
+    distance and observable are the least weight <= K over the observables, as brute force gives it."""
+    from tropidist_amd import sparse_batch
+    rng = random.Random(11)
+    checked = 0
+    for _ in range(60):
+        p = random_problem(rng)
+        order = list(range(p.ncols))
+        rng.shuffle(order)
+        trs = [(i, sparse_dp.Trellis.for_query(p, i, order), None) for i in range(p.nobs)]
+        trs = [t for t in trs if t[1].feasible]
+        if not trs:
+            continue
+        truth = {i: brute(p, i) for i, _, _ in trs}
+        for K in range(1, p.ncols + 1):
+            req = sparse_batch.Request(p, K, order=order)
+            req.trs = trs
+            eng = _OracleEngine()
+            while True:
+                req.admit(eng)
+                for rid in req.ids:
+                    req.collect(eng, rid)
+                req.finish(eng)
+                if not req.pending:
+                    break
+            assert req.error is None, req.error
+            within = {i: t for i, t in truth.items() if t is not None and t <= K}
+            best = min(within.values(), default=None)
+            assert req.record["distance"] == best
+            if best is not None:
+                assert req.record["observable"] == min(i for i, t in within.items() if t == best)
+                assert req.record["witness_ok"]
+            assert len(eng.released) == len(eng.q)        # every query released once
+            checked += 1
+    assert checked > 100
