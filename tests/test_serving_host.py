@@ -115,3 +115,32 @@ def test_claim_count_shares_long_queues_only():
     assert serve.claim_count(5000, 0, 1024) == 1024        # never more than max_batch
     assert serve.claim_count(64, 1, 1024, share_above=64) == 32
     assert serve.claim_count(63, 1, 1024, share_above=64) == 63
+
+
+def test_a_query_out_of_memory_cancels_the_others_and_fails_the_request():
+    p = problem.chain(6)
+    req = sparse_batch.Request(p, 4, order=list(range(p.ncols)))
+    req.trs = [(0, 0, None), (1, 1, None), (2, 2, None)]
+
+    class Eng(_Engine):
+        def __init__(self):
+            super().__init__({0: 4, 1: None, 2: 4}, list(range(p.ncols)))
+            self.cancelled = []
+
+        def result(self, rid):
+            if rid == 1:
+                return sparse_batch.NOMEM, None, {}
+            return super().result(rid)
+
+        def cancel(self, rid):
+            self.cancelled.append(rid)
+
+    eng = Eng()
+    req.admit(eng)
+    req.collect(eng, 1)                           # the out-of-memory query is polled first
+    assert eng.cancelled == [0, 2] and eng.released == [1]
+    for rid in (0, 2):
+        req.collect(eng, rid)
+    req.finish(eng)
+    assert isinstance(req.error, MemoryError) and req.record is None and not req.pending
+    assert sorted(eng.released) == [0, 1, 2]
