@@ -106,3 +106,24 @@ def test_wide_quasi_cyclic_code_beyond_64_slots():
     assert a["value"] == b["value"]
     if a["value"] is not None:
         check_witness(p, a["witness"], a["value"], 0)
+
+
+def test_sp_hist_counts_values():
+    """tdp_sp_hist (sp_hist, cdist/sparse_dp.c:238-242) on device tables, keys in both state words: the value
+    histogram equals numpy's count of the stored values."""
+    import numpy as np
+    print("case sparse hist", flush=True)
+    rng = random.Random(8)
+    tr = None
+    while tr is None or not tr.feasible:
+        p = random_problem(rng)
+        tr = sparse_dp.Trellis.for_query(p, 0, list(range(p.ncols)))
+    be = sparse_dp.HipTables(tr)
+    for n in (1, 7, 1000, 70000):
+        keys = set()
+        while len(keys) < n:
+            keys.add(rng.getrandbits(100))
+        keys = sorted(keys)
+        vals = np.array([rng.randrange(0, 255) for _ in range(n)], dtype=np.uint8)
+        tab = be.from_arrays(keys, vals)
+        assert list(be.hist(tab)) == np.bincount(vals, minlength=256).tolist()

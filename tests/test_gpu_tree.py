@@ -89,3 +89,43 @@ def test_banded_balanced_tree_equals_oracle(seed):
     got, _ = program.run(1 << nrows, max_memory_bytes=None, max_join_work=None)
     assert got == oracle.tree_run(program, 1 << nrows) == kernel.run_plan(plan)[0]
     assert prof["max_join_dimension"] >= 8
+
+
+@pytest.mark.parametrize("itemsize", [1, 2, 4, 8])
+def test_typed_join_entry_points_equal_oracle(itemsize):
+    """tdp_tree_join_u8/16/32/64 (the reference's tree_join_uX names) on random tables and generator maps, whole
+    and in Gray-counter ranges, against the oracle's join of the same item size; inf sits near the dtype's top so
+    the overflow guard (a + b <= inf) is exercised."""
+    import numpy as np
+    from tropidist_amd import _lib
+    print("case tree typed join u%d" % (8 * itemsize), flush=True)
+    t = _lib.torch()
+    L = _lib.lib()
+    dt = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[itemsize]
+    inf = int(np.iinfo(dt).max) - 1
+    rng = np.random.default_rng(40 + itemsize)
+    for trial in range(12):
+        lr, rr = int(rng.integers(1, 9)), int(rng.integers(1, 9))
+        pbits = int(rng.integers(0, min(lr, rr) + 1))
+        gens = pbits + int(rng.integers(0, 4))
+        lm = rng.integers(0, 1 << lr, size=max(1, gens), dtype=np.uint64)
+        rm = rng.integers(0, 1 << rr, size=max(1, gens), dtype=np.uint64)
+        left = rng.integers(0, inf + 1, size=1 << lr, dtype=np.uint64).astype(dt)
+        right = rng.integers(0, inf + 1, size=1 << rr, dtype=np.uint64).astype(dt)
+        if trial % 2:       # small costs, so most sums stay below inf
+            left = (left % 50).astype(dt)
+            right = (right % 50).astype(dt)
+        total = 1 << gens
+        cuts = sorted({0, total, int(rng.integers(0, total + 1)), int(rng.integers(0, total + 1))})
+        want = np.full(1 << pbits, inf, dtype=dt)
+        for a, b in zip(cuts, cuts[1:]):
+            oracle.tree_join(itemsize, left, right, want, lm, rm, pbits, gens, a, b, inf)
+        dl, dr = (t.from_numpy(x.view(np.uint8).copy()).cuda() for x in (left, right))
+        do = t.from_numpy(np.full(1 << pbits, inf, dtype=dt).view(np.uint8).copy()).cuda()
+        f = getattr(L, "tdp_tree_join_u%d" % (8 * itemsize))
+        for a, b in zip(cuts, cuts[1:]):
+            _lib.check(f(dl.data_ptr(), dr.data_ptr(), do.data_ptr(), lm.ctypes.data, rm.ctypes.data, pbits, gens,
+                         a, b, inf, _lib.stream_handle()), "tdp_tree_join_u%d" % (8 * itemsize))
+        t.cuda.synchronize()
+        got = do.cpu().numpy().view(dt)
+        assert (got == want).all(), (itemsize, trial)
