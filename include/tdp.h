@@ -288,6 +288,14 @@ int64_t tdp_tab_count(void *table);
 /* Insert the empty set and every subset of size 1..k (k <= 8) of the n columns given by host arrays zh, zl, lg
  * (gmitm.c:165-186), on the device. */
 int tdp_tab_build(void *table, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg, int k);
+/* tdp_tab_build without any device allocation, for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same inserts, same table
+ * contents, same null-stream synchronous behaviour as tdp_tab_build; the columns and the wide-table retry lists live
+ * in a static device array of each GPU (about 4.3 MiB, held until the process ends) instead of memory allocated per
+ * call.  Limits: n <= 4096 columns (TDP_EINVAL beyond), at most 65,536 deferred wide inserts per retry round
+ * (TDP_ECAPACITY beyond; tdp_tab_build allows 2^20).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (from several host threads) race and must be serialised by the caller. */
+int tdp_tab_build_noalloc(void *table, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg, int k);
 /* For m device queries: d_out[q] = least cost <= d_R[q] of an entry with the query's fingerprint and a different
  * label, -1 if none.  Batched form of tab_lookup (gmitm.c:136-163, static in the reference). */
 int tdp_tab_lookup_batch(void *table, const uint64_t *d_zh, const uint64_t *d_zl, const uint64_t *d_lg,

@@ -62,6 +62,7 @@ SIGNATURES = {
     "tdp_tab_free": (None, [vp]),
     "tdp_tab_count": (i64, [vp]),
     "tdp_tab_build": (ci, [vp, ci, vp, vp, vp, ci]),
+    "tdp_tab_build_noalloc": (ci, [vp, ci, vp, vp, vp, ci]),
     "tdp_tab_lookup_batch": (ci, [vp, vp, vp, vp, vp, u64, vp, vp]),
     "tdp_batch_run": (ci, [vp, vp, ci, vp, ci, vp]),
     "tdp_batch_item_bytes": (ci, []),

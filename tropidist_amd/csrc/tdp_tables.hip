@@ -12,6 +12,7 @@
 // size 2..k is inserted once; singletons and the empty set come from their own launch.
 #include "tdp_common.h"
 
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -242,9 +243,92 @@ __global__ void k_tab_lookup(TabView t, int wide, const uint64_t *zh, const uint
     out[q] = best;
 }
 
+__global__ void k_zero_words(unsigned long long *w, int n) {
+    if ((int)threadIdx.x < n) w[threadIdx.x] = 0;
+}
+
 static TabView view(const DevTable *t) { return TabView{t->mask, t->key, t->lgs, t->cost, t->count}; }
 
 static unsigned tab_grid(uint64_t n) { return (unsigned)((n + kTabBlock - 1) / kTabBlock); }
+
+
+// Shared body of the table builds: columns copied to d_cols (3 x max(n, 1) words), retry lists rbuf (2 x retry_cap
+// records) and their two counters rcnt, all device memory the caller provides.  Null stream, synchronous.
+static int build_table(DevTable *t, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg, int k,
+                       uint64_t *d_cols, Retry *rbuf, unsigned long long *rcnt, uint64_t retry_cap) {
+    t->k = k;
+    const size_t cols = n > 0 ? (size_t)n : 1;
+    uint64_t *dzh = d_cols, *dzl = d_cols + cols, *dlg = d_cols + 2 * cols;
+    if (n) {
+        TDP_HIP_CHECK(hipMemcpy(dzh, zh, n * 8, hipMemcpyHostToDevice));
+        TDP_HIP_CHECK(hipMemcpy(dzl, zl, n * 8, hipMemcpyHostToDevice));
+        TDP_HIP_CHECK(hipMemcpy(dlg, lg, n * 8, hipMemcpyHostToDevice));
+    }
+    k_zero_words<<<1, 64>>>(rcnt, 2);
+    TabView v = view(t);
+    int rc = TDP_OK;
+    int small = k >= 1 ? n + 1 : 1;
+    k_tab_small<<<tab_grid(small), kTabBlock>>>(v, t->wide, k >= 1 ? n : 0, dzh, dzl, dlg, rbuf, rcnt, retry_cap);
+    if (k >= 2 && n >= 2) {
+        uint64_t npairs = (uint64_t)n * (uint64_t)(n - 1) / 2;
+        k_tab_pairs<<<tab_grid(npairs), kTabBlock>>>(v, t->wide, n, k, dzh, dzl, dlg, rbuf, rcnt, retry_cap);
+    }
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        set_error("table build launch failed");
+        rc = TDP_EHIP;
+    }
+    // wide tables: inserts that met an unpublished slot, again until none is left
+    int which = 0;
+    for (int round = 0; rc == TDP_OK && round < 1000; round++) {
+        unsigned long long m = 0;
+        if (hipMemcpy(&m, rcnt + which, 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = TDP_EHIP; break; }
+        if (m == 0) break;
+        if (m > retry_cap) {
+            set_error("table build: %llu deferred inserts exceed the retry list of %llu", m,
+                      (unsigned long long)retry_cap);
+            rc = TDP_ECAPACITY;
+            break;
+        }
+        int other = which ^ 1;
+        k_zero_words<<<1, 64>>>(rcnt + other, 1);
+        k_tab_retry<<<tab_grid(m), kTabBlock>>>(v, rbuf + which * retry_cap, m, rbuf + other * retry_cap,
+                                                rcnt + other, retry_cap);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) { rc = TDP_EHIP; break; }
+        which = other;
+    }
+    if (rc == TDP_EHIP) set_error("table build failed on the device");
+    return rc;
+}
+
+// ---------------------------------------------------------------------------------------------- static scratch
+// Device memory of tdp_tab_build_noalloc: a file-scope array each device holds for the life of the process.  Host
+// code never passes the array's name to a kernel (on the host it is only a stand-in): scratch_base() looks up the
+// device address once per device, with that device current, and every use goes through that address.
+constexpr int kScratchCols = 4096;
+constexpr uint64_t kScratchRetry = 1u << 16;
+constexpr uint64_t kScratchColBytes = 3ull * kScratchCols * 8;
+constexpr uint64_t kScratchBytes = kScratchColBytes + 256 + 2 * kScratchRetry * sizeof(Retry);
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static uint8_t *g_scratch_ptr[kMaxDevices];
+
+static int scratch_base(uint8_t **out) {
+    int dev = 0;
+    TDP_HIP_CHECK(hipGetDevice(&dev));
+    TDP_REQUIRE(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d the static scratch table tracks", dev,
+                kMaxDevices);
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        TDP_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        TDP_REQUIRE(ptr != nullptr, "no device address for the static scratch array");
+        g_scratch_ptr[dev] = static_cast<uint8_t *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return TDP_OK;
+}
 
 }  // namespace tdp
 
@@ -303,57 +387,39 @@ int64_t tdp_tab_count(void *p) {
 int tdp_tab_build(void *p, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg, int k) {
     DevTable *t = static_cast<DevTable *>(p);
     TDP_REQUIRE(t && n >= 0 && k >= 0 && k <= kMaxK, "tab_build: n %d k %d (k <= %d)", n, k, kMaxK);
-    t->k = k;
     uint64_t *d = nullptr;
     size_t cols = n > 0 ? (size_t)n : 1;
     TDP_HIP_CHECK(hipMalloc(&d, 3 * cols * 8));
-    uint64_t *dzh = d, *dzl = d + cols, *dlg = d + 2 * cols;
-    if (n) {
-        TDP_HIP_CHECK(hipMemcpy(dzh, zh, n * 8, hipMemcpyHostToDevice));
-        TDP_HIP_CHECK(hipMemcpy(dzl, zl, n * 8, hipMemcpyHostToDevice));
-        TDP_HIP_CHECK(hipMemcpy(dlg, lg, n * 8, hipMemcpyHostToDevice));
-    }
     const uint64_t retry_cap = 1u << 20;
     Retry *rbuf = nullptr;
     unsigned long long *rcnt = nullptr;
-    TDP_HIP_CHECK(hipMalloc(&rbuf, 2 * retry_cap * sizeof(Retry)));
-    TDP_HIP_CHECK(hipMalloc(&rcnt, 2 * sizeof(unsigned long long)));
-    TDP_HIP_CHECK(hipMemset(rcnt, 0, 2 * sizeof(unsigned long long)));
-    TabView v = view(t);
-    int rc = TDP_OK;
-    int small = k >= 1 ? n + 1 : 1;
-    k_tab_small<<<tab_grid(small), kTabBlock>>>(v, t->wide, k >= 1 ? n : 0, dzh, dzl, dlg, rbuf, rcnt, retry_cap);
-    if (k >= 2 && n >= 2) {
-        uint64_t npairs = (uint64_t)n * (uint64_t)(n - 1) / 2;
-        k_tab_pairs<<<tab_grid(npairs), kTabBlock>>>(v, t->wide, n, k, dzh, dzl, dlg, rbuf, rcnt, retry_cap);
-    }
-    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
-        set_error("table build launch failed");
+    int rc;
+    if (hipMalloc(&rbuf, 2 * retry_cap * sizeof(Retry)) != hipSuccess ||
+        hipMalloc(&rcnt, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        set_error("device allocation of the table build's retry lists failed");
         rc = TDP_EHIP;
-    }
-    // wide tables: inserts that met an unpublished slot, again until none is left
-    int which = 0;
-    for (int round = 0; rc == TDP_OK && round < 1000; round++) {
-        unsigned long long m = 0;
-        if (hipMemcpy(&m, rcnt + which, 8, hipMemcpyDeviceToHost) != hipSuccess) { rc = TDP_EHIP; break; }
-        if (m == 0) break;
-        if (m > retry_cap) {
-            set_error("table build: %llu deferred inserts exceed the retry list", m);
-            rc = TDP_ECAPACITY;
-            break;
-        }
-        int other = which ^ 1;
-        hipMemset(rcnt + other, 0, 8);
-        k_tab_retry<<<tab_grid(m), kTabBlock>>>(v, rbuf + which * retry_cap, m, rbuf + other * retry_cap,
-                                                rcnt + other, retry_cap);
-        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) { rc = TDP_EHIP; break; }
-        which = other;
+    } else {
+        rc = build_table(t, n, zh, zl, lg, k, d, rbuf, rcnt, retry_cap);
     }
     hipFree(rbuf);
     hipFree(rcnt);
     hipFree(d);
-    if (rc == TDP_EHIP) set_error("table build failed on the device");
     return rc;
+}
+
+int tdp_tab_build_noalloc(void *p, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg, int k) {
+    DevTable *t = static_cast<DevTable *>(p);
+    TDP_REQUIRE(t && n >= 0 && k >= 0 && k <= kMaxK, "tab_build_noalloc: n %d k %d (k <= %d)", n, k, kMaxK);
+    TDP_REQUIRE(n <= kScratchCols, "tab_build_noalloc: %d columns exceed the static scratch limit of %d", n,
+                kScratchCols);
+    uint8_t *base = nullptr;
+    int rc = scratch_base(&base);
+    if (rc != TDP_OK) return rc;
+    // layout of the scratch array: columns (zh, zl, lg), the retry counters, the two retry lists
+    uint64_t *cols = reinterpret_cast<uint64_t *>(base);
+    unsigned long long *rcnt = reinterpret_cast<unsigned long long *>(base + kScratchColBytes);
+    Retry *rbuf = reinterpret_cast<Retry *>(base + kScratchColBytes + 256);
+    return build_table(t, n, zh, zl, lg, k, cols, rbuf, rcnt, kScratchRetry);
 }
 
 int tdp_tab_lookup_batch(void *p, const uint64_t *d_zh, const uint64_t *d_zl, const uint64_t *d_lg, const int *d_R,
