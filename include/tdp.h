@@ -182,6 +182,37 @@ int tdp_tree_move(void *d_dst, const void *d_src, size_t bytes, void *stream);
 int tdp_tree_leaf(int itemsize, void *d_out, uint64_t size, uint64_t v0, uint64_t v1, uint64_t fill, void *stream);
 int tdp_tree_fill(int itemsize, void *d_out, uint64_t size, uint64_t value, void *stream);
 
+/* Compiler and runtime of a whole tree realisation (the compilation and run of cdist/tree_dp.py:245-505; the reference
+ * does both in Python around its C joins).  cols: ncols columns of rw uint64 words; nodes in postorder: node_leaf[i]
+ * = column index or -1, node_left[i] / node_right[i] = child node indices (earlier nodes); every column a leaf once.
+ * The compiler finds each node's interface im(M_S) /\ im(M_rest), each join's generator maps and the child order
+ * of least peak live entries, and checks J = rank(M_A) + rank(M_B) + rank(M_rest) - rank(M) at every join.
+ * Returns a handle, NULL on failure with *status TDP_EINVAL (malformed tree, broken identity) or TDP_EDEADLINE
+ * (max_seconds >= 0 exceeded between nodes). */
+void *tdp_tree_compile(const uint64_t *cols, int ncols, int rw, const int32_t *node_leaf, const int32_t *node_left,
+                       const int32_t *node_right, int nnodes, double max_seconds, int *status);
+void tdp_tree_free(void *program);
+/* out[0] rank(M), [1] peak live table entries of the execution order, [2] host bytes of the program, [3] largest
+ * interface rank, [4] largest join dimension J, [5] 1 if every join's maps fit 64-bit words (ranks <= 63, J <= 62). */
+int tdp_tree_info(void *program, int64_t *out);
+/* Per node: interface rank, the execution order (node ids), the child joined first (-1 at leaves) and four join
+ * numbers (left rank, right rank, parent rank, J; -1 at leaves). */
+int tdp_tree_nodes(void *program, int32_t *rank, int32_t *order, int32_t *first, int32_t *joins);
+/* A join's J generator maps (parent generators first), as tdp_tree_join takes them. */
+int tdp_tree_maps(void *program, int node, uint64_t *lm, uint64_t *rm);
+/* A node's interface basis: rank vectors of rw words. */
+int tdp_tree_interface(void *program, int node, uint64_t *vecs);
+/* A particular solution e0 (ceil(ncols / 64) words) of M e0 = target (rw words); *feasible = 0 when the target is
+ * outside the column span (the run's None). */
+int tdp_tree_solve(void *program, const uint64_t *target, uint64_t *e0, int *feasible);
+/* The DP for the target whose particular solution is e0, on the caller's device arena of at least the peak live
+ * entries of itemsize bytes (the smallest unsigned width holding ncols + 1; infinity = ncols + 1): leaf tables, the
+ * joins in Gray-counter chunks of chunk_pairs, the stack compaction, all on `stream`.  *value = the root entry;
+ * stats[0] = join pairs enumerated, stats[1] = the largest live entries seen.  TDP_EDEADLINE when max_seconds >= 0
+ * passes before the last launch or before the result is read. */
+int tdp_tree_run(void *program, const uint64_t *e0, int itemsize, void *d_arena, uint64_t arena_items,
+                 uint64_t chunk_pairs, double max_seconds, uint64_t *value, uint64_t *stats, void *stream);
+
 /* ------------------------------------------------------------------ sparse (weight-truncated) trellis DP
  * A device table is n states of 16 bytes {lo, hv}: key = (hv & (2^56 - 1)) << 64 | lo (up to 120 row slots),
  * value = hv >> 56 (the reference's packed key | value << 56 widened from 56 to 120 key bits,
@@ -220,6 +251,59 @@ int tdp_sp_join(const void *d_f, uint64_t nf, const void *d_bents, const uint32_
 
 /* d_hist[v] = number of states of value v (256 device counters).  Replaces sp_hist (cdist/sparse_dp.c:238-242). */
 int tdp_sp_hist(const void *d_ents, uint64_t n, uint64_t *d_hist, void *stream);
+
+/* ------------------------------------------------------------------ one sparse query, natively
+ * A whole query of cdist/sparse_dp.py:985-1095 (mode TDP_SQ_MITM: backward sweep, forward frontier with joins at
+ * every cut, traceback) or one forward sweep (TDP_SQ_FORWARD, weights <= K, the value of state 0 at cut N), run by
+ * the library on `stream` with its own device tables (stream-ordered allocations, all freed before it returns).
+ * The column words are those of tdp_spb_add: prm per position 10 words (column slots, forward closing mask and
+ * bits, backward closing mask and bits, each (low 64, high 56 bits)), tm per cut 2 words.  split: the forward
+ * share a(c) per cut (N + 1 values, non-increasing, 1..K); the backward sweep keeps weights <= K - a(c).  With
+ * use_h, the lookahead bound of each direction ([0] forward, [1] backward) per cut c: bnd_ncls[d][c] classes from
+ * index bnd_off[d][c] of bnd_mask[d] (2 words each) and bnd_w[d], and bnd_extra[d][c] (tdp_sp_bounds' layout).
+ * Tables are kept at every cut while their bytes stay within keep_bytes, otherwise at checkpoints every `every`
+ * cuts, and recomputed from the kept table before them when a join or the traceback needs them.  lower: a join
+ * total <= max(lower, 0) ends the forward sweep.  Profile modes sweep one direction keeping weights <= K (no
+ * lookahead) and write hist[c * (hist_top + 1) + w] = the states of value w at cut c.  Outputs: per-cut table
+ * counts of each direction and changed counts of the frontier (N + 1 each, may be NULL), the witness's positions
+ * (room for K + 1), and the result record; a state budget or deadline stops the query with that status.  K <= 254 for
+ * one sweep, K <= 508 for the meet in the middle (a(c) <= 254 and K - a(c) <= 254).  The
+ * reference's Sweep/query run this in Python around one C call per column (cdist/sparse_dp.py:621-1095). */
+#define TDP_SQ_FORWARD 0
+#define TDP_SQ_MITM 1
+#define TDP_SQ_PROFILE_FW 2
+#define TDP_SQ_PROFILE_BW 3
+#define TDP_SQ_DONE 0
+#define TDP_SQ_STATES 1       /* a table or changed list exceeded max_states at fail_cut (BudgetExceeded "states") */
+#define TDP_SQ_DEADLINE 2     /* max_seconds passed at fail_cut (BudgetExceeded "deadline") */
+typedef struct {
+    int32_t N, K, mode, use_h, witness, lower, every, hist_top;
+    uint64_t max_states;      /* 0: no limit */
+    double max_seconds;       /* < 0: no limit */
+    uint64_t keep_bytes;
+    const uint64_t *prm, *tm;
+    const int32_t *split;
+    const int32_t *bnd_ncls[2];
+    const int64_t *bnd_off[2];
+    const uint64_t *bnd_mask[2];
+    const int64_t *bnd_w[2];
+    const int64_t *bnd_extra[2];
+    uint64_t *hist;
+} tdp_sq_spec_t;
+typedef struct {
+    int32_t status, fail_cut, value, join_cut, nwitness, pad;
+    uint64_t steps, joined, peak_states, peak_bytes;
+    double seconds, backward_seconds, forward_seconds, traceback_seconds;
+} tdp_sq_result_t;
+int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_counts, uint64_t *ch_counts,
+               int32_t *witness, tdp_sq_result_t *result, void *stream);
+
+/* The lookahead data of one query's trellis (the bound of cdist/sparse_dp.py:148-192), per cut c: ncls[c] classes
+ * at masks + 2 * (c * cap + k) (slot masks, low and high word) with weights + c * cap + k (ascending), and extra[c]
+ * (the target row's weight while it lies wholly on the remaining side).  forward selects the direction; targets as
+ * tdp_sp_trellis for one query; cap >= the trellis' slot count. */
+int tdp_sp_bounds(const uint64_t *h_words, int ncols, int nrows, const int32_t *order, const uint8_t *targets,
+                  int forward, int cap, int32_t *ncls, int64_t *extra, uint64_t *masks, int64_t *weights);
 
 /* ------------------------------------------------------------------ many sparse queries at once
  * A handle holding many meet-in-the-middle queries resident on one device (the query of

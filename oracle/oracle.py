@@ -443,3 +443,90 @@ def brute_tab_lookup(zh, zl, lg, k, qzh, qzl, qlg, R, wide):
             if ok and a == int(qzh):
                 return size
     return best
+
+
+# ---------------------------------------------------------------------------------------------- sparse trellis
+class SparseTrellis:
+    """The sparse trellis of one query, restated from cdist/sparse_dp.py:63-145 for checking the native builder:
+    rows = the check rows over positions in minimal span form plus the reduced target row (last, target 1); every
+    row gets a slot by colouring the closed intervals [start, end] in order of start, reusing the lowest slot whose
+    row ended before the start; per position p: col[p] = slots of the rows containing p, fsel_mask[p] = slots of rows
+    ending at p (fsel_bits: the target row's), bsel_mask[p] / bsel_bits[p] the same for rows starting at p; per cut
+    c: active rows are those with start < c <= end, tmask[c] = the target row's slot while it is active.  Raises
+    ValueError beyond max_slots.  The span-form routines are the dense planner's (pinned by the recorded plans)."""
+
+    def __init__(self, h, lrow, order, nrows, max_slots=120):
+        import heapq
+        from tropidist_amd import trellis as dense
+        self.order, self.N = list(order), len(order)
+        hrows = dense.minimal_span_form(dense.rows_from_columns(h, self.order, nrows))
+        target = dense.reduce_target_row(lrow, hrows)
+        self.feasible = target != 0
+        if not self.feasible:
+            return
+        rows = list(hrows) + [target]
+        lo = [(r & -r).bit_length() - 1 for r in rows]
+        hi = [r.bit_length() - 1 for r in rows]
+        slot, pool, ending, used = [None] * len(rows), [], [], 0
+        for r in sorted(range(len(rows)), key=lambda x: (lo[x], x)):
+            while ending and ending[0][0] < lo[r]:
+                heapq.heappush(pool, heapq.heappop(ending)[1])
+            if pool:
+                slot[r] = heapq.heappop(pool)
+            else:
+                slot[r], used = used, used + 1
+            heapq.heappush(ending, (hi[r], slot[r]))
+        if used > max_slots:
+            raise ValueError("trellis needs %d bits per state (at most %d supported)" % (used, max_slots))
+        self.nslots, self.rows, self.slot, self.lo, self.hi = used, rows, slot, lo, hi
+        N, last = self.N, len(rows) - 1
+        self.col = [0] * N
+        self.fsel_mask, self.fsel_bits = [0] * N, [0] * N
+        self.bsel_mask, self.bsel_bits = [0] * N, [0] * N
+        for r, row in enumerate(rows):
+            bit = 1 << slot[r]
+            for p in range(lo[r], hi[r] + 1):
+                if (row >> p) & 1:
+                    self.col[p] |= bit
+            self.fsel_mask[hi[r]] |= bit
+            self.bsel_mask[lo[r]] |= bit
+            if r == last:
+                self.fsel_bits[hi[r]] |= bit
+                self.bsel_bits[lo[r]] |= bit
+        self.active_rows = [[r for r in range(len(rows)) if lo[r] < c <= hi[r]] for c in range(N + 1)]
+        self.width = [len(a) for a in self.active_rows]
+        self.tmask = [(1 << slot[last]) if lo[last] < c <= hi[last] else 0 for c in range(N + 1)]
+
+    def bound_data(self, forward, Q=720720):
+        """The lookahead bound per cut (cdist/sparse_dp.py:148-192): (slot masks by weight ascending, weights,
+        extra), computed position by position with Python ints."""
+        rows, last, out = self.rows, len(self.rows) - 1, []
+        for c in range(self.N + 1):
+            rel = list(self.active_rows[c])
+            lone = (forward and self.lo[last] >= c) or (not forward and self.hi[last] < c)
+            if lone:
+                rel.append(last)
+            if not rel:
+                out.append(((), (), 0))
+                continue
+            window = range(c, max(self.hi[r] for r in rel) + 1) if forward else range(min(self.lo[r] for r in rel), c)
+            cover = {p: sum((rows[r] >> p) & 1 for r in rel) for p in window}
+            by_weight, extra = {}, 0
+            for r in rel:
+                most = max(cover[p] for p in window if (rows[r] >> p) & 1)
+                w = Q // most
+                if lone and r == last:
+                    extra = w
+                else:
+                    by_weight[w] = by_weight.get(w, 0) | (1 << self.slot[r])
+            ws = sorted(by_weight)
+            out.append((tuple(by_weight[w] for w in ws), tuple(ws), extra))
+        return out
+
+
+def bound_value(state, tmask, hd, Q=720720):
+    """The lookahead lower bound of one state at a cut: ceil((extra + sum_k w_k |(s ^ tmask) & mask_k|) / Q)."""
+    masks, weights, extra = hd
+    u = int(state) ^ int(tmask)
+    tot = int(extra) + sum(int(w) * bin(u & int(m)).count("1") for m, w in zip(masks, weights))
+    return -(-tot // Q)

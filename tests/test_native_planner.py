@@ -87,11 +87,17 @@ def test_greedy_order_native_equals_python():
             ordering.greedy_straddle_order(p.h, p.nrows, start=start)
 
 
-def test_sparse_trellis_words_native_equal_python():
-    """tdp_sp_trellis gives, per observable, the feasibility, slot count and engine words of sparse_dp.Trellis
-    (random problems in random orders and QC codes in the greedy order)."""
+def _target_row(p, q, order):
+    return sum(((p.l[j] >> q) & 1) << pos for pos, j in enumerate(order))
+
+
+def test_sparse_trellis_native_equals_oracle_restatement():
+    """tdp_sp_trellis (sparse_dp.Trellis and the engine's query_words) gives, per observable, the feasibility, slot
+    count, engine words, active rows and widths of the oracle's restatement of the reference's trellis (random
+    problems in random orders and QC codes in the greedy order), and the same slot-limit refusal."""
     import numpy as np
 
+    import oracle
     from tropidist_amd import ordering, sparse_batch, sparse_dp
     rng, probs = _sparse_cases()
     checked = 0
@@ -101,29 +107,60 @@ def test_sparse_trellis_words_native_equal_python():
             order = ordering.greedy_straddle_order(p.h, p.nrows)
         else:
             rng.shuffle(order)
-        want = []
-        too_wide = None
+        want, too_wide = [], None
         for q in range(p.nobs):
             try:
-                tr = sparse_dp.Trellis.for_query(p, q, order)
+                ref = oracle.SparseTrellis(p.h, _target_row(p, q, order), order, p.nrows)
             except ValueError as exc:
                 too_wide = str(exc)
                 break
-            if tr.feasible:
-                want.append((q,) + sparse_batch.trellis_words(tr))
+            if ref.feasible:
+                want.append((q, ref) + sparse_batch.trellis_words(ref))
         if too_wide is not None:
-            try:
-                sparse_batch.query_words(p, order)
-            except ValueError as exc:
-                assert str(exc) == too_wide
-                continue
-            raise AssertionError("native planner accepted a trellis past the slot limit")
+            for call in (lambda: sparse_batch.query_words(p, order),
+                         lambda: [sparse_dp.Trellis.for_query(p, q, order) for q in range(p.nobs)]):
+                try:
+                    call()
+                except ValueError as exc:
+                    assert str(exc) == too_wide
+                    continue
+                raise AssertionError("native planner accepted a trellis past the slot limit")
+            continue
         got = sparse_batch.query_words(p, order)
         assert [g[0] for g in got] == [w[0] for w in want]
-        for (q, prm, tm), (_, wprm, wtm) in zip(got, want):
+        for (q, prm, tm), (_, ref, wprm, wtm) in zip(got, want):
             assert np.array_equal(prm[:p.ncols], wprm[:p.ncols]) and np.array_equal(tm, wtm), q
+            tr = sparse_dp.Trellis.for_query(p, q, order)
+            assert tr.nslots == ref.nslots and tr.width == ref.width and tr.tmask == ref.tmask
+            assert tr.active == [sum(1 << ref.slot[r] for r in rows) for rows in ref.active_rows]
             checked += 1
     assert checked > 200
+
+
+def test_lookahead_bounds_native_equal_oracle_restatement():
+    """tdp_sp_bounds gives the oracle restatement's lookahead classes and extras at every cut, both directions."""
+    import oracle
+    from tropidist_amd import ordering, sparse_dp
+    rng, probs = _sparse_cases()
+    checked = 0
+    for p in probs[:120] + probs[-6:]:
+        order = list(range(p.ncols))
+        if p.ncols > 100:
+            order = ordering.greedy_straddle_order(p.h, p.nrows)
+        else:
+            rng.shuffle(order)
+        for q in range(p.nobs):
+            try:
+                ref = oracle.SparseTrellis(p.h, _target_row(p, q, order), order, p.nrows)
+            except ValueError:
+                continue
+            if not ref.feasible:
+                continue
+            tr = sparse_dp.Trellis.for_query(p, q, order)
+            for fwd in (True, False):
+                assert tr.bound_data(fwd) == ref.bound_data(fwd), (q, fwd)
+            checked += 1
+    assert checked > 100
 
 
 def test_planning_order_is_best_orders_choice():

@@ -51,6 +51,14 @@ SIGNATURES = {
     "tdp_tree_move": (ci, [vp, vp, ctypes.c_size_t, vp]),
     "tdp_tree_leaf": (ci, [ci, vp, u64, u64, u64, u64, vp]),
     "tdp_tree_fill": (ci, [ci, vp, u64, u64, vp]),
+    "tdp_tree_compile": (vp, [vp, ci, ci, vp, vp, vp, ci, ctypes.c_double, ctypes.POINTER(ci)]),
+    "tdp_tree_free": (None, [vp]),
+    "tdp_tree_info": (ci, [vp, vp]),
+    "tdp_tree_nodes": (ci, [vp, vp, vp, vp, vp]),
+    "tdp_tree_maps": (ci, [vp, ci, vp, vp]),
+    "tdp_tree_interface": (ci, [vp, ci, vp]),
+    "tdp_tree_solve": (ci, [vp, vp, vp, ctypes.POINTER(ci)]),
+    "tdp_tree_run": (ci, [vp, vp, ci, vp, u64, u64, ctypes.c_double, ctypes.POINTER(u64), vp, vp]),
     "tdp_sp_index": (ci, [vp, u64, vp, u64, vp]),
     "tdp_sp_step": (ci, [vp, u64, vp, u64, vp, vp, vp, ci, ci, ci, vp, vp, i64, vp, ci, ci, vp, u64, vp, u64, vp,
                          vp]),
@@ -69,6 +77,8 @@ SIGNATURES = {
     "tdp_plan_build": (ci, [vp, ci, ci, vp, vp, ci, vp, vp, i64, vp, vp, vp, vp, vp]),
     "tdp_greedy_order": (ci, [vp, ci, ci, ci, vp]),
     "tdp_sp_trellis": (ci, [vp, ci, ci, vp, vp, ci, ci, vp, vp, vp, vp]),
+    "tdp_sq_run": (ci, [vp, vp, vp, vp, vp, vp, vp]),
+    "tdp_sp_bounds": (ci, [vp, ci, ci, vp, vp, ci, ci, vp, vp, vp, vp]),
     "tdp_spb_create": (vp, [u64, ci, vp]),
     "tdp_spb_free": (ci, [vp]),
     "tdp_spb_add": (ci, [vp, vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, u64, ctypes.POINTER(i64)]),

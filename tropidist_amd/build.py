@@ -29,21 +29,40 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
-def build(verbose=False):
-    """Compile every csrc/*.hip into libtdp.so (always recompiles).  Returns the library path."""
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB + ".tmp%d" % os.getpid()
-    cmd = [hipcc] + HIPCC_FLAGS + ["-I" + os.path.join(HERE, "..", "include")] + sources() + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        try:
-            os.unlink(tmp)
-        except OSError:
-            pass
         raise RuntimeError("hipcc failed:\n" + r.stderr[-8000:])
-    os.replace(tmp, LIB)
+
+
+def build(verbose=False, jobs=None):
+    """Compile every csrc/*.hip and *.cpp (always recompiles), one object per source in parallel, and link them
+    into libtdp.so.  Returns the library path."""
+    import shutil
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    inc = "-I" + os.path.join(HERE, "..", "include")
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+    tmpdir = tempfile.mkdtemp(prefix="tdp_build_")
+    try:
+        objs = [os.path.join(tmpdir, os.path.basename(src) + ".o") for src in sources()]
+        jobs = jobs or min(8, os.cpu_count() or 4)
+        with ThreadPoolExecutor(jobs) as pool:
+            list(pool.map(lambda so: _run([hipcc] + compile_flags + [inc, "-c", so[0], "-o", so[1]], verbose),
+                          zip(sources(), objs)))
+        tmp = LIB + ".tmp%d" % os.getpid()
+        try:
+            _run([hipcc, "-shared", "-fPIC", "--offload-arch=" + ARCH] + objs + ["-o", tmp], verbose)
+        except Exception:
+            if os.path.exists(tmp):
+                os.unlink(tmp)
+            raise
+        os.replace(tmp, LIB)
+    finally:
+        shutil.rmtree(tmpdir, ignore_errors=True)
     return LIB
 
 

@@ -360,6 +360,59 @@ int tdp_rank_profile(const uint64_t *h_words, int ncols, int rw, const int32_t *
 
 }  // extern "C"
 
+namespace {
+
+// A query's rows with their spans and slots: the rows in minimal span form and the reduced target row (last), slots
+// by an interval colouring of the closed spans (rows taken by start; a slot frees once its row has ended; the lowest
+// free slot first).  Returns false when the target reduces to zero.
+struct SlotRows {
+    Row target;
+    std::vector<const Row *> all;
+    std::vector<int> st, en, slot;
+    int ns = 0;
+};
+
+bool slot_rows(const std::vector<Row> &hrows, Row lrow_in, int ncols, SlotRows &t) {
+    t.target = reduce_target_row(lrow_in, hrows, ncols);
+    t.ns = 0;
+    if (t.target.zero()) return false;
+    t.all.clear();
+    for (const Row &r : hrows) t.all.push_back(&r);
+    t.all.push_back(&t.target);
+    const int R = (int)t.all.size();
+    t.st.assign(R, 0);
+    t.en.assign(R, 0);
+    t.slot.assign(R, 0);
+    std::vector<int> idx(R);
+    for (int r = 0; r < R; r++) {
+        t.st[r] = t.all[r]->lowbit();
+        t.en[r] = t.all[r]->bitlen() - 1;
+        idx[r] = r;
+    }
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return t.st[a] < t.st[b]; });
+    std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, std::greater<std::pair<int, int>>>
+        busy;
+    std::priority_queue<int, std::vector<int>, std::greater<int>> freeq;
+    for (int r : idx) {
+        while (!busy.empty() && busy.top().first < t.st[r]) {
+            freeq.push(busy.top().second);
+            busy.pop();
+        }
+        int sl;
+        if (!freeq.empty()) {
+            sl = freeq.top();
+            freeq.pop();
+        } else {
+            sl = t.ns++;
+        }
+        t.slot[r] = sl;
+        busy.push({t.en[r], sl});
+    }
+    return true;
+}
+
+}  // namespace
+
 extern "C" {
 
 // The greedy straddle order of tropidist_amd/ordering.py: grow the processed set one column at a time, taking from
@@ -449,63 +502,113 @@ int tdp_sp_trellis(const uint64_t *h_words, int ncols, int nrows, const int32_t 
     std::vector<Row> hrows = minimal_span_form(std::move(rows), ncols);
     const int N = ncols;
     for (int q = 0; q < nq; q++) {
-        Row lrow = reduce_target_row(target_row(targets + (int64_t)q * ncols, order, ncols), hrows, ncols);
-        nslots[q] = 0;
-        if (lrow.zero()) {
-            feasible[q] = 0;
-            continue;
-        }
-        feasible[q] = 1;
-        std::vector<const Row *> all;
-        for (const Row &r : hrows) all.push_back(&r);
-        all.push_back(&lrow);
-        const int R = (int)all.size(), L = R - 1;
-        std::vector<int> st(R), en(R), slot(R), idx(R);
-        for (int r = 0; r < R; r++) {
-            st[r] = all[r]->lowbit();
-            en[r] = all[r]->bitlen() - 1;
-            idx[r] = r;
-        }
-        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return st[a] < st[b]; });
-        std::priority_queue<std::pair<int, int>, std::vector<std::pair<int, int>>, std::greater<std::pair<int, int>>>
-            busy;
-        std::priority_queue<int, std::vector<int>, std::greater<int>> freeq;
-        int ns = 0;
-        for (int r : idx) {
-            while (!busy.empty() && busy.top().first < st[r]) {
-                freeq.push(busy.top().second);
-                busy.pop();
-            }
-            int sl;
-            if (!freeq.empty()) {
-                sl = freeq.top();
-                freeq.pop();
-            } else {
-                sl = ns++;
-            }
-            slot[r] = sl;
-            busy.push({en[r], sl});
-        }
-        nslots[q] = ns;
-        if (ns > max_slots) continue;
+        SlotRows t;
+        feasible[q] = slot_rows(hrows, target_row(targets + (int64_t)q * ncols, order, ncols), ncols, t) ? 1 : 0;
+        nslots[q] = t.ns;
+        if (!feasible[q] || t.ns > max_slots) continue;
+        const int R = (int)t.all.size(), L = R - 1;
         uint64_t *P = prm + (int64_t)q * N * 10;
         uint64_t *T = tm + (int64_t)q * (N + 1) * 2;
         memset(P, 0, sizeof(uint64_t) * 10 * (size_t)N);
         memset(T, 0, sizeof(uint64_t) * 2 * (size_t)(N + 1));
         for (int r = 0; r < R; r++) {
-            const int w = slot[r] >> 6;                 // word of the slot bit: 0 = low, 1 = high
-            const uint64_t b = 1ull << (slot[r] & 63);
-            const Row &row = *all[r];
+            const int w = t.slot[r] >> 6;               // word of the slot bit: 0 = low, 1 = high
+            const uint64_t b = 1ull << (t.slot[r] & 63);
+            const Row &row = *t.all[r];
             for (int k = row.lo; k < row.hi; k++)
                 for (uint64_t x = row.w[k]; x; x &= x - 1) P[(int64_t)(k * 64 + __builtin_ctzll(x)) * 10 + w] |= b;
-            P[(int64_t)en[r] * 10 + 2 + w] |= b;       // forward closing mask
-            P[(int64_t)st[r] * 10 + 6 + w] |= b;       // backward closing mask
+            P[(int64_t)t.en[r] * 10 + 2 + w] |= b;       // forward closing mask
+            P[(int64_t)t.st[r] * 10 + 6 + w] |= b;       // backward closing mask
             if (r == L) {
-                P[(int64_t)en[r] * 10 + 4 + w] |= b;   // forward closing bits (the target row must be 1)
-                P[(int64_t)st[r] * 10 + 8 + w] |= b;   // backward closing bits
-                for (int c = st[r] + 1; c <= en[r]; c++) T[2 * c + w] |= b;
+                P[(int64_t)t.en[r] * 10 + 4 + w] |= b;   // forward closing bits (the target row must be 1)
+                P[(int64_t)t.st[r] * 10 + 8 + w] |= b;   // backward closing bits
+                for (int c = t.st[r] + 1; c <= t.en[r]; c++) T[2 * c + w] |= b;
             }
         }
+    }
+    return TDP_OK;
+}
+
+// The lookahead lower bound of one query's sparse sweep, per cut (cdist/sparse_dp.py:148-192 defines it): the rows
+// that still need an odd number of flips from the remaining columns -- the active rows, and the target row while it
+// lies wholly on the remaining side -- each get the weight y_r = Q / (the most such rows any remaining column in the
+// rows' window covers together with r), rounded down to a multiple of 1/Q, Q = 720720.  Every remaining column then
+// carries total weight <= 1 (a dual solution of the covering LP), so ceil((extra + sum_r y_r [r odd]) / Q) faults
+// are still needed.  Per cut c: ncls[c] classes (rows of equal weight merged into one slot mask, ascending weight) at
+// masks + 2 * (c * cap + k) / weights + c * cap + k, and extra[c] = the target row's weight when it lies wholly
+// ahead (else 0).  cap >= the slot count.
+int tdp_sp_bounds(const uint64_t *h_words, int ncols, int nrows, const int32_t *order, const uint8_t *targets,
+                  int forward, int cap, int32_t *ncls, int64_t *extra, uint64_t *masks, int64_t *weights) {
+    if (!h_words || !order || !targets || !ncls || !extra || !masks || !weights || ncols < 0 || nrows < 0 ||
+        cap < 1) {
+        tdp::set_error("tdp_sp_bounds: bad arguments");
+        return TDP_EINVAL;
+    }
+    std::vector<Row> rows;
+    int rc = rows_over_positions(h_words, ncols, nrows, order, rows);
+    if (rc) return rc;
+    std::vector<Row> hrows = minimal_span_form(std::move(rows), ncols);
+    SlotRows t;
+    const int N = ncols;
+    if (!slot_rows(hrows, target_row(targets, order, ncols), ncols, t)) {
+        tdp::set_error("tdp_sp_bounds: the target reduces to zero (no query)");
+        return TDP_EINVAL;
+    }
+    if (t.ns > cap) {
+        tdp::set_error("tdp_sp_bounds: %d slots, room for %d classes", t.ns, cap);
+        return TDP_EINVAL;
+    }
+    const int R = (int)t.all.size(), L = R - 1;
+    const int64_t Q = 720720;
+    std::vector<int> rel, cnt(N, 0);
+    std::vector<std::pair<int64_t, int>> wr;     // (weight, slot)
+    for (int c = 0; c <= N; c++) {
+        rel.clear();
+        bool lone = false;
+        for (int r = 0; r < R; r++)
+            if (t.st[r] < c && c <= t.en[r]) rel.push_back(r);
+        if ((forward && t.st[L] >= c) || (!forward && t.en[L] < c)) {
+            rel.push_back(L);
+            lone = true;
+        }
+        ncls[c] = 0;
+        extra[c] = 0;
+        if (rel.empty()) continue;
+        int lo = N, hi = 0;                      // the window [lo, hi) of remaining positions the rows reach
+        if (forward) {
+            lo = c;
+            for (int r : rel) hi = std::max(hi, t.en[r] + 1);
+        } else {
+            hi = c;
+            for (int r : rel) lo = std::min(lo, t.st[r]);
+        }
+        for (int j = lo; j < hi; j++) cnt[j] = 0;
+        for (int r : rel)
+            for (int j = std::max(lo, t.st[r]); j < std::min(hi, t.en[r] + 1); j++) cnt[j] += t.all[r]->test(j);
+        wr.clear();
+        for (int r : rel) {
+            int most = 0;
+            for (int j = std::max(lo, t.st[r]); j < std::min(hi, t.en[r] + 1); j++)
+                if (t.all[r]->test(j)) most = std::max(most, cnt[j]);
+            if (most < 1) {
+                tdp::set_error("tdp_sp_bounds: row without a remaining column at cut %d", c);
+                return TDP_EINVAL;
+            }
+            const int64_t w = Q / most;
+            if (lone && r == L) extra[c] = w;
+            else wr.push_back({w, t.slot[r]});
+        }
+        std::sort(wr.begin(), wr.end());
+        int k = -1;
+        for (size_t i = 0; i < wr.size(); i++) {
+            if (i == 0 || wr[i].first != wr[i - 1].first) {
+                k++;
+                weights[(int64_t)c * cap + k] = wr[i].first;
+                masks[2 * ((int64_t)c * cap + k)] = masks[2 * ((int64_t)c * cap + k) + 1] = 0;
+            }
+            masks[2 * ((int64_t)c * cap + k) + (wr[i].second >> 6)] |= 1ull << (wr[i].second & 63);
+        }
+        ncls[c] = k + 1;
     }
     return TDP_OK;
 }

@@ -1,177 +1,157 @@
-"""Weight-truncated sparse Rank DP with meet in the middle, tables resident on the GPU.
+"""Weight-truncated sparse Rank DP with meet in the middle, run on the GPU.
 
-Call surface of the reference's cdist/sparse_dp.py: Trellis(h, lrow, order, nrows) / Trellis.for_query, BACKENDS,
-Sweep, query(tr, K, mode, split, ...), distance(p, ...), choose_split / choose_schedule, BudgetExceeded, summary.
+The call surface is that of the reference's cdist/sparse_dp.py: Trellis(h, lrow, order, nrows) / Trellis.for_query,
+BACKENDS, Sweep, query(tr, K, mode, split, use_h, witness, lower, every, backend, ...), distance(p, ...),
+choose_split / choose_schedule, BudgetExceeded, summary.  Its semantics (cdist/sparse_dp.py:1-30): a state of weight
+above the threshold K, or whose weight plus an admissible lower bound on the faults still needed exceeds K
+(lookahead, use_h), can never lie on a solution of weight <= K and is dropped; every state on such a solution keeps
+its exact value, so a sweep answers "the least weight if it is <= K".  The meet in the middle splits the weight:
+the backward sweep keeps weights <= K - a(c), the forward one weights <= a(c) - 1 and joins its new or improved
+states (weights <= a(c)) with the backward table at every cut.
 
-The semantics are the reference's (module docstring of cdist/sparse_dp.py:1-30):
-  truncation  a state of weight > K is dropped (its value only grows along the sweep);
-  lookahead   a state with weight + h > K is dropped, h an admissible covering bound (use_h);
-  mitm        a backward sweep keeps weights <= K - a(c), a forward sweep keeps weights <= a(c) - 1 and joins its
-              new or improved states (weights <= a(c)) with the backward table at every cut;
-  witness     a traceback through tables recomputed from checkpoints every isqrt(N) cuts.
-
-Backend "hip" (HipTables): a table is a device array of 16-byte states (keys of up to 120 row slots, the value in
-the top byte) with a device hash index; one column step is one kernel pass over the old states (tdp_sp_step), the
-join is a batched probe with a deterministic minimum (tdp_sp_join).  Tables never leave the device except for
-statistics and single-key lookups.  The dense fallback of the reference (Sweep(dense=...)) is not provided by this
-backend.
+How it runs here:
+  Trellis       built natively (tdp_sp_trellis: span form, slot colouring, closing masks) with the lookahead data
+                from tdp_sp_bounds; a state is a key of up to 120 slots (two 64-bit words on the device).
+  query         one call of libtdp's native runner (tdp_sq_run) for the "hip" backend: the whole query -- backward
+                sweep, forward frontier with joins, checkpoints, traceback -- runs on one stream with device tables,
+                one read-back per cut; per-cut counts come back at the end.  Any other backend object (the
+                reference's backend interface, e.g. a CPU table set used as a checker) runs through Sweep below.
+  Sweep         the sweep of one direction over any backend object, stepping tables one column at a time, with
+                checkpoints and a traceback through recomputed stretches.
+  split choice  probe sweeps (tdp_sq_run's profile mode) give the states of each weight at every cut; the split
+                minimises a cost model of the native runner (which keeps backward tables instead of recomputing
+                them when they fit its budget).
+There is no dense fallback: dense=... raises ValueError.
 """
 from __future__ import annotations
 
 import ctypes
-import heapq
 import math
 import time
 
 import numpy as np
 
-from . import _lib, trellis
+from . import _lib
 
 INF = 255
 MAX_WEIGHT = INF - 1
 Q = 720720
 KEYMASK = (1 << 56) - 1
-MAX_SLOTS = 120                 # 64 + 56 key bits of a device state
-U64 = np.uint64
+MAX_SLOTS = 120                  # key bits of a device state: 64 + 56
 M64 = (1 << 64) - 1
+U64 = np.uint64
+KEEP_BYTES = 48 << 30            # device bytes of tables a native query keeps in full before checkpointing only
 
 
-def _bits(x):
-    out = []
-    while x:
-        low = x & -x
-        out.append(low.bit_length() - 1)
-        x ^= low
-    return out
+def _lo_hi(x):
+    return x & M64, x >> 64
 
 
+def _join(lo, hi):
+    return int(lo) | (int(hi) << 64)
+
+
+# ---------------------------------------------------------------------------------------------- trellis
 class Trellis:
-    """Minimal-span trellis of one query min{|e| : H e = 0, (L e)_obs = 1} in a column order, with rows on fixed
-    bit positions ("slots") so that a state means the same in both sweep directions (cdist/sparse_dp.py:63-145).
-    Cut c separates positions [0, c) from [c, N)."""
+    """The minimal-span trellis of the query min{|e| : H e = 0, (L e)_obs = 1} in a column order, built by libtdp.
+
+    lrow: the target row over positions (bit p = the column at position p flips the observable).  Rows sit on fixed
+    state bits ("slots") so that a state means the same in both directions; cut c separates positions [0, c) and
+    [c, N).  Attributes (as the reference's): order, N, feasible, nslots, col[p], fsel_mask/fsel_bits[p] (rows
+    closing at p going forward and the bits they must have), bsel_mask/bsel_bits[p] (going backward), tmask[c] (the
+    target row's slot while active), active[c] (slots of the active rows), width[c], starts_at/ends_at[p]
+    ((slot, target) of the rows starting / ending at p)."""
 
     def __init__(self, h, lrow, order, nrows, name=""):
+        from .trellis import column_words
         self.name = name
-        self.order = list(order)
+        self.order = [int(x) for x in order]
         self.N = N = len(self.order)
-        hrows = trellis.minimal_span_form(trellis.rows_from_columns(h, self.order, nrows))
-        lr = trellis.reduce_target_row(lrow, hrows)
-        self.feasible = lr != 0
+        self._words = column_words(h, nrows)
+        self._nrows = nrows
+        self._targets = np.zeros(max(1, N), dtype=np.uint8)
+        for p, j in enumerate(self.order):
+            self._targets[j] = (lrow >> p) & 1
+        self._ord = np.asarray(self.order or [0], dtype=np.int32)
+        prm = np.zeros((max(N, 1), 10), dtype=np.uint64)
+        tm = np.zeros((N + 1, 2), dtype=np.uint64)
+        ns, fe = np.zeros(1, dtype=np.int32), np.zeros(1, dtype=np.int32)
+        vp = ctypes.c_void_p
+        _lib.check(_lib.lib().tdp_sp_trellis(self._words.ctypes.data_as(vp), N, nrows, self._ord.ctypes.data_as(vp),
+                                             self._targets.ctypes.data_as(vp), 1, MAX_SLOTS, prm.ctypes.data_as(vp),
+                                             tm.ctypes.data_as(vp), ns.ctypes.data_as(vp), fe.ctypes.data_as(vp)),
+                   "tdp_sp_trellis")
+        self.feasible = bool(fe[0])
         if not self.feasible:
             return
-        rows = list(hrows) + [lr]
-        self.rows = rows
-        self.target = [0] * len(hrows) + [1]
-        self.L = len(rows) - 1
-        self.start = [trellis.lowbit(r) for r in rows]
-        self.end = [r.bit_length() - 1 for r in rows]
-        # interval colouring of the closed spans [start, end]: lowest free slot first
-        slot = [0] * len(rows)
-        free, busy, nslots = [], [], 0
-        for r in sorted(range(len(rows)), key=lambda r: self.start[r]):
-            while busy and busy[0][0] < self.start[r]:
-                heapq.heappush(free, heapq.heappop(busy)[1])
-            if free:
-                s = heapq.heappop(free)
-            else:
-                s, nslots = nslots, nslots + 1
-            slot[r] = s
-            heapq.heappush(busy, (self.end[r], s))
-        if nslots > MAX_SLOTS:
-            raise ValueError("trellis needs %d bits per state (at most %d supported)" % (nslots, MAX_SLOTS))
-        self.slot, self.nslots = slot, nslots
-        col = [0] * N
-        for r, row in enumerate(rows):
-            for pos in _bits(row):
-                col[pos] |= 1 << slot[r]
-        self.col = col
-        fsm, fsb, bsm, bsb = [0] * N, [0] * N, [0] * N, [0] * N
-        starts_at, ends_at = [[] for _ in range(N)], [[] for _ in range(N)]
-        for r in range(len(rows)):
-            b = 1 << slot[r]
-            fsm[self.end[r]] |= b
-            bsm[self.start[r]] |= b
-            if self.target[r]:
-                fsb[self.end[r]] |= b
-                bsb[self.start[r]] |= b
-            starts_at[self.start[r]].append((slot[r], self.target[r]))
-            ends_at[self.end[r]].append((slot[r], self.target[r]))
-        self.fsel_mask, self.fsel_bits, self.bsel_mask, self.bsel_bits = fsm, fsb, bsm, bsb
-        self.starts_at, self.ends_at = starts_at, ends_at
-        delta = [[] for _ in range(N + 1)]
-        for r in range(len(rows)):
-            delta[self.start[r] + 1].append(r)
-            delta[self.end[r] + 1].append(~r)
-        act, cur = [], set()
-        for c in range(N + 1):
-            for x in delta[c]:
-                if x >= 0:
-                    cur.add(x)
-                else:
-                    cur.discard(~x)
-            act.append(sorted(cur))
-        self.active_rows = act
-        self.active = [sum(1 << slot[r] for r in a) for a in act]
-        self.width = [len(a) for a in act]
-        Ls = 1 << slot[self.L]
-        self.tmask = [Ls if self.L in set(a) else 0 for a in act]
-        self._bd = {}
+        self.nslots = int(ns[0])
+        if self.nslots > MAX_SLOTS:
+            raise ValueError("trellis needs %d bits per state (at most %d supported)" % (self.nslots, MAX_SLOTS))
+        self.prm, self.tm = prm, tm
+        w = [[_join(prm[p, 2 * k], prm[p, 2 * k + 1]) for p in range(N)] for k in range(5)]
+        self.col, self.fsel_mask, self.fsel_bits, self.bsel_mask, self.bsel_bits = w
+        self.tmask = [_join(a, b) for a, b in tm]
+        # a row is active at cut c when start < c <= end: it joins after its start position, leaves at its end
+        act, cur = [0] * (N + 1), 0
+        for p in range(N):
+            cur = (cur | self.bsel_mask[p]) & ~self.fsel_mask[p]
+            act[p + 1] = cur
+        self.active = act
+        self.width = [bin(a).count("1") for a in act]
+
+        def pairs(masks, bits):
+            return [[(s, (b >> s) & 1) for s in range(m.bit_length()) if (m >> s) & 1] for m, b in zip(masks, bits)]
+        self.starts_at = pairs(self.bsel_mask, self.bsel_bits)
+        self.ends_at = pairs(self.fsel_mask, self.fsel_bits)
+        self._bounds = {}
 
     @classmethod
     def for_query(cls, p, obs, order):
         lrow = 0
         for pos, j in enumerate(order):
-            if (p.l[j] >> obs) & 1:
-                lrow |= 1 << pos
+            lrow |= ((p.l[j] >> obs) & 1) << pos
         return cls(p.h, lrow, order, p.nrows, name="%s obs %d" % (getattr(p, "name", ""), obs))
 
-    def _bound_data(self, forward):
-        """Per cut: (masks, weights, extra) of h(s) = ceil((extra + sum_k w_k |(s ^ tmask) & mask_k|) / Q)
-        (cdist/sparse_dp.py:148-192): row r gets the weight 1 / max over remaining columns j containing r of
-        |supp(j) /\\ R_c|, R_c the active rows plus the target row while it lies wholly ahead, rounded down to 1/Q."""
-        N, rows = self.N, self.rows
-        S = np.zeros((len(rows), N), dtype=np.int32)
-        for r, row in enumerate(rows):
-            S[r, _bits(row)] = 1
-        out = []
-        for c in range(N + 1):
-            rel = list(self.active_rows[c])
-            lone = False
-            if forward and self.start[self.L] >= c:
-                rel.append(self.L)
-                lone = True
-            if not forward and self.end[self.L] < c:
-                rel.append(self.L)
-                lone = True
-            if not rel:
-                out.append(((), (), 0))
-                continue
-            lo, hi = (c, max(self.end[r] for r in rel) + 1) if forward else (min(self.start[r] for r in rel), c)
-            sub = S[rel, lo:hi]
-            cmax = (sub * sub.sum(axis=0)).max(axis=1)
-            assert (cmax >= 1).all()
-            w = Q // cmax
-            masks, extra = {}, 0
-            for r, wr in zip(rel, w.tolist()):
-                if lone and r == self.L:
-                    extra = wr
-                    continue
-                masks[wr] = masks.get(wr, 0) | (1 << self.slot[r])
-            ks = sorted(masks)
-            out.append((tuple(masks[k] for k in ks), tuple(int(k) for k in ks), int(extra)))
-        return out
+    def words(self):
+        """(prm, tm): the column words (N x 10) and target masks ((N + 1) x 2) of tdp_sq_run and tdp_spb_add."""
+        return self.prm, self.tm
+
+    def bound_arrays(self, forward):
+        """The lookahead data of one direction from tdp_sp_bounds: (ncls int32[N+1], offsets int64[N+1], masks
+        uint64[n, 2], weights int64[n], extra int64[N+1]), the classes of cut c at offsets[c]."""
+        key = bool(forward)
+        if key not in self._bounds:
+            N, cap = self.N, max(1, self.nslots)
+            ncls = np.zeros(N + 1, dtype=np.int32)
+            extra = np.zeros(N + 1, dtype=np.int64)
+            masks = np.zeros(((N + 1) * cap, 2), dtype=np.uint64)
+            weights = np.zeros((N + 1) * cap, dtype=np.int64)
+            vp = ctypes.c_void_p
+            _lib.check(_lib.lib().tdp_sp_bounds(self._words.ctypes.data_as(vp), N, self._nrows,
+                                                self._ord.ctypes.data_as(vp), self._targets.ctypes.data_as(vp),
+                                                int(key), cap, ncls.ctypes.data_as(vp), extra.ctypes.data_as(vp),
+                                                masks.ctypes.data_as(vp), weights.ctypes.data_as(vp)),
+                       "tdp_sp_bounds")
+            off = np.arange(N + 1, dtype=np.int64) * cap
+            self._bounds[key] = (ncls, off, masks, weights, extra)
+        return self._bounds[key]
 
     def bound_data(self, forward):
-        if forward not in self._bd:
-            self._bd[forward] = self._bound_data(forward)
-        return self._bd[forward]
+        """Per cut (masks, weights, extra) of the bound h(s) = ceil((extra + sum_k w_k |(s ^ tmask) & mask_k|) / Q)
+        (the reference's format, cdist/sparse_dp.py:148-192)."""
+        key = ("py", bool(forward))
+        if key not in self._bounds:
+            ncls, off, masks, weights, extra = self.bound_arrays(forward)
+            out = []
+            for c in range(self.N + 1):
+                k0, k = int(off[c]), int(ncls[c])
+                out.append((tuple(_join(*masks[i]) for i in range(k0, k0 + k)),
+                            tuple(int(w) for w in weights[k0:k0 + k]), int(extra[c])))
+            self._bounds[key] = out
+        return self._bounds[key]
 
 
 # ---------------------------------------------------------------------------------------------- device tables
-def _split(x):
-    return (ctypes.c_uint64 * 2)(x & M64, x >> 64)
-
-
 class DevTable:
     """n states on the device (16 bytes each) and, once built, their hash index."""
     __slots__ = ("buf", "n", "slots", "capacity")
@@ -183,8 +163,15 @@ class DevTable:
         return self.buf.data_ptr()
 
 
+def _pair_array(x):
+    return (ctypes.c_uint64 * 2)(x & M64, x >> 64)
+
+
 class HipTables:
-    """Sparse tables on the GPU (the backend interface of cdist/sparse_dp.py:300-445 / 506-605)."""
+    """Sparse tables on the GPU behind the reference's backend interface (cdist/sparse_dp.py:300-445, 506-605):
+    start, size, nbytes, arrays, from_arrays, lookup, step, changed_size, persistent, truncate, join.  step() takes
+    the lookahead data of a cut either as this backend's device form (bound()) or as the reference's host triple
+    (masks, weights, extra) of Trellis.bound_data."""
     name = "hip"
 
     def __init__(self, tr, stream=None):
@@ -194,30 +181,27 @@ class HipTables:
         self.tr = tr
         self.L = _lib.lib()
         self.stream = stream
+        self._sh = _lib.stream_handle(stream)
         self.counters = _lib.device_empty(64)
-        self._host_counters = np.zeros(8, dtype=np.uint64)
-        self._bounds = {}
-        self._sh = _lib.stream_handle(stream)       # the query's stream, fixed for the table set's life
+        self._host = np.zeros(8, dtype=np.uint64)
+        self._dev_bounds = {}
 
-    def sh(self):
-        return self._sh
+    def _read(self, k):
+        _lib.check(self.L.tdp_read_back(self._host.ctypes.data_as(ctypes.c_void_p), self.counters.data_ptr(), 8 * k,
+                                        self._sh), "tdp_read_back")
+        return self._host[:k].tolist()
 
-    def _alloc(self, nstates):
+    @staticmethod
+    def _buffer(nstates):
         return _lib.device_empty(16 * max(1, nstates))
-
-    def _sync_counters(self, k):
-        _lib.check(self.L.tdp_read_back(self._host_counters.ctypes.data_as(ctypes.c_void_p), self.counters.data_ptr(),
-                                        8 * k, self._sh), "tdp_read_back")
-        return self._host_counters[:k].tolist()
 
     def index(self, tab):
         if tab.slots is None:
             cap = 64
             while cap < 2 * tab.n:
-                cap <<= 1
-            tab.slots = _lib.device_empty(4 * cap)
-            tab.capacity = cap
-            _lib.check(self.L.tdp_sp_index(tab.ptr(), tab.n, tab.slots.data_ptr(), cap, self.sh()), "tdp_sp_index")
+                cap *= 2
+            tab.slots, tab.capacity = _lib.device_empty(4 * cap), cap
+            _lib.check(self.L.tdp_sp_index(tab.ptr(), tab.n, tab.slots.data_ptr(), cap, self._sh), "tdp_sp_index")
         return tab
 
     def start(self):
@@ -232,125 +216,120 @@ class HipTables:
         return 16 * tab.n
 
     def arrays(self, tab):
-        """(keys, vals) of a table: keys as Python ints (object array) when any exceeds 64 bits, else uint64."""
-        raw = tab.buf[:16 * tab.n].cpu().numpy().view(np.uint64).reshape(-1, 2) if tab.n else np.zeros((0, 2), U64)
+        """(keys, vals): keys as uint64 when they all fit 64 bits, else Python ints (object array)."""
+        if not tab.n:
+            return np.zeros(0, dtype=U64), np.zeros(0, dtype=np.uint8)
+        raw = tab.buf[:16 * tab.n].cpu().numpy().view(U64).reshape(-1, 2)
         vals = (raw[:, 1] >> U64(56)).astype(np.uint8)
         hi = raw[:, 1] & U64(KEYMASK)
-        if tab.n and hi.any():
-            keys = np.array([int(a) | (int(b) << 64) for a, b in zip(raw[:, 0], hi)], dtype=object)
-        else:
-            keys = raw[:, 0].copy()
-        return keys, vals
+        if hi.any():
+            return np.array([_join(a, b) for a, b in zip(raw[:, 0], hi)], dtype=object), vals
+        return raw[:, 0].copy(), vals
 
     def from_arrays(self, keys, vals):
         n = len(vals)
         raw = np.zeros((max(1, n), 2), dtype=U64)
         for i, (k, v) in enumerate(zip(keys, vals)):
-            k = int(k)
-            raw[i, 0] = k & M64
-            raw[i, 1] = (k >> 64) | (int(v) << 56)
-        buf = self._alloc(n)
-        t = _lib.torch()
-        buf.copy_(t.from_numpy(raw.view(np.uint8).reshape(-1)[:16 * max(1, n)]))
+            lo, hi = _lo_hi(int(k))
+            raw[i] = (lo, hi | (int(v) << 56))
+        buf = self._buffer(n)
+        buf.copy_(_lib.torch().from_numpy(raw.view(np.uint8).reshape(-1)))
         return DevTable(buf, n)
 
     def lookup_many(self, tab, keys):
-        """Values of the given states (INF where absent)."""
         m = len(keys)
         if not m or not tab.n:
             return np.full(m, INF, dtype=np.uint8)
         self.index(tab)
-        q = np.zeros((m, 2), dtype=U64)
-        for i, k in enumerate(keys):
-            k = int(k)
-            q[i] = (k & M64, k >> 64)
-        t = _lib.torch()
-        dq = t.from_numpy(q.view(np.uint8).reshape(-1)).cuda()
+        q = np.array([_lo_hi(int(k)) for k in keys], dtype=U64)
+        dq = _lib.torch().from_numpy(q.view(np.uint8).reshape(-1)).cuda()
         dv = _lib.device_empty(m)
         _lib.check(self.L.tdp_sp_lookup(tab.ptr(), tab.slots.data_ptr(), tab.capacity, dq.data_ptr(), m,
-                                        dv.data_ptr(), self.sh()), "tdp_sp_lookup")
+                                        dv.data_ptr(), self._sh), "tdp_sp_lookup")
         return dv[:m].cpu().numpy()
 
     def lookup(self, tab, s):
         return int(self.lookup_many(tab, [s])[0])
 
-    def bound(self, forward):
-        """Per-cut lookahead data on the device: (mask tensor, weight tensor, ncls, extra)."""
-        if forward not in self._bounds:
+    def _device_bound(self, hd):
+        """The device form (mask tensor, weight tensor, ncls, extra) of one cut's (masks, weights, extra)."""
+        key = id(hd)
+        got = self._dev_bounds.get(key)
+        if got is None or got[0] is not hd:
+            masks, weights, extra = hd
             t = _lib.torch()
-            per = []
-            for masks, weights, extra in self.tr.bound_data(forward):
-                m = np.zeros((max(1, len(masks)), 2), dtype=U64)
-                for i, x in enumerate(masks):
-                    m[i] = (x & M64, x >> 64)
-                w = np.zeros(max(1, len(weights)), dtype=np.int64)
-                w[:len(weights)] = weights
-                per.append((t.from_numpy(m.reshape(-1)).cuda(), t.from_numpy(w).cuda(), len(masks), extra))
-            self._bounds[forward] = per
-        return self._bounds[forward]
+            m = np.array([_lo_hi(int(x)) for x in masks] or [(0, 0)], dtype=U64).reshape(-1)
+            w = np.array([int(x) for x in weights] or [0], dtype=np.int64)
+            got = (hd, (t.from_numpy(m).cuda(), t.from_numpy(w).cuda(), len(masks), int(extra)))
+            self._dev_bounds[key] = got
+        return got[1]
+
+    def bound(self, forward):
+        """Per cut, the lookahead data in this backend's device form."""
+        return [self._device_bound(hd) for hd in self.tr.bound_data(forward)]
 
     def step(self, tab, upd, sel_mask, sel_bits, cap, K, tmask, hd, keep_below=None, want_changed=False):
-        """One column: T'[s] = min(T[s], T[s ^ upd] + 1), the closing-row filter, the cap and the lookahead
-        (cdist/sparse_dp.py:338-403); returns (table, changed)."""
+        """T'[s] = min(T[s], T[s ^ upd] + 1), then the closing-row filter, the cap, keep_below and the lookahead:
+        (table, changed) as cdist/sparse_dp.py:338-403 returns them."""
         self.index(tab)
         n = tab.n
-        out = self._alloc(2 * n)
-        ch = self._alloc(2 * n) if want_changed else None
+        out = self._buffer(2 * n)
+        ch = self._buffer(2 * n) if want_changed else None
         if hd is None:
             ncls, mp, wp, extra, use_h = 0, None, None, 0, 0
         else:
-            dm, dw, ncls, extra = hd
+            dm, dw, ncls, extra = hd if len(hd) == 4 else self._device_bound(hd)
             mp, wp, use_h = dm.data_ptr(), dw.data_ptr(), 1
-        rc = self.L.tdp_sp_step(tab.ptr(), n, tab.slots.data_ptr(), tab.capacity, _split(upd), _split(sel_mask),
-                                _split(sel_bits), cap, keep_below or 0, ncls, mp, wp, extra, _split(tmask), K, use_h,
-                                out.data_ptr(), 2 * n, ch.data_ptr() if ch is not None else None, 2 * n,
-                                self.counters.data_ptr(), self.sh())
-        _lib.check(rc, "tdp_sp_step")
-        no, nc, overflow = self._sync_counters(3)
+        _lib.check(self.L.tdp_sp_step(tab.ptr(), n, tab.slots.data_ptr(), tab.capacity, _pair_array(upd),
+                                      _pair_array(sel_mask), _pair_array(sel_bits), cap, keep_below or 0, ncls, mp,
+                                      wp, extra, _pair_array(tmask), K, use_h, out.data_ptr(), 2 * n,
+                                      ch.data_ptr() if ch is not None else None, 2 * n, self.counters.data_ptr(),
+                                      self._sh), "tdp_sp_step")
+        nout, nch, overflow = self._read(3)
         if overflow:
             raise RuntimeError("sparse step output overflow")
-        return DevTable(out, no), (DevTable(ch, nc) if want_changed else None)
+        return DevTable(out, nout), (DevTable(ch, nch) if want_changed else None)
 
     @staticmethod
     def changed_size(ch):
         return ch.n
 
     def persistent(self, tab):
-        """A table kept across steps: its states in a buffer of its own size."""
+        """The table in a buffer of its own size (step outputs are sized for twice their input)."""
         if tab.buf.numel() > 16 * max(1, tab.n) * 5 // 4 + 16:
-            buf = self._alloc(tab.n)
+            buf = self._buffer(tab.n)
             buf[:16 * tab.n].copy_(tab.buf[:16 * tab.n])
             return DevTable(buf, tab.n)
         return tab
 
     def truncate(self, tab, below):
-        out = self._alloc(tab.n)
+        out = self._buffer(tab.n)
         _lib.check(self.L.tdp_sp_truncate(tab.ptr(), tab.n, below, out.data_ptr(), self.counters.data_ptr(),
-                                          self.sh()), "tdp_sp_truncate")
-        return DevTable(out, self._sync_counters(1)[0])
+                                          self._sh), "tdp_sp_truncate")
+        return DevTable(out, self._read(1)[0])
 
     def join(self, ch, btab, tmask):
-        """Least value(f) + value(b) over changed forward states f and backward states b = f ^ tmask:
-        (total, forward key, forward value, backward key) or None (cdist/sparse_dp.c:204-236); ties go to the
-        least forward key."""
+        """(total, forward key, forward value, backward key) of the least value(f) + value(b) over changed states f
+        and backward states b = f ^ tmask (ties: the least forward key), or None."""
         if not ch.n or not btab.n:
             return None
         self.index(btab)
         _lib.check(self.L.tdp_sp_join(ch.ptr(), ch.n, btab.ptr(), btab.slots.data_ptr(), btab.capacity,
-                                      _split(tmask), self.counters.data_ptr(), self.sh()), "tdp_sp_join")
-        tot, hi, lo, fval = self._sync_counters(4)
+                                      _pair_array(tmask), self.counters.data_ptr(), self._sh), "tdp_sp_join")
+        tot, hi, lo, fval = self._read(4)
         if tot == M64:
             return None
-        fkey = lo | (hi << 64)
+        fkey = _join(lo, hi)
         return int(tot), fkey, int(fval), fkey ^ tmask
 
     def hist(self, tab):
         h = _lib.device_empty(8 * 256)
-        _lib.check(self.L.tdp_sp_hist(tab.ptr(), tab.n, h.data_ptr(), self.sh()), "tdp_sp_hist")
-        return h.cpu().numpy().view(np.uint64).astype(np.int64)
+        _lib.check(self.L.tdp_sp_hist(tab.ptr(), tab.n, h.data_ptr(), self._sh), "tdp_sp_hist")
+        return h.cpu().numpy().view(U64).astype(np.int64)
 
 
 BACKENDS = {"hip": HipTables, "c": HipTables}
+NATIVE = ("hip", "c")
 
 
 class BudgetExceeded(RuntimeError):
@@ -361,331 +340,436 @@ class BudgetExceeded(RuntimeError):
         self.kind = kind
 
 
-def _make_backend(backend, tr):
-    if isinstance(backend, str):
-        return BACKENDS[backend](tr)
-    return backend
+def _check_caps(caps, direction):
+    if max(caps) > MAX_WEIGHT:
+        raise ValueError("a sweep keeping weights up to %d: the tables hold at most %d (8-bit values)"
+                         % (max(caps), MAX_WEIGHT))
+    pairs = zip(caps, caps[1:])
+    if not all((a >= b) if direction > 0 else (a <= b) for a, b in pairs):
+        raise AssertionError("forward caps must not increase" if direction > 0 else "backward caps must not decrease")
 
 
-# ---------------------------------------------------------------------------------------------- sweeps
+# ---------------------------------------------------------------------------------------------- one sweep
 class Sweep:
-    """One direction of the truncated sweep, with tables kept at checkpoints (cdist/sparse_dp.py:621-820).
+    """One direction of the truncated sweep over a table backend (any object with the reference's backend
+    interface), with tables kept at checkpoints.
 
-    direction +1: cuts 0 -> N from {0: 0}; -1: cuts N -> 0.  caps: one per cut, non-increasing in the sweep
-    direction.  frontier (the forward half of the meet in the middle): the step out of cut p keeps weights
-    <= caps[p] - 1 and returns the new or improved states of weight <= caps[p] as `changed`; where caps drops,
-    the table before the cut is returned as `extra` to be joined in full."""
+    direction +1 runs cuts 0 -> N from {0: 0}, -1 runs N -> 0; caps[c] (or one cap) bounds the weights kept at cut c,
+    non-increasing in the sweep's direction.  frontier (the forward half of a meet in the middle): the step out of
+    cut p keeps weights <= caps[p] - 1 and reports the new or improved states of weight <= caps[p] as `changed`;
+    where the caps drop, the table before the drop is reported as `extra`.  Attributes after run(): counts,
+    changed_counts, peak, peak_bytes, steps_done, ckpt (cut -> kept table)."""
 
     def __init__(self, tr, direction, K, cap=None, use_h=True, every=None, frontier=False, backend="hip",
                  dense=None, dense_max=1 << 30, max_states=None, deadline=None, caps=None):
-        self.tr, self.direction, self.K = tr, direction, K
-        self.caps = [min(int(x), K) for x in caps] if caps is not None else [min(cap, K)] * (tr.N + 1)
-        if max(self.caps) > MAX_WEIGHT:
-            raise ValueError("a sweep keeping weights up to %d: the tables hold at most %d (8-bit values)"
-                             % (max(self.caps), MAX_WEIGHT))
-        if direction > 0:
-            assert all(x >= y for x, y in zip(self.caps, self.caps[1:])), "forward caps must not increase"
-        else:
-            assert all(x <= y for x, y in zip(self.caps, self.caps[1:])), "backward caps must not decrease"
         if dense is not None and not frontier:
             raise ValueError("the GPU tables have no dense fallback (dense=None)")
+        self.tr, self.direction, self.K = tr, direction, K
+        self.caps = [min(int(x), K) for x in caps] if caps is not None else [min(cap, K)] * (tr.N + 1)
+        _check_caps(self.caps, direction)
         self.use_h, self.frontier = use_h, frontier
-        self.be = _make_backend(backend, tr)
-        self.hd = self.be.bound(direction > 0) if use_h else None
-        self.every = every or max(1, int(math.isqrt(tr.N)))
+        self.be = BACKENDS[backend](tr) if isinstance(backend, str) else backend
+        self.hd = self._bounds() if use_h else None
+        self.every = every or max(1, math.isqrt(tr.N))
         self.max_states, self.deadline = max_states, deadline
         self.ckpt = {}
         self.counts = [0] * (tr.N + 1)
         self.changed_counts = [0] * (tr.N + 1)
-        self.dense_cuts = 0
-        self.peak = 0
-        self.peak_bytes = 0
-        self.steps_done = 0
+        self.dense_cuts = self.peak = self.peak_bytes = self.steps_done = 0
+
+    def _bounds(self):
+        fwd = self.direction > 0
+        if hasattr(self.be, "bound"):
+            return self.be.bound(fwd)
+        if getattr(self.be, "name", "") == "c" and hasattr(self.tr, "bound_arrays"):
+            ncls, off, masks, weights, extra = self.tr.bound_arrays(fwd)
+            return [(masks[o:o + k, 0].copy(), weights[o:o + k].copy(), int(e))
+                    for k, o, e in zip(ncls.tolist(), off.tolist(), extra.tolist())]
+        return self.tr.bound_data(fwd)
 
     def first(self):
         return (0 if self.direction > 0 else self.tr.N), self.be.start()
 
-    def _step(self, tab, p, want_changed):
-        tr, fwd = self.tr, self.direction > 0
-        cut = p + 1 if fwd else p
-        upd = tr.col[p]
-        sm, sb = (tr.fsel_mask[p], tr.fsel_bits[p]) if fwd else (tr.bsel_mask[p], tr.bsel_bits[p])
+    def _advance(self, tab, p, want_changed):
+        """The column step at position p: (new cut, table, changed, extra)."""
+        tr = self.tr
+        if self.direction > 0:
+            cut, sel = p + 1, (tr.fsel_mask[p], tr.fsel_bits[p])
+        else:
+            cut, sel = p, (tr.bsel_mask[p], tr.bsel_bits[p])
         self.steps_done += 1
-        cap, kb = self.caps[cut], None
-        if self.frontier:
-            cap = kb = self.caps[p]
-        tab, ch = self.be.step(tab, upd, sm, sb, cap, self.K, tr.tmask[cut], self.hd[cut] if self.hd else None,
-                               kb, want_changed)
+        cap = self.caps[p] if self.frontier else self.caps[cut]
+        keep = cap if self.frontier else None
+        tab, changed = self.be.step(tab, tr.col[p], sel[0], sel[1], cap, self.K, tr.tmask[cut],
+                                    self.hd[cut] if self.hd else None, keep, want_changed)
         if self.frontier and self.caps[cut] < cap:
-            return cut, self.be.truncate(tab, self.caps[cut]), ch, tab
-        return cut, tab, ch, None
+            return cut, self.be.truncate(tab, self.caps[cut]), changed, tab
+        return cut, tab, changed, None
 
-    def _iter(self, cut, tab, stop, want_changed=False):
-        rng = range(cut, stop) if self.direction > 0 else range(cut - 1, stop - 1, -1)
-        for p in rng:
-            c, tab, ch, extra = self._step(tab, p, want_changed)
-            yield c, tab, ch, extra
+    def _steps(self, cut, tab, stop, want_changed=False):
+        """(cut, table, changed, extra) for each cut after `cut` up to `stop`."""
+        positions = range(cut, stop) if self.direction > 0 else range(cut - 1, stop - 1, -1)
+        for p in positions:
+            cut, tab, changed, extra = self._advance(tab, p, want_changed)
+            yield cut, tab, changed, extra
 
     def is_ckpt(self, c):
-        return c % self.every == 0 or c == 0 or c == self.tr.N
+        return c == 0 or c == self.tr.N or c % self.every == 0
 
     def run(self, on_cut=None):
-        """Full sweep keeping checkpoints and per-cut counts; on_cut(cut, table, changed, extra) may return True to
+        """The whole sweep, recording counts and checkpoints; on_cut(cut, table, changed, extra) may return True to
         stop.  Returns the last cut reached."""
-        c0, tab = self.first()
-        self.ckpt[c0] = tab
-        self.counts[c0] = self.changed_counts[c0] = 1
-        self.peak = 1
-        if on_cut and on_cut(c0, tab, tab, None):
-            return c0
-        stop = self.tr.N if self.direction > 0 else 0
-        ck_bytes = self.be.nbytes(tab)
-        for c, tab, ch, extra in self._iter(c0, tab, stop, want_changed=on_cut is not None):
+        c, tab = self.first()
+        self.ckpt[c] = tab
+        self.counts[c] = self.changed_counts[c] = self.peak = 1
+        kept_bytes = self.be.nbytes(tab)
+        if on_cut and on_cut(c, tab, tab, None):
+            return c
+        end = self.tr.N if self.direction > 0 else 0
+        for c, tab, changed, extra in self._steps(c, tab, end, on_cut is not None):
             n = self.be.size(tab)
-            nc = self.be.changed_size(ch) if ch is not None else 0
+            nc = self.be.changed_size(changed) if changed is not None else 0
             self.counts[c], self.changed_counts[c] = n, nc
             self.peak = max(self.peak, n, nc)
             if self.is_ckpt(c):
                 self.ckpt[c] = self.be.persistent(tab)
-                ck_bytes += self.be.nbytes(tab)
-            self.peak_bytes = max(self.peak_bytes, ck_bytes + self.be.nbytes(tab))
+                kept_bytes += self.be.nbytes(tab)
+            self.peak_bytes = max(self.peak_bytes, kept_bytes + self.be.nbytes(tab))
             if self.max_states is not None and max(n, nc) > self.max_states:
                 raise BudgetExceeded("%d states at cut %d" % (max(n, nc), c), "states")
             if self.deadline is not None and time.monotonic() > self.deadline:
                 raise BudgetExceeded("deadline at cut %d" % c, "deadline")
-            if on_cut and on_cut(c, tab, ch, extra):
+            if on_cut and on_cut(c, tab, changed, extra):
                 return c
-        return stop
+        return end
 
     def segment(self, lo, hi):
-        """Tables at the cuts lo..hi, recomputed from the nearest checkpoint: dict cut -> table."""
-        out = {}
-        if self.direction > 0:
-            c0 = max(c for c in self.ckpt if c <= lo)
-            tab = self.ckpt[c0]
-            if c0 >= lo:
-                out[c0] = tab
-            for c, tab, _, _ in self._iter(c0, tab, hi):
-                if c >= lo:
-                    out[c] = self.be.persistent(tab)
-        else:
-            c0 = min(c for c in self.ckpt if c >= hi)
-            tab = self.ckpt[c0]
-            if c0 <= hi:
-                out[c0] = tab
-            for c, tab, _, _ in self._iter(c0, tab, lo):
-                if c <= hi:
-                    out[c] = self.be.persistent(tab)
+        """Tables at cuts lo..hi, recomputed from the checkpoint that precedes them in the sweep."""
+        fwd = self.direction > 0
+        base = max(c for c in self.ckpt if c <= lo) if fwd else min(c for c in self.ckpt if c >= hi)
+        tab = self.ckpt[base]
+        out = {base: tab} if lo <= base <= hi else {}
+        for c, t, _, _ in self._steps(base, tab, hi if fwd else lo):
+            if lo <= c <= hi:
+                out[c] = self.be.persistent(t)
         return out
 
     def table(self, c, cache):
-        """Table at cut c for cuts requested against the sweep's direction (cache reused between calls)."""
+        """The table at cut c for requests running against the sweep (cache: a dict the caller keeps): a whole
+        stretch between checkpoints is recomputed at once."""
         if c not in cache:
             cache.clear()
-            if self.direction < 0:
-                cache.update(self.segment(c, min(x for x in self.ckpt if x >= c)))
-            else:
+            if self.direction > 0:
                 cache.update(self.segment(max(x for x in self.ckpt if x <= c), c))
+            else:
+                cache.update(self.segment(c, min(x for x in self.ckpt if x >= c)))
         return cache[c]
 
     def traceback(self, cut, state, value):
-        """Columns (problem indices) of a partial solution reaching `state` with weight `value` at `cut`
-        (cdist/sparse_dp.py:788-819)."""
-        tr, used = self.tr, []
+        """Columns of a partial solution reaching `state` with weight `value` at `cut`: walking back to the sweep's
+        start, a column was used where the table before its step does not hold the state with the same value."""
+        tr, cache, used = self.tr, {}, []
         s, v = int(state), int(value)
-        if self.direction > 0:
-            c = cut
-            while c > 0:
-                lo = max(0, (c - 1) // self.every * self.every)
-                tabs = self.segment(lo, c - 1)
-                for p in range(c - 1, lo - 1, -1):
-                    s, v = self._undo(tabs[p], s, v, s | tr.fsel_bits[p], tr.col[p], tr.order[p], used)
-                c = lo
-        else:
-            c = cut
-            while c < tr.N:
-                hi = min(tr.N, (c // self.every + 1) * self.every)
-                tabs = self.segment(c + 1, hi)
-                for p in range(c, hi):
-                    s, v = self._undo(tabs[p + 1], s, v, s | tr.bsel_bits[p], tr.col[p], tr.order[p], used)
-                c = hi
-        assert s == 0 and v == 0, (s, v)
+        fwd = self.direction > 0
+        for p in (range(cut - 1, -1, -1) if fwd else range(cut, tr.N)):
+            tab = self.table(p if fwd else p + 1, cache)
+            before = s | (tr.fsel_bits[p] if fwd else tr.bsel_bits[p])
+            if self.be.lookup(tab, before) == v:
+                s = before
+                continue
+            if not tr.col[p] or self.be.lookup(tab, before ^ tr.col[p]) != v - 1:
+                raise AssertionError("traceback lost the path at position %d" % p)
+            s, v = before ^ tr.col[p], v - 1
+            used.append(tr.order[p])
+        if (s, v) != (0, 0):
+            raise AssertionError("traceback ended at state %x value %d" % (s, v))
         return used
 
-    def _undo(self, tab, s, v, s_mid, upd, column, used):
-        stay, move = self.be.lookup_many(tab, [s_mid, s_mid ^ upd]) if upd else (self.be.lookup(tab, s_mid), INF)
-        if int(stay) == v:
-            return s_mid, v
-        assert upd and int(move) == v - 1, "traceback lost the path"
-        used.append(column)
-        return s_mid ^ upd, v - 1
+
+# ---------------------------------------------------------------------------------------------- native runs
+class _Spec(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("N", "K", "mode", "use_h", "witness", "lower", "every", "hist_top")] + [
+        ("max_states", ctypes.c_uint64), ("max_seconds", ctypes.c_double), ("keep_bytes", ctypes.c_uint64),
+        ("prm", ctypes.c_void_p), ("tm", ctypes.c_void_p), ("split", ctypes.c_void_p)] + [
+        (k, ctypes.c_void_p * 2) for k in ("bnd_ncls", "bnd_off", "bnd_mask", "bnd_w", "bnd_extra")] + [
+        ("hist", ctypes.c_void_p)]
 
 
-# ---------------------------------------------------------------------------------------------- the split
-def _values(be, tab):
-    return be.arrays(tab)[1]
+class _Result(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int32) for k in ("status", "fail_cut", "value", "join_cut", "nwitness", "pad")] + [
+        (k, ctypes.c_uint64) for k in ("steps", "joined", "peak_states", "peak_bytes")] + [
+        (k, ctypes.c_double) for k in ("seconds", "backward_seconds", "forward_seconds", "traceback_seconds")]
 
 
-def weight_profile(tr, direction, cap, backend="hip"):
-    """States of each weight <= cap summed over all cuts, for a sweep keeping weights <= cap (no lookahead)."""
-    sw = Sweep(tr, direction, cap, cap, use_h=False, backend=backend)
-    c0, tab = sw.first()
-    tot = np.zeros(cap + 1)
-    tot[0] = 1
-    for c, tab, _, _ in sw._iter(c0, tab, tr.N if direction > 0 else 0):
-        tot += sw.be.hist(tab)[:cap + 1]
-    return tot
+SQ_FORWARD, SQ_MITM, SQ_PROFILE_FW, SQ_PROFILE_BW = range(4)
+SQ_DONE, SQ_STATES, SQ_DEADLINE = range(3)
 
 
-def choose_split(tr, K, backend="hip", probe=3, budget=2e9):
-    """Split a minimising sum_{w<=a-1} F(w) + F(a-1) + 2 sum_{w<=K-a} B(w), F and B measured to weight `probe` and
-    extrapolated with the growth of their last two weights (cdist/sparse_dp.py:839-867).  Returns (a, info)."""
-    t0 = time.time()
-    prof = {}
-    for direction in (+1, -1):
-        p = weight_profile(tr, direction, min(2, probe), backend)
-        if probe > 2 and len(p) > 2 and p[1] > 0 and p[2] * p[2] / p[1] < budget:
-            p = weight_profile(tr, direction, probe, backend)
-        prof[direction] = p
+def _native(tr, K, mode, split=None, use_h=False, witness=False, lower=0, every=None, max_states=None,
+            deadline=None, hist_top=None, keep_bytes=None, stream=None):
+    """One tdp_sq_run call: (result record, forward counts, backward counts, changed counts, witness positions,
+    histogram or None)."""
+    N = tr.N
+    spec = _Spec()
+    spec.N, spec.K, spec.mode = N, K, mode
+    spec.use_h, spec.witness, spec.lower = int(bool(use_h)), int(bool(witness)), int(lower)
+    spec.every = int(every or max(1, math.isqrt(N)))
+    spec.max_states = int(max_states or 0)
+    spec.max_seconds = -1.0 if deadline is None else max(0.0, deadline - time.monotonic())
+    spec.keep_bytes = KEEP_BYTES if keep_bytes is None else int(keep_bytes)
+    prm, tm = tr.words()
+    prm, tm = np.ascontiguousarray(prm), np.ascontiguousarray(tm)
+    spec.prm, spec.tm = prm.ctypes.data, tm.ctypes.data
+    keep = [prm, tm]
+    if split is not None:
+        sp = np.ascontiguousarray(split, dtype=np.int32)
+        spec.split = sp.ctypes.data
+        keep.append(sp)
+    if use_h:
+        for d, fwd in enumerate((True, False)):
+            arrays = tr.bound_arrays(fwd)
+            keep += arrays
+            ncls, off, masks, weights, extra = arrays
+            spec.bnd_ncls[d], spec.bnd_off[d] = ncls.ctypes.data, off.ctypes.data
+            spec.bnd_mask[d], spec.bnd_w[d], spec.bnd_extra[d] = masks.ctypes.data, weights.ctypes.data, extra.ctypes.data
+    hist = None
+    if hist_top is not None:
+        spec.hist_top = hist_top
+        hist = np.zeros((N + 1, hist_top + 1), dtype=np.uint64)
+        spec.hist = hist.ctypes.data
+    fw = np.zeros(N + 1, dtype=np.uint64)
+    bw = np.zeros(N + 1, dtype=np.uint64)
+    ch = np.zeros(N + 1, dtype=np.uint64)
+    wit = np.zeros(K + 2, dtype=np.int32)
+    res = _Result()
+    _lib.check(_lib.lib().tdp_sq_run(ctypes.byref(spec), fw.ctypes.data, bw.ctypes.data, ch.ctypes.data,
+                                     wit.ctypes.data, ctypes.byref(res), _lib.stream_handle(stream)), "tdp_sq_run")
+    if res.status == SQ_STATES:
+        raise BudgetExceeded("%d states at cut %d" % (res.peak_states, res.fail_cut), "states")
+    if res.status == SQ_DEADLINE:
+        raise BudgetExceeded("deadline at cut %d" % res.fail_cut, "deadline")
+    return res, fw.tolist(), bw.tolist(), ch.tolist(), wit[:res.nwitness].tolist(), hist
 
-    def extend(p, upto):
-        p = list(p)
-        r = p[-1] / p[-2] if len(p) > 1 and p[-2] > 0 else 1.0
-        while len(p) <= upto:
-            p.append(p[-1] * max(r, 1.0))
-        return p
 
-    F, B = extend(prof[+1], K), extend(prof[-1], K)
-    costs = {a: sum(F[:a]) + F[a - 1] + 2 * sum(B[:K - a + 1]) for a in range(1, K + 1)}
-    a = min(costs, key=costs.get)
-    return a, {"forward_profile": [float(x) for x in prof[+1]], "backward_profile": [float(x) for x in prof[-1]],
-               "predicted_work": {int(k): float(v) for k, v in costs.items()}, "probe_seconds": time.time() - t0}
-
-
+# ---------------------------------------------------------------------------------------------- split choice
 def cut_profiles(tr, direction, cap, backend="hip"):
-    """States of each weight <= cap at every cut (array (N + 1, cap + 1))."""
+    """States of each weight <= cap at every cut, array (N + 1, cap + 1), of a sweep keeping weights <= cap
+    without lookahead."""
+    if isinstance(backend, str) and backend in NATIVE:
+        *_, hist = _native(tr, cap, SQ_PROFILE_FW if direction > 0 else SQ_PROFILE_BW, hist_top=cap)
+        return hist.astype(np.float64)
     sw = Sweep(tr, direction, cap, cap, use_h=False, backend=backend)
     c0, tab = sw.first()
     out = np.zeros((tr.N + 1, cap + 1))
     out[c0, 0] = 1
-    for c, tab, _, _ in sw._iter(c0, tab, tr.N if direction > 0 else 0):
-        out[c] = sw.be.hist(tab)[:cap + 1]
+    for c, tab, _, _ in sw._steps(c0, tab, tr.N if direction > 0 else 0):
+        vals = np.asarray(sw.be.arrays(tab)[1], dtype=np.int64)
+        out[c] = np.bincount(vals, minlength=cap + 1)[:cap + 1]
     return out
 
 
+def weight_profile(tr, direction, cap, backend="hip"):
+    """States of each weight <= cap summed over the cuts of such a sweep."""
+    return cut_profiles(tr, direction, cap, backend).sum(axis=0)
+
+
 def probe_profiles(tr, backend="hip", probe=3, budget=2e9):
-    """Per-cut state counts per weight in both directions up to weight `probe` (or 2 where weight 3 would exceed
-    the budget), kept on the trellis; returns (profiles, seconds spent now) (cdist/sparse_dp.py:882-903)."""
+    """Per-cut state counts by weight in both directions, measured up to weight `probe`, or up to 2 in a direction
+    where weight `probe` would cost more than `budget` states (estimated from the growth between weights 1 and 2;
+    budget may be a number or {+1: ..., -1: ...}).  Cached on the trellis; returns ({+1: array, -1: array},
+    seconds spent now)."""
     t0 = time.time()
+    cache = tr.__dict__.setdefault("_profiles", {})
+    bname = backend if isinstance(backend, str) else getattr(backend, "name", "backend")
+    out = {}
+    for d in (+1, -1):
+        lim = budget[d] if isinstance(budget, dict) else budget
+        low = cache.get((d, 2, bname))
+        if low is None:
+            low = cache[(d, 2, bname)] = cut_profiles(tr, d, min(2, probe), backend)
+        tot = low.sum(axis=0)
+        grown = tot[-1] * (tot[-1] / tot[1]) ** (probe - 2) if len(tot) > 2 and tot[1] > 0 else np.inf
+        if probe > 2 and grown < lim:
+            key = (d, probe, bname)
+            if key not in cache:
+                cache[key] = cut_profiles(tr, d, probe, backend)
+            out[d] = cache[key]
+        else:
+            out[d] = low
+    return out, time.time() - t0
 
-    def measured(direction, cap):
-        key = ("profile", direction, cap)
-        if key not in tr._bd:
-            tr._bd[key] = cut_profiles(tr, direction, cap, backend)
-        return tr._bd[key]
 
-    prof = {}
-    for direction in (+1, -1):
-        b = budget[direction] if isinstance(budget, dict) else budget
-        p = measured(direction, min(2, probe))
-        tot = p.sum(axis=0)
-        if probe > 2 and tot[1] > 0 and tot[2] * tot[2] / tot[1] < b:
-            p = measured(direction, probe)
-        prof[direction] = p
-    return prof, time.time() - t0
+def _extend(per_weight, K, window=0):
+    """Counts by weight extended to weights 0..K: each new weight is the last one times the growth between the two
+    measured ones (at least 1), smoothed over +-window cuts when per_weight is per cut."""
+    p = np.atleast_2d(per_weight).astype(np.float64)
+    top = p.shape[1] - 1
+    if top >= K:
+        return p[:, :K + 1]
+    last, prev = p[:, top], p[:, top - 1] if top >= 1 else p[:, top]
+    if window:
+        def windowed(x):                 # sums over cuts c - window .. c + window
+            cs = np.concatenate([[0.0], np.cumsum(x)])
+            i = np.arange(x.size)
+            return cs[np.minimum(x.size, i + window + 1)] - cs[np.maximum(0, i - window)]
+        last, prev = windowed(last), windowed(prev)
+    growth = np.maximum(1.0, np.divide(last, prev, out=np.ones_like(last), where=prev > 0))
+    cols = [p[:, w] for w in range(top + 1)]
+    while len(cols) <= K:
+        cols.append(cols[-1] * growth)
+    return np.stack(cols, axis=1)
+
+
+def _native_cost(F, B, K, a, backward_factor):
+    """Predicted states of the native meet in the middle at split a, from states by weight (forward F, backward B,
+    per cut or summed): the forward tables (weights <= a - 1) and frontier (weight a), the backward tables (weights
+    <= K - a), the latter twice when they must be recomputed from checkpoints."""
+    cF, cB = np.cumsum(F, axis=-1), np.cumsum(B, axis=-1)
+    return cF[..., a - 1] + F[..., a - 1] + backward_factor * cB[..., K - a]
+
+
+def choose_split(tr, K, backend="hip", probe=3, budget=2e9):
+    """A uniform split a for threshold K minimising the predicted states of the native runner (returns (a, info)).
+    The backward tables count once when the sweep is predicted to keep them all (KEEP_BYTES), twice otherwise."""
+    prof, secs = probe_profiles(tr, backend, probe, budget)
+    F, B = _extend(prof[+1].sum(axis=0), K)[0], _extend(prof[-1].sum(axis=0), K)[0]
+    costs = {}
+    for a in range(1, K + 1):
+        factor = 1.0 if 48 * np.cumsum(B)[K - a] <= KEEP_BYTES else 2.0
+        costs[a] = float(_native_cost(F, B, K, a, factor))
+    a = min(costs, key=costs.get)
+    return a, {"forward_profile": prof[+1].sum(axis=0).tolist(), "backward_profile": prof[-1].sum(axis=0).tolist(),
+               "predicted_work": costs, "probe_seconds": secs}
 
 
 def choose_schedule(tr, K, backend="hip", probe=3, budget=2e9, window=100):
-    """A non-increasing split a(c) per cut minimising sum_c F_c(<= a-1) + F_c(a-1) + 2 B_c(<= K-a) by a dynamic
-    programme over cuts and values of a (cdist/sparse_dp.py:906-981).  Returns (schedule, info)."""
-    prof, probe_seconds = probe_profiles(tr, backend, probe, budget)
-
-    def wsum(x):
-        cs = np.concatenate([[0.0], np.cumsum(x)])
-        idx = np.arange(x.size)
-        return cs[np.minimum(x.size, idx + window + 1)] - cs[np.maximum(0, idx - window)]
-
-    def extend(p):
-        top = p.shape[1] - 1
-        if top >= K:
-            return p[:, :K + 1]
-        hi = wsum(p[:, top])
-        lo = wsum(p[:, top - 1]) if top >= 1 else hi
-        r = np.maximum(np.where(lo > 0, hi / np.maximum(lo, 1e-300), 1.0), 1.0)
-        cols = [p[:, w] for w in range(top + 1)]
-        while len(cols) <= K:
-            cols.append(cols[-1] * r)
-        return np.stack(cols, axis=1)
-
-    F, B = extend(prof[+1]), extend(prof[-1])
-    cF, cB = np.cumsum(F, axis=1), np.cumsum(B, axis=1)
+    """A split a(c) per cut, non-increasing in c, minimising the summed per-cut predicted states (the per-cut form
+    of choose_split's model, with the probe counts extended over a window of cuts).  The search runs from the last
+    cut back: best[c][a] = cost_c(a) + min over a' <= a of best[c + 1][a'], then a(0) = argmin best[0] and each
+    a(c + 1) = the argmin of best[c + 1] over a' <= a(c).  Returns (schedule, info) with the predicted work, peak and
+    mean states and device bytes."""
+    prof, secs = probe_profiles(tr, backend, probe, budget)
+    F, B = _extend(prof[+1], K, window), _extend(prof[-1], K, window)
     n = tr.N + 1
-    cost = np.zeros((n, K + 1))
+    factor = 1.0 if 48 * np.cumsum(B, axis=1)[:, K - 1].sum() <= KEEP_BYTES else 2.0
+    cost = np.full((n, K + 1), np.inf)
     for a in range(1, K + 1):
-        cost[:, a] = cF[:, a - 1] + F[:, a - 1] + 2 * cB[:, K - a]
-    cost[:, 0] = np.inf
+        cost[:, a] = _native_cost(F, B, K, a, factor)
     best = np.empty((n, K + 1))
-    arg = np.zeros((n, K + 1), dtype=np.int64)
-    best[0] = cost[0]
+    best[n - 1] = cost[n - 1]
+    for c in range(n - 2, -1, -1):
+        best[c] = cost[c] + np.minimum.accumulate(best[c + 1])
+    sched = [int(np.argmin(best[0]))]
     for c in range(1, n):
-        prev = best[c - 1]
-        suf = np.minimum.accumulate(prev[::-1])[::-1]
-        sarg = np.empty(K + 1, dtype=np.int64)
-        m, ai = np.inf, K
-        for a in range(K, -1, -1):
-            if prev[a] < m:
-                m, ai = prev[a], a
-            sarg[a] = ai
-        best[c] = cost[c] + suf
-        arg[c] = sarg
-    a = int(np.argmin(best[n - 1]))
-    sched = [0] * n
-    for c in range(n - 1, -1, -1):
-        sched[c] = a
-        if c:
-            a = int(arg[c, a])
-    idx = np.arange(n)
-    av = np.array(sched)
-    fwd = cF[idx, av - 1] + F[idx, av - 1]
-    bwd = cB[idx, K - av]
-    every = max(1, int(math.isqrt(tr.N)))
-    ck = (idx % every == 0) | (idx == 0) | (idx == n - 1)
-    seg = np.add.reduceat(bwd, np.arange(0, n, every)).max()
-    states = bwd[ck].sum() + seg + cF[idx, av - 1][ck].sum() + 4 * max(fwd.max(), bwd.max())
+        sched.append(int(np.argmin(best[c][:sched[-1] + 1])))
+    av, idx = np.array(sched), np.arange(n)
+    fwd = np.cumsum(F, axis=1)[idx, av - 1] + F[idx, av - 1]
+    bwd = np.cumsum(B, axis=1)[idx, K - av]
     uniform = {a: float(cost[:, a].sum()) for a in range(1, K + 1)}
-    return sched, {"predicted_work": float(best[n - 1].min()), "uniform_predicted_work": uniform,
+    return sched, {"predicted_work": float(best[0].min()), "uniform_predicted_work": uniform,
                    "predicted_peak_states": float(max(fwd.max(), bwd.max())),
                    "predicted_mean_states": float((fwd.sum() + bwd.sum()) / (2 * n)),
-                   "predicted_bytes": float(16 * states),
+                   "predicted_bytes": float(48 * (fwd.sum() + bwd.sum())),
                    "schedule": _runs(sched), "forward_profile": prof[+1].sum(axis=0).tolist(),
-                   "backward_profile": prof[-1].sum(axis=0).tolist(), "probe_seconds": probe_seconds}
+                   "backward_profile": prof[-1].sum(axis=0).tolist(), "probe_seconds": secs}
+
+
+def _runs(x):
+    """A per-cut list as 'value' or 'value@cut,value@cut,...' at the cuts where it changes."""
+    marks = [(v, c) for c, v in enumerate(x) if c == 0 or v != x[c - 1]]
+    return str(marks[0][0]) if len(marks) == 1 else ",".join("%d@%d" % m for m in marks)
+
+
+def _split_schedule(tr, K, split, backend):
+    """(per-cut split, split_choice info or None, predicted work or None)."""
+    info = predicted = None
+    if split == "auto":
+        split, info = choose_split(tr, K, backend)
+        predicted = info["predicted_work"][split]
+    elif split == "schedule":
+        split, info = choose_schedule(tr, K, backend)
+        predicted = info["predicted_work"]
+    if split is None:
+        split = K // 2 + 1
+    if isinstance(split, (list, tuple, np.ndarray)):
+        sched = [max(1, min(int(x), K)) for x in split]
+    else:
+        sched = [max(1, min(int(split), K))] * (tr.N + 1)
+    if len(sched) != tr.N + 1 or any(a < b for a, b in zip(sched, sched[1:])):
+        raise AssertionError("split must not increase")
+    return sched, info, predicted
 
 
 # ---------------------------------------------------------------------------------------------- one query
 def query(tr, K, mode="mitm", split=None, use_h=True, witness=True, lower=0, every=None, backend="hip",
           dense=None, dense_max=1 << 30, log=None, max_states=None, deadline=None):
-    """Least weight of a solution of the trellis' query if it is <= K (else None), with per-cut statistics and a
-    witness (cdist/sparse_dp.py:985-1095).  mode "forward": one sweep keeping weights <= K.  mode "mitm": split a(c)
-    per cut, non-increasing (an int, "auto", "schedule" or a list of N + 1 values); the backward sweep keeps
-    weights <= K - a(c), the forward frontier weights <= a(c) - 1 and its new or improved states are joined with the
-    backward table at every cut.  See the reference's docstring for why every solution of weight <= K is seen."""
+    """The least weight of a solution of the trellis' query if it is <= K (value None otherwise), with per-cut
+    statistics and a witness (problem columns); the record of cdist/sparse_dp.py:985-1095.
+
+    mode "forward": one sweep keeping weights <= K.  mode "mitm": split a(c) per cut (an int, "auto", "schedule"
+    or N + 1 non-increasing values; default K // 2 + 1).  lower: a known lower bound, reaching it stops the sweep.
+    max_states / deadline (time.monotonic()) raise BudgetExceeded.  Tables hold weights <= 254: a larger cap raises
+    ValueError.  Why every solution of weight <= K is seen is argued in the reference's docstring: both bounds
+    K - a(c) and a(c) fall along the cuts while a solution's left weight rises by steps of one."""
+    if mode not in ("forward", "mitm"):
+        raise ValueError("mode must be 'forward' or 'mitm'")
+    if dense is not None:
+        raise ValueError("the GPU tables have no dense fallback (dense=None)")
     t0 = time.time()
     bname = backend if isinstance(backend, str) else backend.name
     rec = {"K": K, "mode": mode, "use_h": bool(use_h), "value": None, "witness": None, "width": max(tr.width),
            "backend": bname}
-
-    def new_be():
-        return BACKENDS[backend](tr) if isinstance(backend, str) else backend.__class__(tr)
-
-    be = new_be()
     if mode == "forward":
-        fw = Sweep(tr, +1, K, K, use_h, every, backend=be, dense=dense, dense_max=dense_max, max_states=max_states,
-                   deadline=deadline)
+        _check_caps([K], +1)
+        sched = None
+    else:
+        sched, info, predicted = _split_schedule(tr, K, split, backend)
+        if info is not None:
+            rec["split_choice"] = info
+        _check_caps(sched, +1)
+        _check_caps([K - a for a in sched], -1)
+        if log:
+            log("K=%d split a=%s: backward sweep (weights <= %s)%s" % (
+                K, _runs(sched), _runs([K - a for a in sched]),
+                "" if predicted is None else ", predicted work %.3g" % predicted))
+    if isinstance(backend, str) and backend in NATIVE:
+        return _native_query(tr, K, mode, sched, use_h, witness, lower, every, max_states, deadline, rec, t0)
+    return _sweep_query(tr, K, mode, sched, use_h, witness, lower, every, backend, max_states, deadline, rec, t0,
+                        log)
+
+
+def _native_query(tr, K, mode, sched, use_h, witness, lower, every, max_states, deadline, rec, t0):
+    res, fw, bw, ch, pos, _ = _native(tr, K, SQ_FORWARD if mode == "forward" else SQ_MITM, sched, use_h, witness,
+                                      lower, every, max_states, deadline)
+    common = {"peak_states": int(res.peak_states), "peak_table_bytes": int(res.peak_bytes), "steps": int(res.steps),
+              "dense_cuts": 0}
+    if mode == "forward":
+        rec.update(forward_counts=fw, sweep_seconds=res.forward_seconds, **common)
+    else:
+        rec.update(forward_counts=fw, backward_counts=bw, changed_counts=ch, backward_seconds=res.backward_seconds,
+                   forward_seconds=res.forward_seconds, split=sched[0] if len(set(sched)) == 1 else _runs(sched),
+                   split_values=sched, joined=int(res.joined), **common)
+    if res.value >= 0:
+        rec["value"] = int(res.value)
+        if mode == "mitm":
+            rec["join_cut"] = int(res.join_cut)
+        if witness:
+            rec["witness"] = sorted(tr.order[p] for p in pos)
+            rec["traceback_seconds"] = res.traceback_seconds
+    rec["seconds"] = time.time() - t0
+    return rec
+
+
+def _sweep_query(tr, K, mode, sched, use_h, witness, lower, every, backend, max_states, deadline, rec, t0, log):
+    """query() over a backend object: the sweeps of this module, one column step per backend call."""
+    def fresh():
+        return backend.__class__(tr)
+    if mode == "forward":
+        fw = Sweep(tr, +1, K, K, use_h, every, backend=fresh(), max_states=max_states, deadline=deadline)
         fw.run()
-        tab = fw.ckpt[tr.N]
         rec.update(forward_counts=fw.counts, peak_states=fw.peak, peak_table_bytes=fw.peak_bytes,
-                   sweep_seconds=time.time() - t0, steps=fw.steps_done, dense_cuts=fw.dense_cuts)
-        v = be.lookup(tab, 0)
+                   sweep_seconds=time.time() - t0, steps=fw.steps_done, dense_cuts=0)
+        v = fw.be.lookup(fw.ckpt[tr.N], 0)
         if v != INF:
             rec["value"] = v
             if witness:
@@ -694,85 +778,67 @@ def query(tr, K, mode="mitm", split=None, use_h=True, witness=True, lower=0, eve
                 rec["traceback_seconds"] = time.time() - t1
         rec["seconds"] = time.time() - t0
         return rec
-    predicted = None
-    if split == "auto":
-        split, rec["split_choice"] = choose_split(tr, K, new_be())
-        predicted = rec["split_choice"]["predicted_work"][split]
-    elif split == "schedule":
-        split, rec["split_choice"] = choose_schedule(tr, K, new_be())
-        predicted = rec["split_choice"]["predicted_work"]
-    if split is None:
-        split = K // 2 + 1
-    sched = [max(1, min(int(x), K)) for x in split] if isinstance(split, (list, tuple)) else \
-        [max(1, min(int(split), K))] * (tr.N + 1)
-    assert len(sched) == tr.N + 1 and all(x >= y for x, y in zip(sched, sched[1:])), "split must not increase"
-    bw = Sweep(tr, -1, K, use_h=use_h, every=every, backend=be, dense=dense, dense_max=dense_max,
-               max_states=max_states, deadline=deadline, caps=[K - x for x in sched])
-    fw = Sweep(tr, +1, K, use_h=use_h, every=bw.every, frontier=True, backend=new_be(), max_states=max_states,
+    bw = Sweep(tr, -1, K, use_h=use_h, every=every, backend=fresh(), max_states=max_states, deadline=deadline,
+               caps=[K - a for a in sched])
+    fw = Sweep(tr, +1, K, use_h=use_h, every=bw.every, frontier=True, backend=fresh(), max_states=max_states,
                deadline=deadline, caps=sched)
-    if log:
-        log("K=%d split a=%s: backward sweep (weights <= %s)%s" % (
-            K, _runs(sched), _runs([K - x for x in sched]), "" if predicted is None else
-            ", predicted work %.3g" % predicted))
     bw.run()
     t1 = time.time()
     if log:
-        log("backward sweep done: %.1f s, peak %d states, %.3g states in all" % (t1 - t0, bw.peak, sum(bw.counts)))
-    best, cache, joins = [None], {}, [0]
+        log("backward sweep done: %.1f s, peak %d states" % (t1 - t0, bw.peak))
+    state = {"best": None, "joined": 0}
+    cache = {}
 
-    def join(c, tab, ch, extra):
-        for part in (ch, extra):
-            if part is None or not be.changed_size(part):
+    def at_cut(c, tab, changed, extra):
+        for part in (changed, extra):
+            if part is None or not fw.be.changed_size(part):
                 continue
-            joins[0] += be.changed_size(part)
-            r = be.join(part, bw.table(c, cache), tr.tmask[c])
-            if r is not None and (best[0] is None or r[0] < best[0][0]):
-                best[0] = (r[0], c) + tuple(r[1:])
-        return best[0] is not None and best[0][0] <= max(lower, 0)
+            state["joined"] += fw.be.changed_size(part)
+            hit = fw.be.join(part, bw.table(c, cache), tr.tmask[c])
+            if hit is not None and (state["best"] is None or hit[0] < state["best"][0]):
+                state["best"] = (hit[0], c) + tuple(hit[1:])
+        return state["best"] is not None and state["best"][0] <= max(lower, 0)
 
-    fw.run(on_cut=join)
+    fw.run(on_cut=at_cut)
     cache.clear()
     rec.update(forward_counts=fw.counts, backward_counts=bw.counts, changed_counts=fw.changed_counts,
                peak_states=max(fw.peak, bw.peak), peak_table_bytes=fw.peak_bytes + bw.peak_bytes,
                backward_seconds=t1 - t0, forward_seconds=time.time() - t1,
-               split=sched[0] if len(set(sched)) == 1 else _runs(sched), split_values=sched, joined=joins[0],
-               steps=fw.steps_done + bw.steps_done, dense_cuts=bw.dense_cuts)
-    if best[0] is not None and best[0][0] <= K:
-        value, c, sl, fl, sr = best[0]
-        rec["value"], rec["join_cut"] = value, c
+               split=sched[0] if len(set(sched)) == 1 else _runs(sched), split_values=sched,
+               joined=state["joined"], steps=fw.steps_done + bw.steps_done, dense_cuts=0)
+    best = state["best"]
+    if best is not None and best[0] <= K:
+        total, c, fkey, fval, bkey = best
+        rec["value"], rec["join_cut"] = total, c
         if witness:
             t2 = time.time()
-            rec["witness"] = sorted(fw.traceback(c, sl, fl) + bw.traceback(c, sr, value - fl))
+            rec["witness"] = sorted(fw.traceback(c, fkey, fval) + bw.traceback(c, bkey, total - fval))
             rec["traceback_seconds"] = time.time() - t2
     rec["seconds"] = time.time() - t0
     return rec
 
 
-def _runs(x):
-    """A per-cut list as 'value' or 'value@cut,...' where it changes."""
-    out = [(v, c) for c, v in enumerate(x) if c == 0 or v != x[c - 1]]
-    return str(out[0][0]) if len(out) == 1 else ",".join("%d@%d" % vc for vc in out)
-
-
 # ---------------------------------------------------------------------------------------------- distance
 def default_order(p, seeds=2):
+    """The column order the dense planner picks (any order gives the same exact value)."""
     from . import auto
-    return auto.best_order(p)[0]
+    return auto.best_order(p, seeds=seeds)[0]
 
 
 def distance(p, order=None, start=1, kmax=64, mode="mitm", use_h=True, witness=True, log=None, every=None,
              backend="hip", dense=None, split=None):
-    """Exact distance by iterative deepening of K from `start`, a lower bound (cdist/sparse_dp.py:1110-1151)."""
+    """The exact distance by raising the threshold from `start` (a lower bound): a threshold where no observable
+    has a solution proves the next one.  Returns {"distance", "witness", "runs" (summaries per threshold and
+    observable), "lower_bound", "order_cut_rank", "seconds"}; distance None if none is <= kmax."""
     t0 = time.time()
-    order = default_order(p) if order is None else order
-    trs = [Trellis.for_query(p, i, order) for i in range(p.nobs)]
+    order = default_order(p) if order is None else list(order)
+    trellises = [Trellis.for_query(p, i, order) for i in range(p.nobs)]
+    live = [(i, t) for i, t in enumerate(trellises) if t.feasible]
     out = {"distance": None, "witness": None, "runs": [], "lower_bound": start,
-           "order_cut_rank": max((max(t.width) for t in trs if t.feasible), default=0)}
+           "order_cut_rank": max((max(t.width) for _, t in live), default=0)}
     for K in range(start, kmax + 1):
-        best = None
-        for i, tr in enumerate(trs):
-            if not tr.feasible:
-                continue
+        found = None
+        for i, tr in live:
             r = query(tr, K, mode=mode, use_h=use_h, witness=witness, lower=K, every=every, backend=backend,
                       dense=dense, split=split(K) if callable(split) else split)
             r["obs"] = i
@@ -780,19 +846,19 @@ def distance(p, order=None, start=1, kmax=64, mode="mitm", use_h=True, witness=T
             if log:
                 log("  K=%d obs %d: value %s peak %d states %.2fs" % (K, i, r["value"], r["peak_states"],
                                                                        r["seconds"]))
-            if r["value"] is not None and (best is None or r["value"] < best["value"]):
-                best = r
-                if r["value"] <= K:
-                    break
-        if best is not None:
-            out["distance"] = best["value"]
+            if r["value"] is not None:
+                found = r                       # every lower threshold failed, so this value (<= K) is the least
+                break
+        if found is not None:
+            out["distance"] = found["value"]
             if witness:
-                cols = best["witness"]
-                s = lg = 0
+                cols = found["witness"]
+                syn = lg = 0
                 for j in cols:
-                    s ^= p.h[j]
+                    syn ^= p.h[j]
                     lg ^= p.l[j]
-                assert s == 0 and lg != 0 and len(cols) == best["value"] == len(set(cols)), "witness check failed"
+                if syn or not lg or len(cols) != found["value"] or len(set(cols)) != len(cols):
+                    raise AssertionError("witness check failed")
                 out["witness"] = cols
             break
         out["lower_bound"] = K + 1
@@ -801,9 +867,11 @@ def distance(p, order=None, start=1, kmax=64, mode="mitm", use_h=True, witness=T
 
 
 def summary(r):
-    """A query record without the per-cut arrays (their totals instead)."""
-    out = {k: v for k, v in r.items() if not k.endswith("_counts") and k not in ("witness", "split_values")}
-    for k in ("forward_counts", "backward_counts", "changed_counts"):
-        if k in r:
-            out[k.replace("_counts", "_states_total")] = int(sum(r[k]))
+    """A query record with its per-cut lists replaced by their totals."""
+    out = {}
+    for k, v in r.items():
+        if k.endswith("_counts"):
+            out[k[:-len("_counts")] + "_states_total"] = int(sum(v))
+        elif k not in ("witness", "split_values"):
+            out[k] = v
     return out
