@@ -167,7 +167,7 @@ def test_planning_order_is_best_orders_choice():
     from tropidist_amd import auto, problem
     for seed in range(4):
         p = problem.banded(R=60, N=240, kmin=2, kmax=4, W=8 + seed, nobs=2, rho=0.1, seed=seed)
-        assert auto.planning_order(p) == auto.best_order(p)[0]
+        assert auto.planning_order(p) == auto.best_order(p, seeds=0)[0]
 
 
 def test_more_than_63_active_rows_is_refused():

@@ -7,8 +7,8 @@ op as in the reference (cdist/kernel.py:112-135).  Two executors give identical 
     "fused"   (default) tdp_run_program: per-op kernels while the table is narrow, then passes of up to five
               update columns per kernel (tropidist_amd/fused.py compiles them);
     "per-op"  tdp_run_plan: one streaming kernel per op over the uint8 table in HBM.
-The reference's numpy executor run_plan_numpy is a CPU path and lives in the oracle (oracle/oracle.py) as a checker
-only."""
+run_plan_numpy(plan) keeps the reference's name for its array executor (cdist/kernel.py:72-91) for callers that use
+it, and runs the same plan on the device with the per-op kernels; this module has no CPU executor."""
 from __future__ import annotations
 
 import ctypes
@@ -69,3 +69,9 @@ def run_plan(plan, log=None, progress_every=0.0, stream=None, executor="fused"):
     # "threads" is the key of the reference's stats (its OpenMP thread count); here the device's compute units
     return final_value(raw), {"seconds": seconds, "peak_bits": plan.peak_bits, "cus": cus.value,
                               "threads": cus.value, "executor": executor}
+
+
+def run_plan_numpy(plan):
+    """The value alone, as the reference's run_plan_numpy returns it (None if infeasible, OverflowError at 255),
+    computed on the GPU by the per-op executor."""
+    return run_plan(plan, executor="per-op")[0]
