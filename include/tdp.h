@@ -318,6 +318,7 @@ int tdp_sp_bounds(const uint64_t *h_words, int ncols, int nrows, const int32_t *
 #define TDP_SPB_STATES 2      /* a table or changed list exceeded max_states (BudgetExceeded "states") */
 #define TDP_SPB_NOMEM 3       /* the arena could not hold the query's tables */
 #define TDP_SPB_CANCELLED 4   /* stopped by tdp_spb_cancel */
+#define TDP_SPB_FAILED 5      /* a device-side check failed for this job (tdp_last_error); the others go on */
 
 /* NULL on failure (tdp_last_error). */
 void *tdp_spb_create(uint64_t arena_bytes, int max_jobs, void *stream);

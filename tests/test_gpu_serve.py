@@ -71,3 +71,25 @@ def test_sparse_requests_beyond_the_engine_slots():
         assert rec["distance"] == want["distance"] and rec["observable"] == want["observable"]
         if rec["distance"] is not None:
             assert rec["witness_ok"]
+
+
+def test_unplannable_dense_request_fails_alone():
+    """A dense request whose plan needs more than 63 active rows (a dense random 80 x 200 check matrix) is answered
+    with the planner's ValueError; the good requests claimed with it are answered normally."""
+    import random
+    print("case serve unplannable request", flush=True)
+    rng = random.Random(5)
+    bad = problem.Problem("wide", 80, 1, [rng.getrandbits(80) | 1 for _ in range(200)],
+                          [rng.getrandbits(1) for _ in range(200)])
+    good = [workloads.banded_instance(np.random.default_rng(s), 9300 + s, 10, s % 2 == 1) for s in range(6)]
+    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=1, max_batch=16, sparse_arena_bytes=1 << 30)
+    try:
+        tickets = [srv.submit_dense(p) for p in good[:3]] + [srv.submit_dense(bad)] + \
+                  [srv.submit_dense(p) for p in good[3:]]
+        for tk, p in zip(tickets[:3] + tickets[4:], good):
+            rec = tk.wait(timeout=300)
+            assert rec["distance"] == auto.certify(p)["distance"] and rec["witness_ok"]
+        with pytest.raises(ValueError):
+            tickets[3].wait(timeout=300)
+    finally:
+        srv.close()

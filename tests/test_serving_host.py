@@ -144,3 +144,94 @@ def test_a_query_out_of_memory_cancels_the_others_and_fails_the_request():
     req.finish(eng)
     assert isinstance(req.error, MemoryError) and req.record is None and not req.pending
     assert sorted(eng.released) == [0, 1, 2]
+
+
+class _BrokenEngine:
+    """A stand-in engine whose step raises: the engine thread must answer every sparse ticket with the error (the
+    running ones, the queued ones and any submitted afterwards) and never leave a waiter hanging."""
+    max_jobs = 64
+    max_states = None
+
+    def stats(self):
+        return {"in_use": 0, "arena_bytes": 1 << 30, "nomem_last": 0}
+
+    def add_words(self, prm, tm, N, K):
+        self.n += 1
+        return self.n
+
+    def __init__(self, fail_step):
+        self.n, self.fail_step = 0, fail_step
+
+    def step(self):
+        if self.fail_step:
+            raise RuntimeError("device lost")
+        return 1
+
+    def poll(self):
+        raise RuntimeError("device lost")
+
+    def cancel(self, rid):
+        pass
+
+    def release(self, rid):
+        pass
+
+
+def _bare_server(engine):
+    """A Server with its queues and engine thread only (no device, no dense lanes)."""
+    import collections
+    srv = object.__new__(serve.Server)
+    srv.cv = threading.Condition()
+    srv.sparse_q, srv.dense_q, srv.single_q = collections.deque(), collections.deque(), collections.deque()
+    srv.sparse_failed, srv.closed, srv.done, srv.in_use = None, False, [], 0
+    srv.sparse_engine, srv.sparse_admit = engine, 0.5
+    return srv
+
+
+def _expect_error(tk, timeout):
+    try:
+        tk.wait(timeout=timeout)
+    except RuntimeError as exc:
+        assert "device lost" in str(exc)
+    else:
+        raise AssertionError("a request on a failed engine was answered with a record")
+
+
+def test_engine_thread_failure_answers_every_sparse_ticket():
+    """A failure outside the step (here the poll) stops the engine thread: every ticket gets the error."""
+    srv = _bare_server(_BrokenEngine(fail_step=False))
+    p = problem.quasi_cyclic(3, 3, seed=1)
+    first = [srv.submit_sparse(p, 4) for _ in range(3)]
+    th = threading.Thread(target=srv._sparse_engine_loop, args=(None,), daemon=True)
+    th.start()
+    for tk in first:
+        _expect_error(tk, 30)
+    th.join(timeout=30)
+    assert not th.is_alive() and isinstance(srv.sparse_failed, RuntimeError)
+    _expect_error(srv.submit_sparse(p, 4), 1)     # after the failure: answered at once
+
+
+def test_failed_step_answers_running_requests_and_keeps_serving():
+    srv = _bare_server(_BrokenEngine(fail_step=True))
+    p = problem.quasi_cyclic(3, 3, seed=1)
+    th = threading.Thread(target=srv._sparse_engine_loop, args=(None,), daemon=True)
+    th.start()
+    for tk in [srv.submit_sparse(p, 4) for _ in range(3)]:
+        _expect_error(tk, 30)
+    _expect_error(srv.submit_sparse(p, 4), 30)    # a later request is admitted and answered too
+    with srv.cv:
+        srv.closed = True
+        srv.cv.notify_all()
+    th.join(timeout=30)
+    assert not th.is_alive() and srv.sparse_failed is None
+
+
+def test_ticket_wait_times_out():
+    tk = serve.Ticket("dense", None, None, None)
+    t0 = time.time()
+    try:
+        tk.wait(timeout=0.05)
+    except TimeoutError:
+        assert time.time() - t0 < 5
+    else:
+        raise AssertionError("wait returned without an answer")

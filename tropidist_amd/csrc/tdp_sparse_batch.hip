@@ -529,9 +529,11 @@ public:
             if (j->joined) {
                 j->joined = false;
                 unsigned long long tot = c[3];
-                if (tot != ~0ull && (j->best < 0 || (int)tot < j->best)) {
-                    int rc = pick(j, (int)tot);
-                    if (rc) return rc;
+                if (tot != ~0ull && (j->best < 0 || (int)tot < j->best) && pick(j, (int)tot) != TDP_OK) {
+                    // this job alone fails (its error stays in tdp_last_error); the step goes on for the others, so
+                    // every stepped job still gets its new table indexed and its counters cleared below
+                    fail(j, TDP_SPB_FAILED);
+                    continue;
                 }
             }
             if (j->phase == kFwLast) {
@@ -543,7 +545,8 @@ public:
             if (!j->stepped) continue;
             if (c[2]) {
                 set_error("sparse batch: step output past its capacity (job %lld)", (long long)j->id);
-                return TDP_EHIP;
+                fail(j, TDP_SPB_FAILED);
+                continue;
             }
             j->nout = c[0];
             j->nch = c[1];

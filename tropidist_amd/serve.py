@@ -49,7 +49,9 @@ class Ticket:
         self.footprint = 0
 
     def wait(self, timeout=None):
-        self._event.wait(timeout)
+        """The request's record; raises its error, or TimeoutError when it is not answered within `timeout`."""
+        if not self._event.wait(timeout):
+            raise TimeoutError("request not answered within %s s" % timeout)
         if self.error is not None:
             raise self.error
         return self.result
@@ -88,6 +90,7 @@ class Server:
         self.share_above = 64                   # queue length from which a lane leaves a share to idle lanes
         self.dense_q = collections.deque()
         self.sparse_q = collections.deque()
+        self.sparse_failed = None               # the error that stopped the engine thread, if any
         self.cv = threading.Condition()
         self.closed = False
         self.done = []
@@ -122,8 +125,12 @@ class Server:
     def submit_sparse(self, problem, K, order=None):
         tk = Ticket("sparse", problem, order, K)
         with self.cv:
-            self.sparse_q.append(tk)
-            self.cv.notify_all()
+            failed = self.sparse_failed
+            if failed is None:
+                self.sparse_q.append(tk)
+                self.cv.notify_all()
+        if failed is not None:                  # the engine thread stopped: answered at once with its error
+            self._finish([tk], error=failed)
         return tk
 
     def close(self):
@@ -176,8 +183,14 @@ class Server:
                     return
                 n = claim_count(len(self.dense_q), self.idle_lanes, self.max_batch, self.share_above)
                 claimed = [self.dense_q.popleft() for _ in range(n)]
-            for tk in claimed:                    # host planning outside the lock
-                self._plan(tk)
+            planned = []
+            for tk in claimed:                    # host planning outside the lock, one request at a time
+                try:
+                    planned.append(self._plan(tk))
+                except Exception as exc:          # this request alone is answered with its error
+                    tk.plans, tk.footprint = None, 0
+                    self._finish([tk], error=exc)
+            claimed = planned
             while claimed:
                 with self.cv:
                     # admit the longest prefix that fits the free budget (at least one request when idle)
@@ -225,84 +238,98 @@ class Server:
                 self._finish([r.tag], [r.record])
 
         try:
-            while True:
-                with self.cv:
-                    while not self.sparse_q and not owner and not waiting and not self.closed:
-                        self.cv.wait()
-                    if self.closed and not self.sparse_q and not owner and not waiting:
-                        return
-                    new = list(self.sparse_q)
-                    self.sparse_q.clear()
-                now = time.perf_counter()
-                for tk in new:                      # host planning (C++ trellis words) as the requests arrive
-                    tk.t_start = now
-                    r = sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk)
-                    try:
-                        r.plan()
-                        if len(r.trs) > eng.max_jobs:   # more queries than the engine has slots: one at a time
-                            self._to_single(r)
-                            continue
-                        elif r.trs:
-                            waiting.append(r)
-                            continue
-                        else:
-                            r.finish(eng)
-                    except Exception as exc:
-                        r.error = exc
-                    complete(r)
-                st = eng.stats()
-                while waiting and (not running or (st["in_use"] < self.sparse_admit * st["arena_bytes"] and
-                                                   not st["nomem_last"] and
-                                                   len(owner) + len(waiting[0].trs) <= eng.max_jobs)):
-                    r = waiting.popleft()
-                    try:
-                        r.admit(eng, owner)
-                    except Exception as exc:        # answered with the error; queries already added drain first
-                        r.error = exc
-                        if not r.ids:
-                            complete(r)
-                            continue
-                    running.add(r)
-                    st = eng.stats()
-                if not owner:
-                    continue
-                try:
-                    eng.step()                      # on the engine's own stream
-                except Exception as exc:            # a failed step ends every running request with its error
-                    for r in list(running):
-                        for rid in r.ids:
-                            owner.pop(rid, None)
-                        r.error = exc
-                        complete(r)
-                    running.clear()
-                    continue
-                for rid in eng.poll():
-                    r = owner.pop(rid, None)
-                    if r is None:                   # a query of a request already answered with an error
-                        eng.release(rid)
-                        continue
-                    r.collect(eng, rid)
-                    r.left -= 1
-                    if r.left:
-                        continue
-                    running.discard(r)
-                    alone = not running
-                    r.finish(eng)
-                    if r.pending:                   # no solution at this threshold: admitted again for the next
-                        waiting.appendleft(r)
-                        continue
-                    if isinstance(r.error, MemoryError):
-                        r.error = None
-                        if not alone:
-                            waiting.appendleft(r)   # again when memory has freed
-                            continue
-                        self._to_single(r)          # too large for the arena alone: checkpointed sweeps
-                        continue
-                    complete(r)
+            self._engine_rounds(eng, owner, waiting, running, complete)
+        except Exception as exc:                # the engine itself failed: nothing more runs on it
+            with self.cv:
+                self.sparse_failed = exc
+                queued = list(self.sparse_q)
+                self.sparse_q.clear()
+            for tk in queued:
+                self._finish([tk], error=exc)
+            for r in list(owner.values()) + list(waiting) + list(running):
+                if r.tag.t_done is None:
+                    self._finish([r.tag], error=exc)
         finally:
             for r in list(owner.values()) + list(waiting):
                 if r.tag.t_done is None:
                     self._finish([r.tag], error=RuntimeError("server closed"))
+
+    def _engine_rounds(self, eng, owner, waiting, running, complete):
+        """The engine thread's loop (see _sparse_engine_loop); returns when the server closes."""
+        while True:
+            with self.cv:
+                while not self.sparse_q and not owner and not waiting and not self.closed:
+                    self.cv.wait()
+                if self.closed and not self.sparse_q and not owner and not waiting:
+                    return
+                new = list(self.sparse_q)
+                self.sparse_q.clear()
+            now = time.perf_counter()
+            for tk in new:                      # host planning (C++ trellis words) as the requests arrive
+                tk.t_start = now
+                r = sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk)
+                try:
+                    r.plan()
+                    if len(r.trs) > eng.max_jobs:   # more queries than the engine has slots: one at a time
+                        self._to_single(r)
+                        continue
+                    elif r.trs:
+                        waiting.append(r)
+                        continue
+                    else:
+                        r.finish(eng)
+                except Exception as exc:
+                    r.error = exc
+                complete(r)
+            st = eng.stats()
+            while waiting and (not running or (st["in_use"] < self.sparse_admit * st["arena_bytes"] and
+                                               not st["nomem_last"] and
+                                               len(owner) + len(waiting[0].trs) <= eng.max_jobs)):
+                r = waiting.popleft()
+                try:
+                    r.admit(eng, owner)
+                except Exception as exc:        # answered with the error; queries already added drain first
+                    r.error = exc
+                    if not r.ids:
+                        complete(r)
+                        continue
+                running.add(r)
+                st = eng.stats()
+            if not owner:
+                continue
+            try:
+                eng.step()                      # on the engine's own stream
+            except Exception as exc:            # a failed step ends every running request with its error
+                for r in list(running):
+                    for rid in r.ids:
+                        owner.pop(rid, None)
+                    r.error = exc
+                    complete(r)
+                running.clear()
+                continue
+            for rid in eng.poll():
+                r = owner.pop(rid, None)
+                if r is None:                   # a query of a request already answered with an error
+                    eng.release(rid)
+                    continue
+                r.collect(eng, rid)
+                r.left -= 1
+                if r.left:
+                    continue
+                running.discard(r)
+                alone = not running
+                r.finish(eng)
+                if r.pending:                   # no solution at this threshold: admitted again for the next
+                    waiting.appendleft(r)
+                    continue
+                if isinstance(r.error, MemoryError):
+                    r.error = None
+                    if not alone:
+                        waiting.appendleft(r)   # again when memory has freed
+                        continue
+                    self._to_single(r)          # too large for the arena alone: checkpointed sweeps
+                    continue
+                complete(r)
 
     def _to_single(self, r):
         with self.cv:

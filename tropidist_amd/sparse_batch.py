@@ -26,7 +26,7 @@ from . import _lib, sparse_dp
 from .problem import verify_witness
 
 M64 = (1 << 64) - 1
-RUNNING, DONE, STATES, NOMEM, CANCELLED = 0, 1, 2, 3, 4
+RUNNING, DONE, STATES, NOMEM, CANCELLED, FAILED = 0, 1, 2, 3, 4, 5
 
 
 def trellis_words(tr):
@@ -244,7 +244,7 @@ class Request:
             self.results[rid] = res
             if res[1] is None:
                 eng.release(rid)
-            if res[0] in (NOMEM, STATES):         # the request fails: its queries still running stop now
+            if res[0] in (NOMEM, STATES, FAILED):  # the request fails: its queries still running stop now
                 for other in self.ids:
                     if other not in self.results:
                         eng.cancel(other)
@@ -268,6 +268,9 @@ class Request:
                                                    % (i, eng.max_states), "states")
             if any(st == NOMEM for _, _, st, _, _ in res):
                 raise MemoryError("sparse batch arena full")
+            if any(st == FAILED for _, _, st, _, _ in res):
+                raise RuntimeError("a query of the request failed on the device: %s"
+                                   % _lib.lib().tdp_last_error().decode(errors="replace"))
             if any(st != DONE for _, _, st, _, _ in res):
                 raise RuntimeError("a query of the request was cancelled")
             for i, rid, st, v, info in res:
