@@ -124,18 +124,49 @@ def dominant_passes(progs):
     return kname, top, prog.arr[idx].copy()
 
 
-def measured_traffic(kname, alg_bytes):
-    """HBM bytes per launch of the dominant pass from the committed counter summary, when it was measured on the
-    same kernel and pass geometry; None otherwise."""
-    path = os.path.join(ROOT, "profiles", "pass_d5_banded_w25.json")
+def measured_traffic(name, kernel, alg_bytes):
+    """HBM bytes per launch from a committed counter summary (profiles/<name>.json: rocprofv3 FETCH_SIZE and
+    WRITE_SIZE, scaled by a device copy of known size), when it was measured on the same kernel and algorithmic
+    bytes per launch (within 0.1%); None otherwise."""
+    path = os.path.join(ROOT, "profiles", name + ".json")
     try:
         with open(path) as f:
             prof = json.load(f)
     except (OSError, ValueError):
         return None
-    if kname + "(" not in prof.get("kernel", "") or prof.get("alg_bytes_per_launch") != alg_bytes:
+    a = prof.get("alg_bytes_per_launch") or 0
+    if kernel not in prof.get("kernel", "") or abs(a - alg_bytes) > 1e-3 * alg_bytes:
         return None
     return prof["traffic_bytes_per_launch"]
+
+
+def step_roofline(plans, stream, step_seconds):
+    """Roofline of the step's dominant kernel, k_pass_multi (one launch per round: the next pass of each of the
+    four programs): one more step with HIP events recorded on the step's stream around every round
+    (tdp_run_programs_timed), and each round's algorithmic bytes (every input entry read once, every output entry
+    written once).  achieved = all rounds' bytes / all rounds' time; the widest rounds alone beside it."""
+    import numpy as np
+    from tropidist_amd import fused
+    timing = {}
+    fused.run_many(plans, stream=stream, timing=timing)
+    us, by = timing["round_us"], timing["round_bytes"]
+    kernel_s, alg = float(us.sum()) * 1e-6, float(by.sum())
+    wide = by >= 0.999 * by.max()
+    wide_s = float(us[wide].sum()) * 1e-6
+    return {"bound": "hbm", "achieved": alg / kernel_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+            "frac": alg / kernel_s / HBM_PEAK,
+            "traffic": measured_traffic("step_pass_multi_banded_w25", "k_pass_multi", alg / len(us)),
+            "traffic_source": "profiles/step_pass_multi_banded_w25.json (rocprofv3 FETCH_SIZE / WRITE_SIZE of every "
+                              "k_pass_multi launch of one step, scaled by a device copy of 2^26 bytes)",
+            "kernel": "tdp::k_pass_multi", "launches": int(len(us)), "alg_bytes_per_launch": alg / len(us),
+            "launch_seconds": kernel_s / len(us), "kernel_seconds_per_step": kernel_s,
+            "kernel_share_of_step": kernel_s / step_seconds, "step_alg_bytes": alg,
+            "step_frac": alg / step_seconds / HBM_PEAK,
+            "widest_rounds": {"launches": int(wide.sum()), "alg_bytes_per_launch": float(by.max()),
+                              "launch_seconds": wide_s / max(1, int(wide.sum())),
+                              "frac": float(by[wide].sum()) / wide_s / HBM_PEAK},
+            "round_bytes_histogram": {str(int(b)): int(c) for b, c in zip(*np.unique(by, return_counts=True))
+                                      if c >= 8}}
 
 
 def bench_dense(args, L, torch):
@@ -174,9 +205,9 @@ def bench_dense(args, L, torch):
         per_op_values.append(kernel.final_value(v.value))
     per_op_seconds = time.perf_counter() - t2
 
-    # roofline of the dominant kernel: the widest fused passes of observable 0 (all of them that run the dominant
-    # kernel), each replayed on its own input (T0 -> T1), the average launch; algorithmic bytes = every input entry
-    # read once + every output entry written once
+    roof = step_roofline(plans, stream, elapsed / args.steps)
+    # beside it: the widest fused passes of observable 0 alone, each replayed on its own input (T0 -> T1) by the
+    # single-program kernel, the average launch
     kname, pass_bytes, descs = dominant_passes(progs)
     d = descs[0]
     _lib.check(L.tdp_fill(_lib.dptr(T0), 1 << int(d["b_in"]), 7, sh), "fill")
@@ -188,7 +219,7 @@ def bench_dense(args, L, torch):
     n = 1 << peak
     masks = [int(m) for c, m, _ in encoded[0].tolist() if c == 1 and (n >> 1) <= m < n][:1] or [n >> 1]
     upd_s = _time_launches(torch, stream, lambda: _lib.check(L.tdp_update_w(_lib.dptr(T0), n, masks[0], 2, sh),
-                                                            "update"), args.roofline_launches)
+                                                            "update"), max(1, args.roofline_launches))
     del T0, T1
     dram = dram_pass(args, L, torch, stream, sh)
     return p, plans, values, {
@@ -204,18 +235,16 @@ def bench_dense(args, L, torch):
                    "per_op_executor": {"states_per_s": total_work / per_op_seconds,
                                        "ms_per_step": 1e3 * per_op_seconds, "distances": per_op_values,
                                        "identical": per_op_values == values}},
-        "roofline": {"bound": "hbm", "achieved": pass_bytes / pass_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": pass_bytes / pass_s / HBM_PEAK, "traffic": measured_traffic(kname, pass_bytes),
-                     "traffic_source": "profiles/pass_d5_banded_w25.json (rocprofv3 FETCH_SIZE / WRITE_SIZE over the "
-                                       "same 59 passes, scaled by a device copy of 2^26 bytes)",
-                     "kernel": kname,
-                     "launch_seconds": pass_s, "alg_bytes_per_launch": pass_bytes,
-                     "passes": {"count": len(descs), "b_in": int(d["b_in"]), "b_out": int(d["b_out"]),
-                                "updates": [int(x["D"]) + int(x["nlow"]) for x in descs[:1]][0],
-                                "threads": 1 << int(d["nfree"]), "launches_each": per},
-                     "per_op_update": {"kernel": "tdp::k_update_hi", "table_entries": n, "launch_seconds": upd_s,
-                                       "achieved": 2 * n / upd_s / 1e9, "frac": 2 * n / upd_s / HBM_PEAK},
-                     "dram": dram},
+        "roofline": dict(roof, pass_replay={
+            "kernel": kname, "launch_seconds": pass_s, "alg_bytes_per_launch": pass_bytes,
+            "achieved": pass_bytes / pass_s / 1e9, "frac": pass_bytes / pass_s / HBM_PEAK,
+            "traffic": measured_traffic("pass_d5_banded_w25", kname + "(", pass_bytes),
+            "passes": {"count": len(descs), "b_in": int(d["b_in"]), "b_out": int(d["b_out"]),
+                       "updates": [int(x["D"]) + int(x["nlow"]) for x in descs[:1]][0],
+                       "threads": 1 << int(d["nfree"]), "launches_each": per}},
+            per_op_update={"kernel": "tdp::k_update_hi", "table_entries": n, "launch_seconds": upd_s,
+                           "achieved": 2 * n / upd_s / 1e9, "frac": 2 * n / upd_s / HBM_PEAK},
+            dram=dram),
     }
 
 
@@ -516,6 +545,23 @@ def bench_serving(args, requests):
             "witnesses_ok": ok, "same_distances_both_runs": same}
 
 
+def profile_step(args):
+    """For hardware counters: N headline steps (the four fused programs as multi-program rounds, k_pass_multi), then
+    N torch device copies of 2^26 bytes (the reference kernel for scaling the byte counters)."""
+    from tropidist_amd import _lib, fused
+    torch = _lib.torch()
+    L = _lib.lib()
+    p, plans, progs, peak, T0, T1, _, _ = dense_setup(L, torch)
+    stream = torch.cuda.current_stream()
+    for _ in range(args.profile_step):
+        values = fused.run_many(plans, stream=stream)
+    src, dst = T0[:1 << 26], T1[:1 << 26]
+    for _ in range(args.profile_step):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"steps": args.profile_step, "distances": values, "copy_bytes": 1 << 26}))
+
+
 def profile_pass(args):
     """Replays for hardware counters: the dominant pass (T0 -> T1) and a torch device copy of 2^b_in bytes."""
     import ctypes
@@ -580,6 +626,8 @@ def main():
     ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
                     help="only replay the dominant fused pass N times and a device copy of the same input size N "
                          "times (the reference kernel for scaling hardware byte counters), then exit")
+    ap.add_argument("--profile-step", type=int, default=0, metavar="N",
+                    help="only run N headline steps and N device copies of 2^26 bytes (for counter runs), then exit")
     ap.add_argument("--profile-width", type=int, default=25,
                     help="band window of the instance whose widest pass --profile-pass replays (25: the headline, "
                          "2^26 tables; 27: 2^28 tables, larger than the MALL)")
@@ -591,6 +639,9 @@ def main():
 
     if args.profile_pass:
         profile_pass(args)
+        return
+    if args.profile_step:
+        profile_step(args)
         return
     def progress(msg):
         print("bench: %s (%.1f s)" % (msg, time.time() - T_START), file=sys.stderr, flush=True)

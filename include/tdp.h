@@ -129,6 +129,13 @@ int tdp_run_record_bytes(void);
 int tdp_run_programs_workspace(const tdp_run_t *runs, int nruns, uint64_t *bytes);
 int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
                      uint64_t staging_bytes, void *stream);
+/* tdp_run_programs with HIP events recorded on `stream` around each of the first `cap` multi-program launches:
+ * round_us[k] = the launch's duration in microseconds, round_bytes[k] = its algorithmic bytes (sum over its pass
+ * jobs of 2^b_in + 2^b_out: every input entry read once, every output entry written once); *nrounds = rounds
+ * recorded.  For measurement; the results are those of tdp_run_programs. */
+int tdp_run_programs_timed(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
+                           uint64_t staging_bytes, void *stream, double *round_us, double *round_bytes, int64_t cap,
+                           int64_t *nrounds);
 
 /* Host compiler of fused programs (no device work): the segments (capacity seg_cap triples), pass descriptors
  * (capacity pass_cap records) and, when steps is not NULL, the witness steps (capacity step_cap records) of an

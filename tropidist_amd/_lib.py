@@ -35,6 +35,7 @@ SIGNATURES = {
     "tdp_rank_profile": (ci, [vp, ci, ci, vp, ci, vp]),
     "tdp_run_programs_workspace": (ci, [vp, ci, ctypes.POINTER(u64)]),
     "tdp_run_programs": (ci, [vp, ci, vp, u64, vp, u64, vp]),
+    "tdp_run_programs_timed": (ci, [vp, ci, vp, u64, vp, u64, vp, vp, vp, i64, ctypes.POINTER(i64)]),
     "tdp_fused_compile": (ci, [vp, i64, ci, ci, ci, vp, i64, ctypes.POINTER(i64), vp, i64, ctypes.POINTER(i64), vp,
                                i64, ctypes.POINTER(i64)]),
     "tdp_program_witness_bytes": (ci, [vp, i64, ctypes.POINTER(u64)]),

@@ -1022,8 +1022,32 @@ int tdp_run_programs_workspace(const tdp_run_t *runs, int nruns, uint64_t *bytes
     return TDP_OK;
 }
 
+// Per-round timing of tdp_run_programs_timed: HIP events around every multi-program launch, on its stream.
+struct RoundTiming {
+    double *us, *bytes;
+    int64_t cap, *n;
+};
+
+static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
+                        uint64_t staging_bytes, void *stream, const RoundTiming *timing);
+
 int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
                      uint64_t staging_bytes, void *stream) {
+    return run_programs(runs, nruns, d_work, work_bytes, host_staging, staging_bytes, stream, nullptr);
+}
+
+int tdp_run_programs_timed(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
+                           uint64_t staging_bytes, void *stream, double *round_us, double *round_bytes, int64_t cap,
+                           int64_t *nrounds) {
+    TDP_REQUIRE(round_us && round_bytes && nrounds && cap >= 0, "tdp_run_programs_timed: bad timing arrays");
+    RoundTiming t{round_us, round_bytes, cap, nrounds};
+    return run_programs(runs, nruns, d_work, work_bytes, host_staging, staging_bytes, stream, &t);
+}
+
+}  // extern "C"
+
+static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
+                        uint64_t staging_bytes, void *stream, const RoundTiming *timing) {
     hipStream_t st = as_stream(stream);
     Schedule S;
     int rc = build_schedule(runs, nruns, S);
@@ -1074,6 +1098,23 @@ int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_
     }
     const PassArgs *d_descs = reinterpret_cast<const PassArgs *>(d_work);
     const Job *d_jobs = reinterpret_cast<const Job *>(d_work + S.off_jobs);
+    // timing: the algorithmic bytes of each round (every input entry read once, every output entry written once)
+    // and a pair of events around its launch
+    std::vector<hipEvent_t> evs;
+    std::vector<double> rbytes;
+    std::vector<const PassArgs *> desc_of;
+    if (timing) {
+        desc_of.resize(S.ndescs);
+        for (int r = 0; r < nruns; r++)
+            for (int64_t k = 0; k < runs[r].npasses; k++)
+                desc_of[S.desc_base[r] + k] = static_cast<const PassArgs *>(runs[r].passes) + k;
+    }
+    struct EventGuard {
+        std::vector<hipEvent_t> &e;
+        ~EventGuard() {
+            for (hipEvent_t x : e) (void)hipEventDestroy(x);
+        }
+    } guard{evs};
     for (const Action &act : S.actions) {
         const tdp_run_t &R = runs[act.run];
         if (act.kind == 0) {
@@ -1084,8 +1125,25 @@ int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_
             rc = run_witness_step(R.ops, static_cast<const WStep *>(R.steps)[act.a], R.d_arena,
                                   static_cast<const PassArgs *>(R.passes), st);
         } else {
+            const bool timed = timing && (int64_t)rbytes.size() < timing->cap;
+            if (timed) {
+                double b = 0;
+                for (int64_t j = act.a; j < act.a + act.b; j++)
+                    if (S.jobs[j].desc != kOpJob) {
+                        const PassArgs *pa = desc_of[S.jobs[j].desc];
+                        b += (double)(1ull << pa->b_in) + (double)(1ull << pa->b_out);
+                    }
+                rbytes.push_back(b);
+                hipEvent_t e0, e1;
+                TDP_HIP_CHECK(hipEventCreate(&e0));
+                evs.push_back(e0);
+                TDP_HIP_CHECK(hipEventCreate(&e1));
+                evs.push_back(e1);
+                TDP_HIP_CHECK(hipEventRecord(e0, st));
+            }
             k_pass_multi<<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, (uint32_t)act.b);
             TDP_LAUNCH_CHECK();
+            if (timed) TDP_HIP_CHECK(hipEventRecord(evs.back(), st));
         }
         if (rc) return rc;
     }
@@ -1115,6 +1173,15 @@ int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_
         }
     }
     TDP_HIP_CHECK(hipStreamSynchronize(st));
+    if (timing) {
+        for (size_t k = 0; k < rbytes.size(); k++) {
+            float ms = 0;
+            TDP_HIP_CHECK(hipEventElapsedTime(&ms, evs[2 * k], evs[2 * k + 1]));
+            timing->us[k] = 1e3 * (double)ms;
+            timing->bytes[k] = rbytes[k];
+        }
+        *timing->n = (int64_t)rbytes.size();
+    }
     for (size_t k = 0; k < S.value_runs.size(); k++) runs[S.value_runs[k]].value = vals[k];
     for (size_t k = 0; k < S.witness_runs.size(); k++) {
         tdp_run_t &R = runs[S.witness_runs[k]];
@@ -1131,4 +1198,4 @@ int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_
     return TDP_OK;
 }
 
-}  // extern "C"
+

@@ -851,11 +851,13 @@ def _staging(nbytes):
     return buf
 
 
-def run_many(plans, witness=None, stream=None, min_bits=2):
+def run_many(plans, witness=None, stream=None, min_bits=2, timing=None):
     """Run many feasible plans together by tdp_run_programs (one launch per round runs the next pass of every
     program).  witness[i] true: a witness run, result (value, sorted used op indices); else the value.  Values of
     255 or more raise OverflowError.  ValueError if a plan's column exceeds the pass limits (run it alone).
-    min_bits: value runs use passes from 2^min_bits entries up (program_for); smaller tables run op by op."""
+    min_bits: value runs use passes from 2^min_bits entries up (program_for); smaller tables run op by op.
+    timing (a dict): filled with "round_us" and "round_bytes" (numpy arrays, one entry per multi-program launch:
+    its duration by HIP events on the stream and its algorithmic bytes), by tdp_run_programs_timed."""
     import ctypes
     from . import _lib, kernel
     from .trellis import as_plan
@@ -898,8 +900,16 @@ def run_many(plans, witness=None, stream=None, min_bits=2):
     _lib.check(L.tdp_run_programs_workspace(runs, len(plans), ctypes.byref(nb)), "tdp_run_programs_workspace")
     work = _lib.device_empty(nb.value)
     stage = _staging(nb.value)
-    _lib.check(L.tdp_run_programs(runs, len(plans), _lib.dptr(work), nb.value, stage.data_ptr(), stage.numel(),
-                                  _lib.stream_handle(stream)), "tdp_run_programs")
+    if timing is None:
+        _lib.check(L.tdp_run_programs(runs, len(plans), _lib.dptr(work), nb.value, stage.data_ptr(), stage.numel(),
+                                      _lib.stream_handle(stream)), "tdp_run_programs")
+    else:
+        cap = sum(int(getattr(r, "npasses")) for r in runs) + len(plans) + 16
+        us, by, nr = np.zeros(cap), np.zeros(cap), ctypes.c_int64(0)
+        _lib.check(L.tdp_run_programs_timed(runs, len(plans), _lib.dptr(work), nb.value, stage.data_ptr(),
+                                            stage.numel(), _lib.stream_handle(stream), us.ctypes.data,
+                                            by.ctypes.data, cap, ctypes.byref(nr)), "tdp_run_programs_timed")
+        timing["round_us"], timing["round_bytes"] = us[:nr.value].copy(), by[:nr.value].copy()
     out = []
     for r, w in enumerate(witness):
         v = kernel.final_value(runs[r].value)
