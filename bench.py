@@ -195,6 +195,21 @@ def bench_dense(args, L, torch):
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t1
 
+    # the same step captured once as a HIP graph (the schedule built, uploaded and recorded outside the timing) and
+    # replayed: the device work of every step again, without the host's per-round launch calls
+    graph = fused.ProgramGraph(plans, stream=stream)
+    graph.launch(stream)
+    graph_values = graph.values(stream)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.launch(stream)
+    torch.cuda.synchronize()
+    graph_s = (time.perf_counter() - t3) / args.steps
+    graph_values_after = graph.values(stream)
+    graph_nodes = graph.nodes
+    graph.close()
+
     # per-op executor on the same step, once, for comparison
     t2 = time.perf_counter()
     per_op_values = []
@@ -232,6 +247,9 @@ def bench_dense(args, L, torch):
                    "passes_per_step": sum(pr.npasses for pr in progs),
                    "executor": "fused passes, the four programs advanced together one pass each per round "
                                "(tdp_run_programs)",
+                   "graph_replay": {"states_per_s": total_work / graph_s, "ms_per_step": 1e3 * graph_s,
+                                    "nodes": graph_nodes, "distances": graph_values,
+                                    "identical": graph_values == values == graph_values_after},
                    "per_op_executor": {"states_per_s": total_work / per_op_seconds,
                                        "ms_per_step": 1e3 * per_op_seconds, "distances": per_op_values,
                                        "identical": per_op_values == values}},
@@ -299,6 +317,7 @@ def bench_batched(args, scan):
     plain_s = time.perf_counter() - t2
     same = all(a["per_observable"] == b["per_observable"] and b["witness_ok"] for a, b in zip(recs[:ns], plain))
     widths = sorted(set(pl.peak_bits for pl in flat if pl.feasible))
+    graph = scan_graph(flat, torch)
     return {"workload": "%d banded scans, 200-1200 detectors, 5-15 faults/detector, width 10-22, 2 logical rows, "
                         "half with costs 1-3" % len(scan),
             "certs_per_s": len(scan) / batched_s, "batched_seconds": batched_s,
@@ -307,7 +326,41 @@ def bench_batched(args, scan):
             "sample_identical_to_plain": same, "states": int(sum(pl.work() for pl in flat if pl.feasible)),
             "states_per_s": sum(pl.work() for pl in flat if pl.feasible) / batched_s,
             "host_planning_seconds": round(plan_s, 3), "peak_bits_range": [widths[0], widths[-1]],
-            "latency_s": batched_s, "repeats_seconds": [round(x, 4) for x in times]}
+            "latency_s": batched_s, "repeats_seconds": [round(x, 4) for x in times], "hipgraph": graph}
+
+
+def scan_graph(flat, torch, repeats=5):
+    """The value programs of the scan's wide plans (2^WIDE_BITS entries and more, the multi-program rounds of the
+    batched path) run by fused.run_many, against the same schedule captured once in a HIP graph and replayed
+    (fused.ProgramGraph): the host time the replays save, median of `repeats` each."""
+    from tropidist_amd import batch, fused
+    wide = [pl for pl in flat if pl.feasible and pl.peak_bits >= batch.WIDE_BITS]
+    if not wide:
+        return None
+    want = fused.run_many(wide)
+    eager = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        fused.run_many(wide)
+        torch.cuda.synchronize()
+        eager.append(time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    g = fused.ProgramGraph(wide)
+    capture_s = time.perf_counter() - t0
+    g.launch()
+    same = g.values() == want
+    replay = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        g.launch()
+        torch.cuda.synchronize()
+        replay.append(time.perf_counter() - t0)
+        same = same and g.values() == want
+    g.close()
+    e, r = sorted(eager)[repeats // 2], sorted(replay)[repeats // 2]
+    return {"workload": "value programs of the %d wide plans of the scan (multi-program rounds)" % len(wide),
+            "graph_nodes": g.nodes, "run_many_seconds": e, "graph_replay_seconds": r, "capture_seconds": capture_s,
+            "saved_seconds": e - r, "saved_per_node_us": 1e6 * (e - r) / max(1, g.nodes), "identical": same}
 
 
 def _balanced(lo, hi):

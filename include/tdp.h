@@ -137,6 +137,19 @@ int tdp_run_programs_timed(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t
                            uint64_t staging_bytes, void *stream, double *round_us, double *round_bytes, int64_t cap,
                            int64_t *nrounds);
 
+/* The launches of one tdp_run_programs call captured once as a HIP graph and replayed: tdp_programs_graph builds and
+ * uploads the schedule (value runs only), records every launch of the call (fills, per-op kernels, rounds, the final
+ * gather) with stream capture on `stream` and instantiates it; nothing runs yet.  tdp_graph_launch replays all of
+ * it with one launch call; tdp_graph_values waits for the stream and reads the final entries of the value runs, in
+ * run order (255 = cost >= 255).  The graph refers to the runs' tables and to d_work, which must outlive it.  For
+ * repeated schedules (a benchmark step, a scan certified again): it takes the host's per-launch cost, one launch
+ * call per round, off the critical path.  No reference counterpart.  tdp_graph_nodes: the node count. */
+void *tdp_programs_graph(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *stream);
+int tdp_graph_launch(void *graph, void *stream);
+int tdp_graph_values(void *graph, int32_t *values, int n, void *stream);
+int tdp_graph_nodes(void *graph, uint64_t *nodes);
+void tdp_graph_free(void *graph);
+
 /* Host compiler of fused programs (no device work): the segments (capacity seg_cap triples), pass descriptors
  * (capacity pass_cap records) and, when steps is not NULL, the witness steps (capacity step_cap records) of an
  * encoded op stream.  dmax <= 5 directions per pass, tables below 2^min_bits (>= 2) run per op, max_updates > 0

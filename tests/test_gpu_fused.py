@@ -241,3 +241,24 @@ def test_run_many_overflow():
     good = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
     with pytest.raises(OverflowError):
         fused.run_many([good, bad])
+
+
+def test_program_graph_replays_equal_run_many():
+    """A captured schedule (tdp_programs_graph) replayed several times, with the program's own eager work and
+    synchronisations between the replays: every replay gives run_many's values (widths 8-20, costs 1-3)."""
+    from tropidist_amd import _lib
+    print("case program graph replays", flush=True)
+    plans = []
+    for W, weights in ((8, None), (12, (1, 3)), (16, None), (20, (1, 2))):
+        p = problem.banded(R=100, N=500, kmin=2, kmax=6, W=W, nobs=1, rho=0.1, weights=weights, seed=400 + W)
+        plans.append(trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows, weights=p.w))
+    want = fused.run_many(plans)
+    g = fused.ProgramGraph(plans)
+    assert g.nodes > len(plans)
+    t = _lib.torch()
+    for rep in range(4):
+        g.launch()
+        assert g.values() == want, rep
+        assert fused.run_many(plans[:2]) == want[:2]          # eager work of the same program between replays
+        t.cuda.synchronize()
+    g.close()

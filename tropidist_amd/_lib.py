@@ -35,6 +35,11 @@ SIGNATURES = {
     "tdp_rank_profile": (ci, [vp, ci, ci, vp, ci, vp]),
     "tdp_run_programs_workspace": (ci, [vp, ci, ctypes.POINTER(u64)]),
     "tdp_run_programs": (ci, [vp, ci, vp, u64, vp, u64, vp]),
+    "tdp_programs_graph": (vp, [vp, ci, vp, u64, vp]),
+    "tdp_graph_launch": (ci, [vp, vp]),
+    "tdp_graph_values": (ci, [vp, vp, ci, vp]),
+    "tdp_graph_nodes": (ci, [vp, ctypes.POINTER(u64)]),
+    "tdp_graph_free": (None, [vp]),
     "tdp_run_programs_timed": (ci, [vp, ci, vp, u64, vp, u64, vp, vp, vp, i64, ctypes.POINTER(i64)]),
     "tdp_fused_compile": (ci, [vp, i64, ci, ci, ci, vp, i64, ctypes.POINTER(i64), vp, i64, ctypes.POINTER(i64), vp,
                                i64, ctypes.POINTER(i64)]),

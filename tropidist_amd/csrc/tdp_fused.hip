@@ -1028,8 +1028,22 @@ struct RoundTiming {
     int64_t cap, *n;
 };
 
+// A captured schedule (tdp_programs_graph): the launches of one tdp_run_programs call as a HIP graph, replayed by
+// tdp_graph_launch; the value runs' final entries land at results (device) on every replay.
+struct ProgramGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const uint8_t *results = nullptr;
+    int nvalues = 0;
+    uint64_t nodes = 0;
+    ~ProgramGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
 static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
-                        uint64_t staging_bytes, void *stream, const RoundTiming *timing);
+                        uint64_t staging_bytes, void *stream, const RoundTiming *timing, ProgramGraph *capture = nullptr);
 
 int tdp_run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
                      uint64_t staging_bytes, void *stream) {
@@ -1044,14 +1058,65 @@ int tdp_run_programs_timed(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t
     return run_programs(runs, nruns, d_work, work_bytes, host_staging, staging_bytes, stream, &t);
 }
 
+void *tdp_programs_graph(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *stream) {
+    // capture on a stream of the library's own (the null stream, which callers often pass, cannot be captured),
+    // ordered after the caller's stream so the upload follows the caller's earlier work
+    hipStream_t cs = nullptr;
+    hipEvent_t after = nullptr;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&after) != hipSuccess ||
+        hipEventRecord(after, as_stream(stream)) != hipSuccess || hipStreamWaitEvent(cs, after, 0) != hipSuccess) {
+        set_error("tdp_programs_graph: capture stream setup failed");
+        if (after) (void)hipEventDestroy(after);
+        if (cs) (void)hipStreamDestroy(cs);
+        return nullptr;
+    }
+    ProgramGraph *g = new ProgramGraph();
+    const int rc = run_programs(runs, nruns, d_work, work_bytes, nullptr, 0, cs, nullptr, g);
+    (void)hipStreamSynchronize(cs);
+    (void)hipEventDestroy(after);
+    (void)hipStreamDestroy(cs);
+    if (rc != TDP_OK) {
+        delete g;
+        return nullptr;
+    }
+    return g;
+}
+
+int tdp_graph_launch(void *graph, void *stream) {
+    TDP_REQUIRE(graph, "tdp_graph_launch: null graph");
+    TDP_HIP_CHECK(hipGraphLaunch(static_cast<ProgramGraph *>(graph)->exec, as_stream(stream)));
+    return TDP_OK;
+}
+
+int tdp_graph_values(void *graph, int32_t *values, int n, void *stream) {
+    TDP_REQUIRE(graph && values, "tdp_graph_values: null argument");
+    const ProgramGraph *g = static_cast<ProgramGraph *>(graph);
+    TDP_REQUIRE(n == g->nvalues, "tdp_graph_values: %d values asked, the graph has %d", n, g->nvalues);
+    std::vector<uint8_t> v(g->nvalues > 0 ? g->nvalues : 1);
+    if (g->nvalues) TDP_HIP_CHECK(hipMemcpyAsync(v.data(), g->results, g->nvalues, hipMemcpyDeviceToHost,
+                                                 as_stream(stream)));
+    TDP_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+    for (int k = 0; k < n; k++) values[k] = v[k];
+    return TDP_OK;
+}
+
+int tdp_graph_nodes(void *graph, uint64_t *nodes) {
+    TDP_REQUIRE(graph && nodes, "tdp_graph_nodes: null argument");
+    *nodes = static_cast<ProgramGraph *>(graph)->nodes;
+    return TDP_OK;
+}
+
+void tdp_graph_free(void *graph) { delete static_cast<ProgramGraph *>(graph); }
+
 }  // extern "C"
 
 static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t work_bytes, void *host_staging,
-                        uint64_t staging_bytes, void *stream, const RoundTiming *timing) {
+                        uint64_t staging_bytes, void *stream, const RoundTiming *timing, ProgramGraph *capture) {
     hipStream_t st = as_stream(stream);
     Schedule S;
     int rc = build_schedule(runs, nruns, S);
     if (rc) return rc;
+    TDP_REQUIRE(!capture || S.witness_runs.empty(), "a captured schedule takes value runs only");
     TDP_REQUIRE(d_work && work_bytes >= S.total, "workspace of %llu bytes, %llu needed",
                 (unsigned long long)work_bytes, (unsigned long long)S.total);
     // one upload: descriptors, jobs, final-table pointers, traceback records (through the caller's pinned staging
@@ -1092,6 +1157,22 @@ static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t wo
         memcpy(host.data() + S.off_trace + k * sizeof(TraceJob), &T, sizeof(T));
     }
     TDP_HIP_CHECK(hipMemcpyAsync(d_work, host.data(), S.total, hipMemcpyHostToDevice, st));
+    if (capture) {
+        // the upload stays outside the graph (it is the same on every replay); every launch from here on is
+        // recorded: kernels and device-to-device copies only
+        TDP_HIP_CHECK(hipStreamSynchronize(st));
+        TDP_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    }
+    struct CaptureGuard {               // ends a capture left open by an error return
+        hipStream_t s;
+        bool on;
+        ~CaptureGuard() {
+            if (on) {
+                hipGraph_t g = nullptr;
+                if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+            }
+        }
+    } cguard{st, capture != nullptr};
     for (int r = 0; r < nruns; r++) {
         rc = launch_fill(runs[r].steps ? runs[r].d_arena : runs[r].d_T0, 1, 0, st);
         if (rc) return rc;
@@ -1153,7 +1234,19 @@ static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t wo
         k_gather<<<(nv + 255) / 256, 256, 0, st>>>(reinterpret_cast<const uint8_t *const *>(d_work + S.off_finals),
                                                    nv, d_work + S.off_results);
         TDP_LAUNCH_CHECK();
-        TDP_HIP_CHECK(hipMemcpyAsync(vals.data(), d_work + S.off_results, nv, hipMemcpyDeviceToHost, st));
+        if (!capture)
+            TDP_HIP_CHECK(hipMemcpyAsync(vals.data(), d_work + S.off_results, nv, hipMemcpyDeviceToHost, st));
+    }
+    if (capture) {
+        cguard.on = false;
+        TDP_HIP_CHECK(hipStreamEndCapture(st, &capture->graph));
+        size_t nn = 0;
+        TDP_HIP_CHECK(hipGraphGetNodes(capture->graph, nullptr, &nn));
+        capture->nodes = nn;
+        TDP_HIP_CHECK(hipGraphInstantiate(&capture->exec, capture->graph, nullptr, nullptr, 0));
+        capture->results = d_work + S.off_results;
+        capture->nvalues = (int)S.value_runs.size();
+        return TDP_OK;
     }
     std::vector<int64_t> io(3 * S.witness_runs.size());
     if (!S.witness_runs.empty()) {

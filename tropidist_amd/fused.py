@@ -915,3 +915,65 @@ def run_many(plans, witness=None, stream=None, min_bits=2, timing=None):
         v = kernel.final_value(runs[r].value)
         out.append((v, sorted(int(i) for i in used_of[r][:runs[r].nused])) if w else v)
     return out
+
+
+class ProgramGraph:
+    """Value runs of many plans captured once as one HIP graph (tdp_programs_graph) and replayed by launch(): the
+    same rounds as run_many, with the host's schedule, upload and per-launch calls paid once at capture.  values()
+    waits for the stream and returns the distances in plan order (OverflowError at 255).  The graph owns the plans'
+    device tables and workspace while it lives; close() (or garbage collection) frees it."""
+
+    def __init__(self, plans, stream=None, min_bits=2):
+        import ctypes
+        from . import _lib
+        from .trellis import as_plan
+        L = self._L = _lib.lib()
+        TdpRun = _run_record_type()
+        self._plans = [as_plan(p) for p in plans]
+        runs = (TdpRun * len(self._plans))()
+        self._keep = []
+        for r, plan in enumerate(self._plans):
+            if not plan.feasible:
+                raise ValueError("ProgramGraph takes feasible plans only")
+            ops = np.ascontiguousarray(plan.encode(), dtype=np.uint64)
+            prog = program_for(plan, min_bits=min_bits)
+            T0 = _lib.device_empty(1 << plan.peak_bits)
+            T1 = _lib.device_empty(1 << plan.peak_bits)
+            R = runs[r]
+            R.ops, R.nops, R.peak_bits = ops.ctypes.data, len(ops), plan.peak_bits
+            R.segments, R.nseg = prog.segs.ctypes.data, len(prog.segs)
+            R.passes, R.npasses = prog.arr.ctypes.data, prog.npasses
+            R.d_T0, R.d_T1 = _lib.dptr(T0), _lib.dptr(T1)
+            self._keep += [ops, prog, T0, T1]
+        nb = ctypes.c_uint64(0)
+        _lib.check(L.tdp_run_programs_workspace(runs, len(runs), ctypes.byref(nb)), "tdp_run_programs_workspace")
+        self._work = _lib.device_empty(nb.value)
+        self._h = L.tdp_programs_graph(runs, len(runs), _lib.dptr(self._work), nb.value, _lib.stream_handle(stream))
+        if not self._h:
+            raise _lib.TdpError("tdp_programs_graph failed: " + L.tdp_last_error().decode(errors="replace"),
+                                _lib.TDP_EHIP)
+        n = ctypes.c_uint64(0)
+        _lib.check(L.tdp_graph_nodes(self._h, ctypes.byref(n)), "tdp_graph_nodes")
+        self.nodes = int(n.value)
+
+    def launch(self, stream=None):
+        from . import _lib
+        _lib.check(self._L.tdp_graph_launch(self._h, _lib.stream_handle(stream)), "tdp_graph_launch")
+
+    def values(self, stream=None):
+        from . import _lib, kernel
+        out = np.zeros(len(self._plans), dtype=np.int32)
+        _lib.check(self._L.tdp_graph_values(self._h, out.ctypes.data, len(out), _lib.stream_handle(stream)),
+                   "tdp_graph_values")
+        return [kernel.final_value(v) for v in out]
+
+    def close(self):
+        h, self._h = getattr(self, "_h", None), None
+        if h:
+            self._L.tdp_graph_free(h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
