@@ -611,8 +611,68 @@ def profile_step(args):
     src, dst = T0[:1 << 26], T1[:1 << 26]
     for _ in range(args.profile_step):
         dst.copy_(src)
+    # one more step with per-round HIP events: the rounds' algorithmic bytes (the same launches as the others)
+    timing = {}
+    fused.run_many(plans, stream=stream, timing=timing)
     torch.cuda.synchronize()
-    print(json.dumps({"steps": args.profile_step, "distances": values, "copy_bytes": 1 << 26}))
+    us, by = timing["round_us"], timing["round_bytes"]
+    print(json.dumps({"steps": args.profile_step + 1, "distances": values, "copy_bytes": 1 << 26,
+                      "launches_per_step": int(len(by)), "alg_bytes_per_step": float(by.sum()),
+                      "alg_bytes_per_launch": float(by.mean()), "event_launch_ns": float(us.mean()) * 1e3}))
+
+
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def summarise_step_profile(args):
+    """Write profiles/step_pass_multi_banded_w25.json from the three rocprofv3 runs of --profile-step: the kernel
+    trace stats, FETCH_SIZE and WRITE_SIZE per dispatch.  The byte counters of k_pass_multi are scaled by what they
+    read for the torch copy of 2^26 bytes (the copyBuffer dispatch of the largest grid): scale = bytes moved /
+    counter bytes, per counter."""
+    trace_csv, fetch_csv, write_csv, log = args.summarise_step_profile
+    with open(log) as f:
+        info = [json.loads(line) for line in f if line.startswith("{")][-1]
+    kernel = "k_pass_multi"
+    stats = [r for r in _csv_rows(trace_csv) if kernel in r["Name"]]
+    if len(stats) != 1:
+        raise SystemExit("expected one %s row in %s" % (kernel, trace_csv))
+    st = stats[0]
+
+    def counter(path, name):
+        rows = [r for r in _csv_rows(path) if r["Counter_Name"] == name]
+        mine = [float(r["Counter_Value"]) for r in rows if kernel in r["Kernel_Name"]]
+        copies = [r for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        g = max(int(r["Grid_Size"]) for r in copies)
+        cp = [float(r["Counter_Value"]) for r in copies if int(r["Grid_Size"]) == g]
+        return sum(mine) / len(mine), len(mine), sum(cp) / len(cp), len(cp)
+
+    f_k, f_n, f_c, f_cn = counter(fetch_csv, "FETCH_SIZE")
+    w_k, w_n, w_c, w_cn = counter(write_csv, "WRITE_SIZE")
+    cb = info["copy_bytes"]
+    fetch_scale, write_scale = cb / (f_c * 1024), cb / (w_c * 1024)
+    rd, wr = f_k * 1024 * fetch_scale, w_k * 1024 * write_scale
+    alg = info["alg_bytes_per_launch"]
+    out = {"kernel": st["Name"],
+           "workload": "the headline step: the four observables' fused programs of the banded R=600 N=6000 W=25 "
+                       "instance (seed 2000) as multi-program rounds, one k_pass_multi launch per round",
+           "alg_bytes_per_launch": alg, "launches_per_step": info["launches_per_step"],
+           "alg_bytes_per_step": info["alg_bytes_per_step"],
+           "copy_kernel": {"name": "__amd_rocclr_copyBuffer", "bytes_read": cb, "bytes_written": cb,
+                           "FETCH_SIZE_KiB": f_c, "WRITE_SIZE_KiB": w_c, "fetch_scale": fetch_scale,
+                           "write_scale": write_scale, "dispatches": [f_cn, w_cn]},
+           "step_counters": {"FETCH_SIZE_KiB": f_k, "WRITE_SIZE_KiB": w_k, "launches": [f_n, w_n]},
+           "traffic_bytes_per_launch": rd + wr, "read_bytes": rd, "write_bytes": wr, "traffic_over_alg": (rd + wr) / alg,
+           "trace_calls": int(st["Calls"]), "trace_avg_ns": float(st["AverageNs"]), "trace_min_ns": int(st["MinNs"]),
+           "trace_max_ns": int(st["MaxNs"]), "event_avg_ns": info["event_launch_ns"],
+           "trace_frac": alg / (float(st["AverageNs"]) * 1e-9) / HBM_PEAK}
+    path = os.path.join(ROOT, "profiles", "step_pass_multi_banded_w25.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+        f.write("\n")
+    print(json.dumps(out, indent=1))
 
 
 def profile_pass(args):
@@ -681,6 +741,10 @@ def main():
                          "times (the reference kernel for scaling hardware byte counters), then exit")
     ap.add_argument("--profile-step", type=int, default=0, metavar="N",
                     help="only run N headline steps and N device copies of 2^26 bytes (for counter runs), then exit")
+    ap.add_argument("--summarise-step-profile", nargs=4, metavar=("STATS_CSV", "FETCH_CSV", "WRITE_CSV", "LOG"),
+                    help="write the committed counter summary of k_pass_multi from the rocprofv3 outputs of "
+                         "--profile-step runs (kernel stats, FETCH_SIZE, WRITE_SIZE) and that run's printed line, "
+                         "then exit (no GPU needed)")
     ap.add_argument("--profile-width", type=int, default=25,
                     help="band window of the instance whose widest pass --profile-pass replays (25: the headline, "
                          "2^26 tables; 27: 2^28 tables, larger than the MALL)")
@@ -695,6 +759,9 @@ def main():
         return
     if args.profile_step:
         profile_step(args)
+        return
+    if args.summarise_step_profile:
+        summarise_step_profile(args)
         return
     def progress(msg):
         print("bench: %s (%.1f s)" % (msg, time.time() - T_START), file=sys.stderr, flush=True)
