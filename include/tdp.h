@@ -406,6 +406,40 @@ int tdp_tab_build_noalloc(void *table, int n, const uint64_t *zh, const uint64_t
 int tdp_tab_lookup_batch(void *table, const uint64_t *d_zh, const uint64_t *d_zl, const uint64_t *d_lg,
                          const int *d_R, uint64_t m, int *d_out, void *stream);
 
+/* ------------------------------------------------------------------ weight-bounded search
+ * The anchored canonical search of cdist/gmitm.c:188-1023 on the device, with tdp_tab handles as its tables.  The
+ * options of the next searches are set as in the reference (its gm_set_* globals, here per host thread); every
+ * pointer they keep must stay valid until the search that uses them returns. */
+
+/* Replaces gm_set_cover (gmitm.c:248-250): gate4 > 0 tests the covering bound at nodes with 4|sigma| >= gate4 * R;
+ * ascent: one greedy dual-ascent pass. */
+void tdp_gm_set_cover(int gate4, int ascent);
+/* Replaces gm_set_kernel (gmitm.c:242-244): zero-syndrome zero-logical column sets (CSR), nks = 0 turns it off. */
+void tdp_gm_set_kernel(int nks, const int *ks_start, const int *ks_cols, const int *kc_start, const int *kc_ids);
+/* Replaces gm_set_kernel_mode (gmitm.c:246): 1 records zero-syndrome sets with zero logical instead of solutions. */
+void tdp_gm_set_kernel_mode(int on);
+/* Replaces gm_set_full_labels (gmitm.c:240): label-free mode, the logical value of a complete set from lg (N x lw
+ * words); NULL turns it off. */
+void tdp_gm_set_full_labels(const uint64_t *lg, int lw);
+/* Replaces gm_set_pattern (gmitm.c:252-254): class per column (-1: none), on-pattern flag per class, slack. */
+void tdp_gm_set_pattern(const int *cls, const uint8_t *onpat, int nclass, int slack);
+/* Replaces gm_num_threads (gmitm.c:1025-1031): nodes the device expands at once. */
+int tdp_gm_num_threads(void);
+/* Replaces gm_search (gmitm.c:857-1023); same arguments, host arrays, tabs[] are tdp_tab handles.  *status gets the
+ * reference's return code: 0 exhausted, 1 stopped at the first solution (enum_all = 0), 2 time limit, 4 K > 40.
+ * Codes 3 (too many tasks), 5 and 6 (task ledger) never occur: the device search keeps no task list and no ledger.
+ * nodes_out[0..40]: nodes visited per depth; stats_out[8]: lookups, hits, false hits, nodes at depth task_depth that
+ * reach their expansion (the reference's tasks), cover prunes, ascent prunes, cover tests, solutions found.  Nodes
+ * per depth, hits, false hits, tasks and prunes equal the reference's on an exhausted run; lookups and cover tests
+ * count every probe and test once (the reference counts its tasks' twice and skips probes its prefix phase does not
+ * make).  The returned int is TDP_OK unless the arguments are invalid or the device fails. */
+int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint64_t *lg, const int *row_start,
+                  const int *row_cols, const int *col_start, const int *col_rows, int ntab, void **tabs,
+                  const uint64_t **tzh, const uint64_t **tzl, int main_tab, const int *rank, int n_anchor,
+                  const int *anchors, const int *ex_start, const int *ex_cols, int K, int enum_all, int task_depth,
+                  double time_limit, int64_t *nodes_out, int64_t *stats_out, int *best, int sol_cap, int sol_stride,
+                  int *sol_buf, int *nsol, int *status);
+
 /* ------------------------------------------------------------------ batched dense plans
  * d_items: device records of tdp_batch_item_bytes() bytes (layout in tropidist_amd/csrc/tdp_batch.hip: op stream,
  * peak bits, checkpoint interval, table / checkpoint / decision-map / result / barrier pointers, group size);

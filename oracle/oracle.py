@@ -114,6 +114,15 @@ def rlib(name):
         L.tab_count.restype = i64
         L.tab_count.argtypes = [vp]
         L.tab_build.argtypes = [vp, ci, vp, vp, vp, ci]
+        L.gm_set_cover.argtypes = [ci, ci]
+        L.gm_set_kernel.argtypes = [ci, vp, vp, vp, vp]
+        L.gm_set_kernel_mode.argtypes = [ci]
+        L.gm_set_full_labels.argtypes = [vp, ci]
+        L.gm_set_pattern.argtypes = [vp, vp, ci, ci]
+        L.gm_set_ledger.argtypes = [ctypes.c_char_p, ci, ci]
+        L.gm_search.restype = ci
+        L.gm_search.argtypes = [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, vp, vp, vp, ci, vp, ci, vp, vp, vp, ci,
+                                ci, ci, ctypes.c_double, vp, vp, ctypes.POINTER(ci), ci, ci, vp, ctypes.POINTER(ci)]
     L._typed = True
     return L
 
@@ -530,3 +539,84 @@ def bound_value(state, tmask, hd, Q=720720):
     u = int(state) ^ int(tmask)
     tot = int(extra) + sum(int(w) * bin(u & int(m)).count("1") for m, w in zip(masks, weights))
     return -(-tot // Q)
+
+
+# ---------------------------------------------------------------------------------------------- weight-bounded search
+def have_gmitm():
+    return os.path.exists(os.path.join(HERE, "_ref", "libgmitm.so"))
+
+
+def gm_search_ref(S, K, tables, main, rank, anchors, use_ex=True, enum_all=False, task_depth=1, sol_cap=100000,
+                  cover_gate=0, ascent=1, kernel=None, kernel_mode=False, pattern=None, dmax=40):
+    """The reference's own search (gmitm.c gm_search, compiled from source into oracle/_ref/libgmitm.so) on the
+    arrays of a tropidist_amd.search.Searcher S.  tables: (zh, zl, lg, k, log2slots, wide) per table, built here as
+    the reference's CPU tables with the same hashes.  Returns the record fields the device search reports: status
+    code, best, sorted solutions, nodes per depth and the eight statistics."""
+    L = rlib("gmitm")
+    built = [tab_build(zh, zl, lg, k, log2, wide, impl="ref") for zh, zl, lg, k, log2, wide in tables]
+    keep = [(np.ascontiguousarray(t[0], dtype=np.uint64), np.ascontiguousarray(t[1], dtype=np.uint64))
+            for t in tables]
+    nt = len(tables)
+    handles = (ctypes.c_void_p * max(nt, 1))(*[b[0] for b in built])
+    tzh = (ctypes.c_void_p * max(nt, 1))(*[k[0].ctypes.data for k in keep])
+    tzl = (ctypes.c_void_p * max(nt, 1))(*[k[1].ctypes.data for k in keep])
+    L.gm_set_cover(int(cover_gate), int(ascent))
+    if kernel is not None:
+        nks, a, b, c, d = kernel
+        L.gm_set_kernel(nks, ptr(a), ptr(b), ptr(c), ptr(d))
+    else:
+        L.gm_set_kernel(0, None, None, None, None)
+    L.gm_set_kernel_mode(1 if kernel_mode else 0)
+    L.gm_set_full_labels(ptr(S.full_lg), S.lw if S.full_lg is not None else 0)
+    pkeep = []
+    if pattern is not None:
+        cls = np.ascontiguousarray(pattern[0], dtype=np.int32)
+        onpat = np.ascontiguousarray(pattern[1], dtype=np.uint8)
+        pkeep = [cls, onpat]
+        L.gm_set_pattern(ptr(cls), ptr(onpat), len(onpat), int(pattern[2]))
+    else:
+        L.gm_set_pattern(None, None, 0, 0)
+    L.gm_set_ledger(None, 0, 1)
+    rank = np.ascontiguousarray(rank, dtype=np.int32)
+    anchors = np.ascontiguousarray(anchors, dtype=np.int32)
+    nodes = np.zeros(dmax + 2, dtype=np.int64)
+    stats = np.zeros(8, dtype=np.int64)
+    stride = K + 2
+    sol = np.zeros((max(sol_cap, 1), stride), dtype=np.int32)
+    best, nsol = ctypes.c_int(-1), ctypes.c_int(0)
+    try:
+        rc = L.gm_search(S.W, S.p.ncols, S.p.nrows, S.wmax, ptr(S.col), ptr(S.lg), ptr(S.row_start),
+                         ptr(S.row_cols), ptr(S.col_start), ptr(S.col_rows), nt, handles, tzh, tzl, main, ptr(rank),
+                         len(anchors), ptr(anchors), ptr(S.ex_start) if use_ex else None,
+                         ptr(S.ex_cols) if use_ex else None, K, 1 if enum_all else 0, task_depth, 0.0, ptr(nodes),
+                         ptr(stats), ctypes.byref(best), sol_cap, stride, ptr(sol), ctypes.byref(nsol))
+    finally:
+        L.gm_set_kernel_mode(0)
+        L.gm_set_kernel(0, None, None, None, None)
+        L.gm_set_cover(0, 0)
+        L.gm_set_full_labels(None, 0)
+        L.gm_set_pattern(None, None, 0, 0)
+        for h, _, free in built:
+            free(h)
+    del pkeep
+    sols = sorted(tuple(sorted(int(x) for x in sol[i, 1:1 + int(sol[i, 0])])) for i in range(min(nsol.value, sol_cap)))
+    return {"status": rc, "best": None if best.value < 0 else best.value, "solutions": sols,
+            "nodes": [int(x) for x in nodes[:K + 2]], "stats": [int(x) for x in stats]}
+
+
+def brute_all(h, l):
+    """Minimum weight and all minimum-weight supports with zero syndrome and nonzero logical (exhaustive, small n;
+    the check of tests/test_search.py:13-28)."""
+    n = len(h)
+    for w in range(1, n + 1):
+        sols = []
+        for sup in itertools.combinations(range(n), w):
+            s = lg = 0
+            for j in sup:
+                s ^= h[j]
+                lg ^= l[j]
+            if s == 0 and lg != 0:
+                sols.append(sup)
+        if sols:
+            return w, sols
+    return None, []

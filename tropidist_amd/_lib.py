@@ -78,6 +78,14 @@ SIGNATURES = {
     "tdp_tab_build": (ci, [vp, ci, vp, vp, vp, ci]),
     "tdp_tab_build_noalloc": (ci, [vp, ci, vp, vp, vp, ci]),
     "tdp_tab_lookup_batch": (ci, [vp, vp, vp, vp, vp, u64, vp, vp]),
+    "tdp_gm_set_cover": (None, [ci, ci]),
+    "tdp_gm_set_kernel": (None, [ci, vp, vp, vp, vp]),
+    "tdp_gm_set_kernel_mode": (None, [ci]),
+    "tdp_gm_set_full_labels": (None, [vp, ci]),
+    "tdp_gm_set_pattern": (None, [vp, vp, ci, ci]),
+    "tdp_gm_num_threads": (ci, []),
+    "tdp_gm_search": (ci, [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, vp, vp, vp, ci, vp, ci, vp, vp, vp, ci, ci, ci,
+                           ctypes.c_double, vp, vp, vp, ci, ci, vp, vp, vp]),
     "tdp_batch_run": (ci, [vp, vp, ci, vp, ci, vp]),
     "tdp_batch_item_bytes": (ci, []),
     "tdp_plan_build": (ci, [vp, ci, ci, vp, vp, ci, vp, vp, i64, vp, vp, vp, vp, vp]),

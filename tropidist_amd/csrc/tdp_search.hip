@@ -1,0 +1,865 @@
+// Weight-bounded exact search on gfx950: column sets e with H e = 0, L e != 0 and |e| <= K, by the canonical
+// anchored search of cdist/gmitm.c:188-1023 (the engine behind cdist/search.py:145-441).
+//
+// The search tree is the reference's: a node is a prefix A (the anchor, then one column per level) with its residual
+// syndrome; its children are the columns of the residual row with the fewest free columns, each child forbidding the
+// siblings before it; when the remaining budget R = K - |A| reaches the depth k of the exact table, one lookup
+// decides whether a completion can exist, and every hit is re-derived by an explicit completion search.  Optional
+// prunings are the reference's: relaxed tables, the covering bound with its dual ascent, exclusion pairs, kernel
+// dominance sets and the pattern budget.
+//
+// Device design.  The DFS runs level by level: one node buffer per depth (struct of arrays, word w of node i at
+// base[w * cap + i]), and every launch expands a chunk of nodes of one depth, one thread per node, into the next
+// depth's buffer.  The host always expands the deepest non-empty depth (refilling it from the depth above when it
+// runs short), so the buffers stay bounded by depth x chunk x row degree.  A node carries its full forbidden set as
+// a bitmask instead of the reference's per-thread stamps and undo lists: a child's set is its parent's plus the
+// siblings before it, the exclusion partners and the kernel sets its column closes, which is the set the
+// reference's stamps hold at that node.  Nodes whose children are decided by one lookup each (R - 1 <= k) probe
+// their children in place and store only those that hit; hits are completed in the thread that finds them.
+//
+// The visited nodes, hits, false hits and pruning counts per node are the reference's (its per-depth node counts
+// are a function of the tree alone); solutions come in another order (callers sort them).
+#include "tdp_tables.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <vector>
+
+namespace tdp {
+namespace {
+
+constexpr int kDmax = 40;          // deepest threshold (cdist/gmitm.c:31)
+constexpr int kTmax = 4;           // lookup tables per search
+constexpr int kCvMax = 512;        // defect rows the dual ascent handles
+constexpr int kInvMax = 1024;
+constexpr int kBlock = 256;
+constexpr int kCompleteMax = 8;    // completion depth: the deepest table
+
+// per-launch counters, summed into the run's statistics
+enum { S_LOOKUPS, S_HITS, S_FALSE, S_TASKS, S_CPRUNED, S_APRUNED, S_CCALLS, S_NODES_A, S_NODES_A1, S_N };
+// run statistics on the device: nodes per depth [0, kDmax + 2), then the counters of cdist/gmitm.c:857-869
+constexpr int kStatBase = kDmax + 2;
+
+struct GmFlags {
+    int stop;
+    int overflow;
+    int best;                      // INT_MAX: none
+    unsigned nsol;                 // solutions claimed (stored while below the cap)
+    unsigned long long nfound;
+    unsigned count[kDmax + 2];     // nodes held per depth
+};
+
+struct Level {
+    uint64_t *base;
+    uint32_t cap;
+};
+
+struct GmDev {
+    int W, N, M, wmax, K, td, enum_all, kernel_mode, lw, cover, ascent, ntab, main_tab;
+    int off_hdr, off_path, off_s, off_zh, off_zl, off_lg, off_forb, off_used;
+    double inv_wmax;
+    const uint64_t *col, *lg, *full_lg;
+    const int *row_start, *row_cols, *col_start, *col_rows;
+    const uint64_t *tzh[kTmax], *tzl[kTmax];
+    TabView tab[kTmax];
+    int tab_wide[kTmax], tab_k[kTmax];
+    const int *ex_start, *ex_cols;
+    int nks;
+    const int *ks_start, *ks_cols, *kc_start, *kc_ids;
+    const int *cls;
+    const uint8_t *onpat;
+    int slack;
+    const double *inv;             // inv[m] = 1 / m, 1 <= m <= kInvMax
+    int sol_cap, sol_stride;
+    int *sol;
+    // dual-ascent scratch per thread slot
+    double *cv_load;
+    unsigned *cv_stamp, *cv_id;
+    int *cv_drow, *cv_av, *cv_ord;
+    double *cv_dy;
+};
+
+#define NODE(L, w, i) ((L).base[(size_t)(w) * (L).cap + (i)])
+
+__device__ __forceinline__ bool stopped(const GmFlags *fl) {
+    return __hip_atomic_load(&fl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
+
+__device__ __forceinline__ bool is_forb(const GmDev &P, const Level &L, uint32_t i, int f) {
+    return (NODE(L, P.off_forb + (f >> 6), i) >> (f & 63)) & 1;
+}
+
+__device__ __forceinline__ int path_at(const GmDev &P, const Level &L, uint32_t i, int p) {
+    return (int)((NODE(L, P.off_path + (p >> 2), i) >> ((p & 3) * 16)) & 0xFFFF);
+}
+
+__device__ __forceinline__ int free_in_row(const GmDev &P, const Level &L, uint32_t i, int r) {
+    int av = 0;
+    for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) av += !is_forb(P, L, i, P.row_cols[q]);
+    return av;
+}
+
+// 0: over the pattern budget; 1: takes the free slot of its class; 2: counts as an extra (cdist/gmitm.c:347-365)
+__device__ __forceinline__ int pat_how(const GmDev &P, const Level &L, uint32_t i, int f) {
+    if (!P.cls) return 1;
+    int cl = P.cls[f];
+    if (cl >= 0 && P.onpat[cl] && !((NODE(L, P.off_used + (cl >> 6), i) >> (cl & 63)) & 1)) return 1;
+    int extras = (int)(NODE(L, P.off_hdr, i) & 0xFFFF);
+    return extras < P.slack ? 2 : 0;
+}
+
+// is the logical value of path(0..a) + B(0..nb) nonzero (g: the labelled-mode running label)
+__device__ bool sol_nonzero(const GmDev &P, const Level &L, uint32_t i, int a, const int *B, int nb, uint64_t g) {
+    if (!P.full_lg) return g != 0;
+    for (int w = 0; w < P.lw; w++) {
+        uint64_t x = 0;
+        for (int p = 0; p < a; p++) x ^= P.full_lg[(size_t)path_at(P, L, i, p) * P.lw + w];
+        for (int b = 0; b < nb; b++) x ^= P.full_lg[(size_t)B[b] * P.lw + w];
+        if (x) return true;
+    }
+    return false;
+}
+
+__device__ void record(const GmDev &P, GmFlags *fl, const Level &L, uint32_t i, int a, const int *B, int nb) {
+    const int weight = a + nb;
+    atomicMin(&fl->best, weight);
+    atomicAdd(&fl->nfound, 1ull);
+    unsigned idx = atomicAdd(&fl->nsol, 1u);
+    if ((int)idx < P.sol_cap) {
+        int *dst = P.sol + (size_t)idx * P.sol_stride;
+        dst[0] = weight;
+        for (int q = 0; q < weight && q + 1 < P.sol_stride; q++) dst[q + 1] = q < a ? path_at(P, L, i, q) : B[q - a];
+    }
+    if (!P.enum_all) __hip_atomic_store(&fl->stop, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Every B (|B| <= R) that cancels the node's syndrome, with a nonzero total label, disjoint from the node's path
+// (cdist/gmitm.c:444-482): branch on the lowest residual row; a column taken at a level stays excluded for its
+// later siblings and their subtrees.  Returns the number of completions recorded.
+__device__ int complete(const GmDev &P, const Level &L, uint32_t i, int a, int R, GmFlags *fl) {
+    int B[kCompleteMax + 1], rowB[kCompleteMax + 1], posB[kCompleteMax + 1];
+    int found = 0, lev = 0;
+    bool enter = true;
+    for (;;) {
+        if (enter) {
+            enter = false;
+            bool descend = false;
+            int pc = 0, r = -1;
+            for (int w = 0; w < P.W; w++) {
+                uint64_t x = NODE(L, P.off_s + w, i);
+                for (int b = 0; b < lev; b++) x ^= P.col[(size_t)B[b] * P.W + w];
+                pc += __popcll(x);
+                if (r < 0 && x) r = w * 64 + __builtin_ctzll(x);
+            }
+            if (pc == 0) {
+                uint64_t g = NODE(L, P.off_lg, i);
+                for (int b = 0; b < lev; b++) g ^= P.lg[B[b]];
+                if (sol_nonzero(P, L, i, a, B, lev, g) != (bool)P.kernel_mode) {
+                    record(P, fl, L, i, a, B, lev);
+                    found++;
+                }
+            } else if (R - lev > 0 && pc <= P.wmax * (R - lev)) {
+                rowB[lev] = r;
+                posB[lev] = P.row_start[r] - 1;
+                descend = true;
+            }
+            if (!descend) {
+                if (lev == 0) return found;
+                lev--;
+            }
+        }
+        if (stopped(fl)) return found;
+        {
+            const int r = rowB[lev], end = P.row_start[r + 1];
+            int q = posB[lev] + 1, f = -1;
+            for (; q < end; q++) {
+                const int c = P.row_cols[q];
+                bool excluded = false;
+                for (int p = 0; p < a && !excluded; p++) excluded = path_at(P, L, i, p) == c;
+                for (int l = 0; l < lev && !excluded; l++)
+                    for (int t = P.row_start[rowB[l]]; t <= posB[l]; t++)
+                        if (P.row_cols[t] == c) { excluded = true; break; }
+                if (!excluded) { f = c; break; }
+            }
+            if (f < 0) {
+                if (lev == 0) return found;
+                lev--;
+                continue;
+            }
+            posB[lev] = q;
+            B[lev] = f;
+            lev++;
+            enter = true;
+        }
+    }
+}
+
+// Covering bound (cdist/gmitm.c:484-596): a feasible dual y of the covering LP of the residual rows by the free
+// columns; true if it exceeds R.  Same operations in the same order as the reference, in double precision.
+__device__ bool cover_prune(const GmDev &P, const Level &L, uint32_t i, int R, int pc, uint32_t slot,
+                            unsigned &apruned) {
+    double sum = 0.0;
+    const double lim = R + 1e-9, inv_wmax = P.inv_wmax;
+    int left = pc;
+    const bool ascent = P.ascent && pc <= kCvMax;
+    int *drow = P.cv_drow + (size_t)slot * kCvMax;
+    double *dy = P.cv_dy + (size_t)slot * kCvMax;
+    int nd = 0;
+    if (left * inv_wmax > lim) return true;
+    for (int w = 0; w < P.W; w++) {
+        uint64_t x = NODE(L, P.off_s + w, i);
+        while (x) {
+            const int r = w * 64 + __builtin_ctzll(x);
+            x &= x - 1;
+            left--;
+            int m = 0;
+            for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
+                const int f = P.row_cols[q];
+                if (is_forb(P, L, i, f)) continue;
+                int o = 0;
+                for (int t = P.col_start[f]; t < P.col_start[f + 1]; t++) {
+                    const int rr = P.col_rows[t];
+                    o += (int)((NODE(L, P.off_s + (rr >> 6), i) >> (rr & 63)) & 1);
+                }
+                m = o > m ? o : m;
+            }
+            if (m == 0) return true;
+            sum += P.inv[m];
+            if (sum + left * inv_wmax > lim) return true;
+            if (ascent) {
+                drow[nd] = r;
+                dy[nd] = P.inv[m];
+                nd++;
+            } else if (sum + left <= lim) {
+                return false;
+            }
+        }
+    }
+    if (!ascent) return false;
+    double *load = P.cv_load + (size_t)slot * P.N;
+    unsigned *stamp = P.cv_stamp + (size_t)slot * P.N;
+    unsigned id = ++P.cv_id[slot];
+    if (id == 0) {                                   // wrapped: no stale stamp may equal a new id
+        for (int f = 0; f < P.N; f++) stamp[f] = 0;
+        id = P.cv_id[slot] = 1;
+    }
+    for (int d = 0; d < nd; d++) {
+        const int r = drow[d];
+        const double y = dy[d];
+        for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
+            const int f = P.row_cols[q];
+            if (is_forb(P, L, i, f)) continue;
+            if (stamp[f] != id) {
+                stamp[f] = id;
+                load[f] = y;
+            } else {
+                load[f] += y;
+            }
+        }
+    }
+    // rows with the fewest free columns first, ties in row order
+    int *av = P.cv_av + (size_t)slot * kCvMax;
+    int *ord = P.cv_ord + (size_t)slot * kCvMax;
+    for (int d = 0; d < nd; d++) {
+        av[d] = free_in_row(P, L, i, drow[d]);
+        int j = d;
+        while (j > 0 && av[ord[j - 1]] > av[d]) {
+            ord[j] = ord[j - 1];
+            j--;
+        }
+        ord[j] = d;
+    }
+    for (int t = 0; t < nd; t++) {
+        const int r = drow[ord[t]];
+        double slackv = 1.0;
+        for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
+            const int f = P.row_cols[q];
+            if (is_forb(P, L, i, f)) continue;
+            const double s = 1.0 - load[f];
+            if (s < slackv) slackv = s;
+        }
+        if (slackv <= 1e-12) continue;
+        for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
+            const int f = P.row_cols[q];
+            if (!is_forb(P, L, i, f)) load[f] += slackv;
+        }
+        sum += slackv;
+        if (sum > lim) {
+            apruned++;
+            return true;
+        }
+    }
+    return false;
+}
+
+__device__ __forceinline__ void set_bit(const GmDev &P, const Level &L, uint32_t j, int off, int b) {
+    NODE(L, off + (b >> 6), j) |= 1ull << (b & 63);
+}
+
+// Child of node i (depth a) by column f into slot j of `out`.  pos >= 0: the general branch, whose child forbids
+// the pattern-admissible columns of row piv up to position pos, the exclusion partners of f and the kernel sets f
+// closes; pos < 0: a leaf-probe hit, which forbids f only (it is completed, never expanded).
+__device__ void write_child(const GmDev &P, const Level &in, uint32_t i, int a, const Level &out, uint32_t j, int f,
+                            int how, int piv, int pos) {
+    uint64_t hdr = NODE(in, P.off_hdr, i);
+    if (how == 2) hdr += 1;
+    NODE(out, P.off_hdr, j) = hdr;
+    for (int w = P.off_path; w < P.off_s; w++) NODE(out, w, j) = NODE(in, w, i);
+    {
+        uint64_t &pw = NODE(out, P.off_path + (a >> 2), j);
+        const int sh = (a & 3) * 16;
+        pw = (pw & ~(0xFFFFull << sh)) | ((uint64_t)f << sh);
+    }
+    for (int w = 0; w < P.W; w++) NODE(out, P.off_s + w, j) = NODE(in, P.off_s + w, i) ^ P.col[(size_t)f * P.W + w];
+    for (int t = 0; t < P.ntab; t++) {
+        NODE(out, P.off_zh + t, j) = NODE(in, P.off_zh + t, i) ^ P.tzh[t][f];
+        NODE(out, P.off_zl + t, j) = NODE(in, P.off_zl + t, i) ^ P.tzl[t][f];
+    }
+    NODE(out, P.off_lg, j) = NODE(in, P.off_lg, i) ^ P.lg[f];
+    for (int w = P.off_forb; w < P.off_used; w++) NODE(out, w, j) = NODE(in, w, i);
+    set_bit(P, out, j, P.off_forb, f);
+    if (pos >= 0) {
+        for (int q = P.row_start[piv]; q < pos; q++) {
+            const int g = P.row_cols[q];
+            if (!is_forb(P, in, i, g) && pat_how(P, in, i, g)) set_bit(P, out, j, P.off_forb, g);
+        }
+        if (P.ex_start)
+            for (int q = P.ex_start[f]; q < P.ex_start[f + 1]; q++) set_bit(P, out, j, P.off_forb, P.ex_cols[q]);
+        for (int q = P.nks ? P.kc_start[f] : 0; P.nks && q < P.kc_start[f + 1]; q++) {
+            const int s = P.kc_ids[q];
+            const int b0 = P.ks_start[s], b1 = P.ks_start[s + 1];
+            int cnt = 0;
+            for (int t = b0; t < b1; t++) {
+                const int g = P.ks_cols[t];
+                bool in_path = g == f;
+                for (int p = 0; p < a && !in_path; p++) in_path = path_at(P, in, i, p) == g;
+                cnt += in_path;
+            }
+            if (cnt == (b1 - b0) / 2)
+                for (int t = b0; t < b1; t++) set_bit(P, out, j, P.off_forb, P.ks_cols[t]);
+        }
+    }
+}
+
+__device__ __forceinline__ void copy_used(const GmDev &P, const Level &in, uint32_t i, const Level &out, uint32_t j,
+                                          int nwords, int f, int how) {
+    for (int w = P.off_used; w < nwords; w++) NODE(out, w, j) = NODE(in, w, i);
+    if (how == 1 && P.cls) set_bit(P, out, j, P.off_used, P.cls[f]);
+}
+
+// One launch: nodes [first, first + n) of depth a (buffer `in`) are visited; children go to `out` (depth a + 1).
+__global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level in, uint32_t first, uint32_t n,
+                                                      int a, const Level out, int nwords, GmFlags *fl,
+                                                      unsigned long long *stats) {
+    __shared__ unsigned cnt[S_N];
+    __shared__ unsigned nchild_sh[kBlock];
+    __shared__ unsigned base_sh;
+    const int tid = threadIdx.x;
+    if (tid < S_N) cnt[tid] = 0;
+    if (blockIdx.x == 0 && tid == 0) atomicSub(&fl->count[a], n);
+    __syncthreads();
+
+    const uint32_t t = blockIdx.x * kBlock + tid;
+    const uint32_t i = first + t;
+    unsigned c_lookups = 0, c_hits = 0, c_false = 0, c_tasks = 0, c_cpr = 0, c_apr = 0, c_cc = 0, c_na = 0,
+             c_na1 = 0;
+    int piv = -1;          // >= 0: general branch over row piv
+    unsigned nchild = 0;
+    const int R = P.K - a;
+    const bool leaf = P.main_tab >= 0 && R - 1 <= P.tab_k[P.main_tab];
+
+    if (t < n && !stopped(fl)) {
+        c_na = 1;
+        int pc = 0;
+        for (int w = 0; w < P.W; w++) pc += __popcll(NODE(in, P.off_s + w, i));
+        bool go = true;
+        if (pc == 0) {
+            if (sol_nonzero(P, in, i, a, nullptr, 0, NODE(in, P.off_lg, i)) != (bool)P.kernel_mode)
+                record(P, fl, in, i, a, nullptr, 0);
+            go = false;
+        } else if (R <= 0 || pc > P.wmax * R) {
+            go = false;
+        } else if (P.cover && pc > R && 4 * pc >= P.cover * R && (P.main_tab < 0 || R > P.tab_k[P.main_tab])) {
+            c_cc = 1;
+            if (cover_prune(P, in, i, R, pc, t, c_apr)) {
+                c_cpr = 1;
+                go = false;
+            }
+        }
+        for (int tb = 0; go && tb < P.ntab; tb++) {
+            if (R > P.tab_k[tb]) continue;
+            c_lookups++;
+            const uint64_t label = P.full_lg ? 1ull : NODE(in, P.off_lg, i);
+            const int cost = tab_probe(P.tab[tb], P.tab_wide[tb], NODE(in, P.off_zh + tb, i),
+                                       NODE(in, P.off_zl + tb, i), label, R);
+            if (cost < 0) {
+                go = false;
+            } else if (tb == P.main_tab) {
+                // the remainder exists up to a fingerprint collision: derive it, with the full budget R
+                if (complete(P, in, i, a, R, fl) > 0) c_hits++;
+                else c_false++;
+                go = false;
+            }
+        }
+        if (go && a == P.td) c_tasks = 1;
+        if (go) {
+            int best = INT_MAX;
+            for (int w = 0; w < P.W; w++) {
+                uint64_t x = NODE(in, P.off_s + w, i);
+                while (x) {
+                    const int r = w * 64 + __builtin_ctzll(x);
+                    x &= x - 1;
+                    const int av = free_in_row(P, in, i, r);
+                    if (av < best) {
+                        best = av;
+                        piv = r;
+                    }
+                }
+            }
+            if (best == 0) piv = -1;
+        }
+        if (piv >= 0 && leaf) {
+            // children decided by one probe each: store only the hits
+            const int mt = P.main_tab;
+            for (int q = P.row_start[piv]; q < P.row_start[piv + 1]; q++) {
+                const int f = P.row_cols[q];
+                if (is_forb(P, in, i, f)) continue;
+                const int how = pat_how(P, in, i, f);
+                if (!how) continue;
+                const uint64_t zh = NODE(in, P.off_zh + mt, i) ^ P.tzh[mt][f];
+                const uint64_t zl = NODE(in, P.off_zl + mt, i) ^ P.tzl[mt][f];
+                const uint64_t label = P.full_lg ? 1ull : (NODE(in, P.off_lg, i) ^ P.lg[f]);
+                if (tab_probe(P.tab[mt], P.tab_wide[mt], zh, zl, label, R - 1) < 0) {
+                    c_na1++;
+                    c_lookups++;
+                    continue;
+                }
+                const unsigned j = atomicAdd(&fl->count[a + 1], 1u);
+                if (j >= out.cap) {
+                    __hip_atomic_store(&fl->overflow, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    continue;
+                }
+                write_child(P, in, i, a, out, j, f, how, piv, -1);
+                copy_used(P, in, i, out, j, nwords, f, how);
+            }
+            piv = -1;
+        } else if (piv >= 0) {
+            for (int q = P.row_start[piv]; q < P.row_start[piv + 1]; q++) {
+                const int f = P.row_cols[q];
+                if (!is_forb(P, in, i, f) && pat_how(P, in, i, f)) nchild++;
+            }
+        }
+    }
+    // reserve the general branches' children for the whole block at once
+    nchild_sh[tid] = nchild;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned run = 0;
+        for (int q = 0; q < kBlock; q++) {
+            const unsigned c = nchild_sh[q];
+            nchild_sh[q] = run;
+            run += c;
+        }
+        base_sh = run ? atomicAdd(&fl->count[a + 1], run) : 0;
+    }
+    __syncthreads();
+    if (nchild) {
+        uint32_t j = base_sh + nchild_sh[tid];
+        for (int q = P.row_start[piv]; q < P.row_start[piv + 1]; q++) {
+            const int f = P.row_cols[q];
+            if (is_forb(P, in, i, f)) continue;
+            const int how = pat_how(P, in, i, f);
+            if (!how) continue;
+            if (j >= out.cap) {
+                __hip_atomic_store(&fl->overflow, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                break;
+            }
+            write_child(P, in, i, a, out, j, f, how, piv, q);
+            copy_used(P, in, i, out, j, nwords, f, how);
+            j++;
+        }
+    }
+    atomicAdd(&cnt[S_LOOKUPS], c_lookups);
+    atomicAdd(&cnt[S_HITS], c_hits);
+    atomicAdd(&cnt[S_FALSE], c_false);
+    atomicAdd(&cnt[S_TASKS], c_tasks);
+    atomicAdd(&cnt[S_CPRUNED], c_cpr);
+    atomicAdd(&cnt[S_APRUNED], c_apr);
+    atomicAdd(&cnt[S_CCALLS], c_cc);
+    atomicAdd(&cnt[S_NODES_A], c_na);
+    atomicAdd(&cnt[S_NODES_A1], c_na1);
+    __syncthreads();
+    if (tid < S_N && cnt[tid]) {
+        unsigned long long *dst;
+        if (tid == S_NODES_A) dst = stats + a;
+        else if (tid == S_NODES_A1) dst = stats + a + 1;
+        else dst = stats + kStatBase + tid;
+        atomicAdd(dst, (unsigned long long)cnt[tid]);
+    }
+}
+
+__global__ void k_gm_init(GmFlags *fl, unsigned long long *stats, int nstats) {
+    const int t = threadIdx.x;
+    if (t < nstats) stats[t] = 0;
+    if (t == 0) {
+        fl->stop = 0;
+        fl->overflow = 0;
+        fl->best = INT_MAX;
+        fl->nsol = 0;
+        fl->nfound = 0;
+    }
+    if (t < kDmax + 2) fl->count[t] = 0;
+}
+
+// process-wide options of the next searches (the reference's gm_set_* globals, cdist/gmitm.c:230-253), per thread
+struct Options {
+    int cover = 0, ascent = 0;
+    int nks = 0;
+    const int *ks_start = nullptr, *ks_cols = nullptr, *kc_start = nullptr, *kc_ids = nullptr;
+    int kernel_mode = 0;
+    const uint64_t *full_lg = nullptr;
+    int lw = 0;
+    const int *cls = nullptr;
+    const uint8_t *onpat = nullptr;
+    int nclass = 0, slack = 0;
+};
+thread_local Options g_opt;
+
+// device copies of the run's host arrays, freed together
+struct DevArena {
+    std::vector<void *> blocks;
+    ~DevArena() {
+        for (void *p : blocks) (void)hipFree(p);
+    }
+    template <typename T>
+    int put(const T *h, size_t n, const T **out) {
+        *out = nullptr;
+        if (!h || !n) return TDP_OK;
+        void *d = nullptr;
+        TDP_HIP_CHECK(hipMalloc(&d, n * sizeof(T)));
+        blocks.push_back(d);
+        TDP_HIP_CHECK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+        *out = static_cast<const T *>(d);
+        return TDP_OK;
+    }
+    int alloc(size_t bytes, void **out) {
+        *out = nullptr;
+        if (hipMalloc(out, bytes ? bytes : 8) != hipSuccess) {
+            *out = nullptr;
+            set_error("device allocation of %zu bytes for the search failed", bytes);
+            return TDP_ENOMEM;
+        }
+        blocks.push_back(*out);
+        return TDP_OK;
+    }
+};
+
+#define GM_TRY(expr)                 \
+    do {                             \
+        int _rc = (expr);            \
+        if (_rc != TDP_OK) return _rc; \
+    } while (0)
+
+}  // namespace
+}  // namespace tdp
+
+using namespace tdp;
+
+extern "C" {
+
+void tdp_gm_set_cover(int gate4, int ascent) {
+    g_opt.cover = gate4;
+    g_opt.ascent = ascent;
+}
+
+void tdp_gm_set_kernel(int nks, const int *ks_start, const int *ks_cols, const int *kc_start, const int *kc_ids) {
+    g_opt.nks = nks;
+    g_opt.ks_start = ks_start;
+    g_opt.ks_cols = ks_cols;
+    g_opt.kc_start = kc_start;
+    g_opt.kc_ids = kc_ids;
+}
+
+void tdp_gm_set_kernel_mode(int on) { g_opt.kernel_mode = on; }
+
+void tdp_gm_set_full_labels(const uint64_t *lg, int lw) {
+    g_opt.full_lg = lg;
+    g_opt.lw = lw;
+}
+
+void tdp_gm_set_pattern(const int *cls, const uint8_t *onpat, int nclass, int slack) {
+    g_opt.cls = cls;
+    g_opt.onpat = onpat;
+    g_opt.nclass = nclass;
+    g_opt.slack = slack;
+}
+
+int tdp_gm_num_threads(void) { return 65536; }
+
+int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint64_t *lg, const int *row_start,
+                  const int *row_cols, const int *col_start, const int *col_rows, int ntab, void **tabs,
+                  const uint64_t **tzh, const uint64_t **tzl, int main_tab, const int *rank, int n_anchor,
+                  const int *anchors, const int *ex_start, const int *ex_cols, int K, int enum_all, int task_depth,
+                  double time_limit, int64_t *nodes_out, int64_t *stats_out, int *best, int sol_cap, int sol_stride,
+                  int *sol_buf, int *nsol, int *status) {
+    *status = -1;
+    TDP_REQUIRE(best && nsol && nodes_out && stats_out && status, "tdp_gm_search: null output");
+    *best = -1;
+    *nsol = 0;
+    for (int d = 0; d <= kDmax; d++) nodes_out[d] = 0;
+    for (int s = 0; s < 8; s++) stats_out[s] = 0;
+    if (K > kDmax) {
+        *status = 4;
+        return TDP_OK;
+    }
+    const Options opt = g_opt;
+    TDP_REQUIRE(W >= 1 && N >= 1 && M >= 1 && M <= 64 * W && N < 65536, "tdp_gm_search: W %d N %d M %d", W, N, M);
+    TDP_REQUIRE(ntab >= 0 && ntab <= kTmax && main_tab < ntab && K >= 0 && n_anchor >= 0,
+                "tdp_gm_search: %d tables (main %d), K %d", ntab, main_tab, K);
+    TDP_REQUIRE(sol_cap >= 0 && (sol_cap == 0 || (sol_buf && sol_stride >= 1)), "tdp_gm_search: solution buffer");
+    TDP_REQUIRE(rank && (n_anchor == 0 || anchors), "tdp_gm_search: rank and anchors are required");
+    const auto t0 = std::chrono::steady_clock::now();
+    const int Nw = (N + 63) / 64;
+    const int Cw = opt.cls ? std::max(1, (opt.nclass + 63) / 64) : 0;
+    const int KL = std::max(K, 1);           // deepest level held: the anchors are visited even at K = 0
+    const int PW = (KL + 3) / 4;
+    GmDev P{};
+    P.W = W;
+    P.N = N;
+    P.M = M;
+    P.wmax = wmax;
+    P.K = K;
+    P.td = std::min(std::max(task_depth, 1), std::max(K, 1));
+    P.enum_all = enum_all;
+    P.kernel_mode = opt.kernel_mode;
+    P.lw = opt.full_lg ? opt.lw : 0;
+    P.cover = wmax > kInvMax ? 0 : opt.cover;
+    P.ascent = opt.ascent;
+    P.ntab = ntab;
+    P.main_tab = main_tab;
+    P.off_hdr = 0;
+    P.off_path = 1;
+    P.off_s = P.off_path + PW;
+    P.off_zh = P.off_s + W;
+    P.off_zl = P.off_zh + ntab;
+    P.off_lg = P.off_zl + ntab;
+    P.off_forb = P.off_lg + 1;
+    P.off_used = P.off_forb + Nw;
+    const int nwords = P.off_used + Cw;
+    P.inv_wmax = 1.0 / wmax;
+    P.nks = opt.nks;
+    P.slack = opt.slack;
+    P.sol_cap = sol_cap;
+    P.sol_stride = sol_stride;
+
+    DevArena D;
+    const size_t nnz = (size_t)row_start[M];
+    GM_TRY(D.put(col, (size_t)N * W, &P.col));
+    std::vector<uint64_t> zero_lg;
+    if (!lg) zero_lg.assign(N, 0);
+    GM_TRY(D.put(lg ? lg : zero_lg.data(), (size_t)N, &P.lg));
+    if (opt.full_lg) GM_TRY(D.put(opt.full_lg, (size_t)N * opt.lw, &P.full_lg));
+    GM_TRY(D.put(row_start, (size_t)M + 1, &P.row_start));
+    GM_TRY(D.put(row_cols, nnz ? nnz : 1, &P.row_cols));
+    GM_TRY(D.put(col_start, (size_t)N + 1, &P.col_start));
+    GM_TRY(D.put(col_rows, (size_t)col_start[N] ? (size_t)col_start[N] : 1, &P.col_rows));
+    for (int t = 0; t < ntab; t++) {
+        DevTable *tb = static_cast<DevTable *>(tabs[t]);
+        TDP_REQUIRE(tb != nullptr, "tdp_gm_search: table %d is null", t);
+        P.tab[t] = view(tb);
+        P.tab_wide[t] = tb->wide;
+        P.tab_k[t] = tb->k;
+        TDP_REQUIRE(t != main_tab || tb->k <= kCompleteMax, "tdp_gm_search: exact table depth %d above %d", tb->k,
+                    kCompleteMax);
+        GM_TRY(D.put(tzh[t], (size_t)N, &P.tzh[t]));
+        GM_TRY(D.put(tzl[t], (size_t)N, &P.tzl[t]));
+    }
+    if (ex_start) {
+        GM_TRY(D.put(ex_start, (size_t)N + 1, &P.ex_start));
+        GM_TRY(D.put(ex_cols, (size_t)std::max(ex_start[N], 1), &P.ex_cols));
+    }
+    if (opt.nks > 0) {
+        GM_TRY(D.put(opt.ks_start, (size_t)opt.nks + 1, &P.ks_start));
+        GM_TRY(D.put(opt.ks_cols, (size_t)std::max(opt.ks_start[opt.nks], 1), &P.ks_cols));
+        GM_TRY(D.put(opt.kc_start, (size_t)N + 1, &P.kc_start));
+        GM_TRY(D.put(opt.kc_ids, (size_t)std::max(opt.kc_start[N], 1), &P.kc_ids));
+    }
+    if (opt.cls) {
+        GM_TRY(D.put(opt.cls, (size_t)N, &P.cls));
+        std::vector<uint8_t> on(std::max(opt.nclass, 1), 0);
+        for (int c = 0; c < opt.nclass; c++) on[c] = opt.onpat[c];
+        GM_TRY(D.put(on.data(), on.size(), &P.onpat));
+    }
+    std::vector<double> inv(kInvMax + 1, 0.0);
+    for (int m = 1; m <= kInvMax; m++) inv[m] = 1.0 / m;
+    GM_TRY(D.put(inv.data(), inv.size(), &P.inv));
+
+    int deg = 1;
+    for (int r = 0; r < M; r++) deg = std::max(deg, row_start[r + 1] - row_start[r]);
+    // chunk: nodes per launch.  Buffers: depth 1 holds the anchors, deeper depths chunk x degree nodes each.
+    uint32_t chunk = 65536;
+    const size_t node_bytes = (size_t)nwords * 8;
+    const size_t level_budget = (size_t)24 << 30;
+    while (chunk > 256 && (size_t)KL * chunk * deg * node_bytes > level_budget) chunk /= 2;
+    if (P.cover && P.ascent) {
+        const size_t per_slot = (size_t)N * 12 + (size_t)kCvMax * 20 + 4;
+        while (chunk > 256 && chunk * per_slot > ((size_t)2 << 30)) chunk /= 2;
+        void *p = nullptr;
+        GM_TRY(D.alloc((size_t)chunk * N * 8, &p));
+        P.cv_load = static_cast<double *>(p);
+        GM_TRY(D.alloc((size_t)chunk * N * 4, &p));
+        P.cv_stamp = static_cast<unsigned *>(p);
+        TDP_HIP_CHECK(hipMemset(p, 0, (size_t)chunk * N * 4));
+        GM_TRY(D.alloc((size_t)chunk * 4, &p));
+        P.cv_id = static_cast<unsigned *>(p);
+        TDP_HIP_CHECK(hipMemset(p, 0, (size_t)chunk * 4));
+        GM_TRY(D.alloc((size_t)chunk * kCvMax * 4, &p));
+        P.cv_drow = static_cast<int *>(p);
+        GM_TRY(D.alloc((size_t)chunk * kCvMax * 8, &p));
+        P.cv_dy = static_cast<double *>(p);
+        GM_TRY(D.alloc((size_t)chunk * kCvMax * 4, &p));
+        P.cv_av = static_cast<int *>(p);
+        GM_TRY(D.alloc((size_t)chunk * kCvMax * 4, &p));
+        P.cv_ord = static_cast<int *>(p);
+    }
+    void *pv = nullptr;
+    GM_TRY(D.alloc(sizeof(GmFlags), &pv));
+    GmFlags *d_fl = static_cast<GmFlags *>(pv);
+    const int nstats = kStatBase + S_N;
+    GM_TRY(D.alloc(nstats * 8, &pv));
+    unsigned long long *d_stats = static_cast<unsigned long long *>(pv);
+    if (sol_cap > 0) {
+        GM_TRY(D.alloc((size_t)sol_cap * sol_stride * 4, &pv));
+        P.sol = static_cast<int *>(pv);
+    }
+    k_gm_init<<<1, 128>>>(d_fl, d_stats, nstats);
+    TDP_LAUNCH_CHECK();
+
+    // depth-1 nodes: the anchors, each forbidding the columns of lower rank, itself, its exclusion partners and
+    // the kernel sets it closes (cdist/gmitm.c:727-740, 744-760)
+    std::vector<Level> lev(kDmax + 2, Level{nullptr, 0});
+    std::vector<uint32_t> count(kDmax + 2, 0);
+    if (n_anchor > 0) {
+        const uint32_t cap = (uint32_t)n_anchor;
+        std::vector<uint64_t> h((size_t)nwords * cap, 0);
+        auto at = [&](int w, uint32_t i) -> uint64_t & { return h[(size_t)w * cap + i]; };
+        for (uint32_t i = 0; i < cap; i++) {
+            const int f = anchors[i];
+            TDP_REQUIRE(f >= 0 && f < N, "tdp_gm_search: anchor %d out of range", f);
+            at(P.off_path, i) = (uint64_t)f;
+            for (int w = 0; w < W; w++) at(P.off_s + w, i) = col[(size_t)f * W + w];
+            for (int t = 0; t < ntab; t++) {
+                at(P.off_zh + t, i) = tzh[t][f];
+                at(P.off_zl + t, i) = tzl[t][f];
+            }
+            at(P.off_lg, i) = lg ? lg[f] : 0;
+            auto forbid = [&](int g) { at(P.off_forb + (g >> 6), i) |= 1ull << (g & 63); };
+            for (int g = 0; g < N; g++)
+                if (rank[g] < rank[f]) forbid(g);
+            forbid(f);
+            if (ex_start)
+                for (int q = ex_start[f]; q < ex_start[f + 1]; q++) forbid(ex_cols[q]);
+            for (int q = opt.nks ? opt.kc_start[f] : 0; opt.nks && q < opt.kc_start[f + 1]; q++) {
+                const int s = opt.kc_ids[q];
+                if ((opt.ks_start[s + 1] - opt.ks_start[s]) / 2 == 1)   // the anchor alone reaches half the set
+                    for (int t = opt.ks_start[s]; t < opt.ks_start[s + 1]; t++) forbid(opt.ks_cols[t]);
+            }
+            if (opt.cls) {
+                const int cl = opt.cls[f];
+                if (cl >= 0 && opt.onpat[cl]) at(P.off_used + (cl >> 6), i) |= 1ull << (cl & 63);
+                else if (opt.slack > 0) at(P.off_hdr, i) = 1;
+            }
+        }
+        void *p = nullptr;
+        GM_TRY(D.alloc(h.size() * 8, &p));
+        TDP_HIP_CHECK(hipMemcpy(p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+        lev[1] = Level{static_cast<uint64_t *>(p), cap};
+        count[1] = cap;
+        TDP_HIP_CHECK(hipMemcpy(&d_fl->count[1], &count[1], 4, hipMemcpyHostToDevice));
+    }
+    const uint32_t deep_cap = chunk * (uint32_t)deg;
+    auto level = [&](int d, Level *out) -> int {
+        if (d > KL) {
+            *out = Level{nullptr, 0};
+            return TDP_OK;
+        }
+        if (!lev[d].base) {
+            void *p = nullptr;
+            GM_TRY(D.alloc((size_t)deep_cap * node_bytes, &p));
+            lev[d] = Level{static_cast<uint64_t *>(p), deep_cap};
+        }
+        *out = lev[d];
+        return TDP_OK;
+    };
+
+    GmFlags *h_fl = nullptr;
+    TDP_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h_fl), sizeof(GmFlags), hipHostMallocDefault));
+    struct PinnedFree {
+        GmFlags *p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pin{h_fl};
+    bool timed_out = false, overflow = false, stop = false;
+    const double deadline = time_limit > 0 ? time_limit : 0.0;
+    while (true) {
+        int d = 0;
+        for (int q = KL; q >= 1; q--)
+            if (count[q]) {
+                d = q;
+                break;
+            }
+        if (!d) break;
+        if (deadline > 0 &&
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() > deadline) {
+            timed_out = true;
+            break;
+        }
+        int src = d;
+        uint32_t n = std::min(count[d], chunk);
+        if (d >= 2 && count[d] < chunk / 2 && count[d - 1] > 0 && lev[d].cap > count[d]) {
+            const uint32_t room = (lev[d].cap - count[d]) / (uint32_t)deg;
+            if (room >= 1) {
+                src = d - 1;
+                n = std::min(std::min(count[d - 1], room), chunk);
+            }
+        }
+        Level out{};
+        GM_TRY(level(src + 1, &out));
+        const uint32_t first = count[src] - n;
+        k_gm_expand<<<(n + kBlock - 1) / kBlock, kBlock>>>(P, lev[src], first, n, src, out, nwords, d_fl, d_stats);
+        TDP_LAUNCH_CHECK();
+        TDP_HIP_CHECK(hipMemcpyAsync(h_fl, d_fl, sizeof(GmFlags), hipMemcpyDeviceToHost, 0));
+        TDP_HIP_CHECK(hipStreamSynchronize(0));
+        if (h_fl->overflow) {
+            overflow = true;
+            break;
+        }
+        for (int q = 1; q <= KL; q++) count[q] = h_fl->count[q];
+        if (h_fl->stop) {
+            stop = true;
+            break;
+        }
+    }
+    TDP_HIP_CHECK(hipMemcpy(h_fl, d_fl, sizeof(GmFlags), hipMemcpyDeviceToHost));
+    std::vector<unsigned long long> st(nstats);
+    TDP_HIP_CHECK(hipMemcpy(st.data(), d_stats, nstats * 8, hipMemcpyDeviceToHost));
+    TDP_REQUIRE(!overflow, "tdp_gm_search: a node buffer overflowed");
+    for (int q = 0; q <= kDmax; q++) nodes_out[q] = (int64_t)st[q];
+    stats_out[0] = (int64_t)st[kStatBase + S_LOOKUPS];
+    stats_out[1] = (int64_t)st[kStatBase + S_HITS];
+    stats_out[2] = (int64_t)st[kStatBase + S_FALSE];
+    stats_out[3] = (int64_t)st[kStatBase + S_TASKS];
+    stats_out[4] = (int64_t)st[kStatBase + S_CPRUNED];
+    stats_out[5] = (int64_t)st[kStatBase + S_APRUNED];
+    stats_out[6] = (int64_t)st[kStatBase + S_CCALLS];
+    stats_out[7] = (int64_t)h_fl->nfound;
+    *best = h_fl->best == INT_MAX ? -1 : h_fl->best;
+    *nsol = (int)std::min<unsigned>(h_fl->nsol, (unsigned)sol_cap);
+    if (*nsol > 0)
+        TDP_HIP_CHECK(hipMemcpy(sol_buf, P.sol, (size_t)*nsol * sol_stride * 4, hipMemcpyDeviceToHost));
+    stop = stop || h_fl->stop;
+    *status = timed_out ? 2 : (stop ? 1 : 0);
+    return TDP_OK;
+}
+
+}  // extern "C"

@@ -10,7 +10,7 @@
 // launch, after the owner has published.
 // Build: one thread per leading index pair (i < j) walks the remaining indices depth first, so every subset of
 // size 2..k is inserted once; singletons and the empty set come from their own launch.
-#include "tdp_common.h"
+#include "tdp_tables.h"
 
 #include <mutex>
 #include <new>
@@ -19,27 +19,7 @@
 namespace tdp {
 
 constexpr int kTabBlock = 256;
-constexpr uint64_t kFpMask = 0x7FFFFFFFFFFF0000ull;
 constexpr int kMaxK = 8;
-
-struct DevTable {
-    uint64_t mask;
-    uint64_t *key;
-    uint64_t *lgs;              // wide only
-    uint8_t *cost;
-    unsigned long long *count;
-    int k;
-    int wide;
-    uint64_t log2slots;
-};
-
-struct TabView {
-    uint64_t mask;
-    uint64_t *key;
-    uint64_t *lgs;
-    uint8_t *cost;
-    unsigned long long *count;
-};
 
 struct Retry {
     uint64_t zh, zl, lg;
@@ -226,28 +206,12 @@ __global__ void k_tab_lookup(TabView t, int wide, const uint64_t *zh, const uint
                              const int *R, uint64_t m, int *out) {
     uint64_t q = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
     if (q >= m) return;
-    uint64_t i = zh[q] & t.mask;
-    int best = -1, r = R[q];
-    uint64_t fp = zl[q] & kFpMask, wkey = zl[q] | (1ull << 63), label = lg[q];
-    for (;;) {
-        uint64_t cur = t.key[i];
-        if (cur == 0) break;
-        bool match = wide ? (cur == wkey && t.lgs[i] != label)
-                          : ((cur & kFpMask) == fp && (cur & 0xFFFF) != (label & 0xFFFF));
-        if (match) {
-            int c = t.cost[i];
-            if (c <= r && (best < 0 || c < best)) best = c;
-        }
-        i = (i + 1) & t.mask;
-    }
-    out[q] = best;
+    out[q] = tab_probe(t, wide, zh[q], zl[q], lg[q], R[q]);
 }
 
 __global__ void k_zero_words(unsigned long long *w, int n) {
     if ((int)threadIdx.x < n) w[threadIdx.x] = 0;
 }
-
-static TabView view(const DevTable *t) { return TabView{t->mask, t->key, t->lgs, t->cost, t->count}; }
 
 static unsigned tab_grid(uint64_t n) { return (unsigned)((n + kTabBlock - 1) / kTabBlock); }
 
