@@ -432,7 +432,16 @@ int tdp_gm_num_threads(void);
  * reach their expansion (the reference's tasks), cover prunes, ascent prunes, cover tests, solutions found.  Nodes
  * per depth, hits, false hits, tasks and prunes equal the reference's on an exhausted run; lookups and cover tests
  * count every probe and test once (the reference counts its tasks' twice and skips probes its prefix phase does not
- * make).  The returned int is TDP_OK unless the arguments are invalid or the device fails. */
+ * make).  The returned int is TDP_OK unless the arguments are invalid or the device fails.  The device buffers a
+ * search needs are kept for the next one (tdp_gm_release frees them); searches on one device run one at a time. */
+/* Nodes expanded per device launch of the next searches (0: the default, 2^19).  No counterpart in the reference:
+ * a tuning knob of the device search. */
+void tdp_gm_set_chunk(int nodes);
+/* Device launches of the calling thread's last tdp_gm_search.  No counterpart in the reference. */
+int tdp_gm_last_launches(void);
+/* Frees the device buffers the searches keep between calls on the current device (node levels, scratch, uploads).
+ * No counterpart in the reference. */
+void tdp_gm_release(void);
 int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint64_t *lg, const int *row_start,
                   const int *row_cols, const int *col_start, const int *col_rows, int ntab, void **tabs,
                   const uint64_t **tzh, const uint64_t **tzl, int main_tab, const int *rank, int n_anchor,

@@ -84,6 +84,9 @@ SIGNATURES = {
     "tdp_gm_set_full_labels": (None, [vp, ci]),
     "tdp_gm_set_pattern": (None, [vp, vp, ci, ci]),
     "tdp_gm_num_threads": (ci, []),
+    "tdp_gm_set_chunk": (None, [ci]),
+    "tdp_gm_last_launches": (ci, []),
+    "tdp_gm_release": (None, []),
     "tdp_gm_search": (ci, [ci, ci, ci, ci, vp, vp, vp, vp, vp, vp, ci, vp, vp, vp, ci, vp, ci, vp, vp, vp, ci, ci, ci,
                            ctypes.c_double, vp, vp, vp, ci, ci, vp, vp, vp]),
     "tdp_batch_run": (ci, [vp, vp, ci, vp, ci, vp]),

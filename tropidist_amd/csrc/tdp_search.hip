@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <climits>
+#include <mutex>
 #include <vector>
 
 namespace tdp {
@@ -33,7 +34,9 @@ constexpr int kDmax = 40;          // deepest threshold (cdist/gmitm.c:31)
 constexpr int kTmax = 4;           // lookup tables per search
 constexpr int kCvMax = 512;        // defect rows the dual ascent handles
 constexpr int kInvMax = 1024;
-constexpr int kBlock = 256;
+constexpr int kBlock = 128;
+constexpr int kIncLdsBytes = 32 << 10;   // incidence arrays staged in LDS up to this size
+constexpr int kStageWords = 24;           // syndrome + forbidden words staged in LDS up to this many
 constexpr int kCompleteMax = 8;    // completion depth: the deepest table
 
 // per-launch counters, summed into the run's statistics
@@ -57,6 +60,8 @@ struct Level {
 
 struct GmDev {
     int W, N, M, wmax, K, td, enum_all, kernel_mode, lw, cover, ascent, ntab, main_tab;
+    int nnz, inc_lds, inc_words, stage;
+    int cvmax;                     // defect rows per slot of the ascent arrays: min(kCvMax, wmax * K) >= pc
     int off_hdr, off_path, off_s, off_zh, off_zl, off_lg, off_forb, off_used;
     double inv_wmax;
     const uint64_t *col, *lg, *full_lg;
@@ -86,17 +91,30 @@ __device__ __forceinline__ bool stopped(const GmFlags *fl) {
     return __hip_atomic_load(&fl->stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
 }
 
-__device__ __forceinline__ bool is_forb(const GmDev &P, const Level &L, uint32_t i, int f) {
-    return (NODE(L, P.off_forb + (f >> 6), i) >> (f & 63)) & 1;
-}
+// What one thread reads of the node it visits: its syndrome and forbidden words (staged in LDS, or in the level
+// buffer), the row/column incidence (LDS or global), and the node's index for the rest of its words.
+struct Ctx {
+    const uint64_t *s;
+    unsigned ss;                   // syndrome word w at s[w * ss]
+    const uint64_t *fb;
+    unsigned fs;                   // forbidden word w at fb[w * fs]
+    const int *rs, *rc, *cs, *cr;  // row_start, row_cols, col_start, col_rows
+    uint32_t i;
+};
+
+__device__ __forceinline__ uint64_t syn(const Ctx &C, int w) { return C.s[(size_t)w * C.ss]; }
+
+__device__ __forceinline__ uint64_t forb_word(const Ctx &C, int w) { return C.fb[(size_t)w * C.fs]; }
+
+__device__ __forceinline__ bool is_forb(const Ctx &C, int f) { return (forb_word(C, f >> 6) >> (f & 63)) & 1; }
 
 __device__ __forceinline__ int path_at(const GmDev &P, const Level &L, uint32_t i, int p) {
     return (int)((NODE(L, P.off_path + (p >> 2), i) >> ((p & 3) * 16)) & 0xFFFF);
 }
 
-__device__ __forceinline__ int free_in_row(const GmDev &P, const Level &L, uint32_t i, int r) {
+__device__ __forceinline__ int free_in_row(const Ctx &C, int r) {
     int av = 0;
-    for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) av += !is_forb(P, L, i, P.row_cols[q]);
+    for (int q = C.rs[r]; q < C.rs[r + 1]; q++) av += !is_forb(C, C.rc[q]);
     return av;
 }
 
@@ -137,7 +155,8 @@ __device__ void record(const GmDev &P, GmFlags *fl, const Level &L, uint32_t i, 
 // Every B (|B| <= R) that cancels the node's syndrome, with a nonzero total label, disjoint from the node's path
 // (cdist/gmitm.c:444-482): branch on the lowest residual row; a column taken at a level stays excluded for its
 // later siblings and their subtrees.  Returns the number of completions recorded.
-__device__ int complete(const GmDev &P, const Level &L, uint32_t i, int a, int R, GmFlags *fl) {
+__device__ int complete(const GmDev &P, const Level &L, const Ctx &C, int a, int R, GmFlags *fl) {
+    const uint32_t i = C.i;
     int B[kCompleteMax + 1], rowB[kCompleteMax + 1], posB[kCompleteMax + 1];
     int found = 0, lev = 0;
     bool enter = true;
@@ -147,7 +166,7 @@ __device__ int complete(const GmDev &P, const Level &L, uint32_t i, int a, int R
             bool descend = false;
             int pc = 0, r = -1;
             for (int w = 0; w < P.W; w++) {
-                uint64_t x = NODE(L, P.off_s + w, i);
+                uint64_t x = syn(C, w);
                 for (int b = 0; b < lev; b++) x ^= P.col[(size_t)B[b] * P.W + w];
                 pc += __popcll(x);
                 if (r < 0 && x) r = w * 64 + __builtin_ctzll(x);
@@ -161,7 +180,7 @@ __device__ int complete(const GmDev &P, const Level &L, uint32_t i, int a, int R
                 }
             } else if (R - lev > 0 && pc <= P.wmax * (R - lev)) {
                 rowB[lev] = r;
-                posB[lev] = P.row_start[r] - 1;
+                posB[lev] = C.rs[r] - 1;
                 descend = true;
             }
             if (!descend) {
@@ -170,57 +189,60 @@ __device__ int complete(const GmDev &P, const Level &L, uint32_t i, int a, int R
             }
         }
         if (stopped(fl)) return found;
-        {
-            const int r = rowB[lev], end = P.row_start[r + 1];
-            int q = posB[lev] + 1, f = -1;
-            for (; q < end; q++) {
-                const int c = P.row_cols[q];
-                bool excluded = false;
-                for (int p = 0; p < a && !excluded; p++) excluded = path_at(P, L, i, p) == c;
-                for (int l = 0; l < lev && !excluded; l++)
-                    for (int t = P.row_start[rowB[l]]; t <= posB[l]; t++)
-                        if (P.row_cols[t] == c) { excluded = true; break; }
-                if (!excluded) { f = c; break; }
+        const int r = rowB[lev], end = C.rs[r + 1];
+        int q = posB[lev] + 1, f = -1;
+        for (; q < end; q++) {
+            const int c = C.rc[q];
+            bool excluded = false;
+            for (int p = 0; p < a && !excluded; p++) excluded = path_at(P, L, i, p) == c;
+            for (int l = 0; l < lev && !excluded; l++)
+                for (int t = C.rs[rowB[l]]; t <= posB[l]; t++)
+                    if (C.rc[t] == c) {
+                        excluded = true;
+                        break;
+                    }
+            if (!excluded) {
+                f = c;
+                break;
             }
-            if (f < 0) {
-                if (lev == 0) return found;
-                lev--;
-                continue;
-            }
-            posB[lev] = q;
-            B[lev] = f;
-            lev++;
-            enter = true;
         }
+        if (f < 0) {
+            if (lev == 0) return found;
+            lev--;
+            continue;
+        }
+        posB[lev] = q;
+        B[lev] = f;
+        lev++;
+        enter = true;
     }
 }
 
 // Covering bound (cdist/gmitm.c:484-596): a feasible dual y of the covering LP of the residual rows by the free
 // columns; true if it exceeds R.  Same operations in the same order as the reference, in double precision.
-__device__ bool cover_prune(const GmDev &P, const Level &L, uint32_t i, int R, int pc, uint32_t slot,
-                            unsigned &apruned) {
+__device__ bool cover_prune(const GmDev &P, const Ctx &C, int R, int pc, uint32_t slot, unsigned &apruned) {
     double sum = 0.0;
     const double lim = R + 1e-9, inv_wmax = P.inv_wmax;
     int left = pc;
     const bool ascent = P.ascent && pc <= kCvMax;
-    int *drow = P.cv_drow + (size_t)slot * kCvMax;
-    double *dy = P.cv_dy + (size_t)slot * kCvMax;
+    int *drow = P.cv_drow + (size_t)slot * P.cvmax;
+    double *dy = P.cv_dy + (size_t)slot * P.cvmax;
     int nd = 0;
     if (left * inv_wmax > lim) return true;
     for (int w = 0; w < P.W; w++) {
-        uint64_t x = NODE(L, P.off_s + w, i);
+        uint64_t x = syn(C, w);
         while (x) {
             const int r = w * 64 + __builtin_ctzll(x);
             x &= x - 1;
             left--;
             int m = 0;
-            for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
-                const int f = P.row_cols[q];
-                if (is_forb(P, L, i, f)) continue;
+            for (int q = C.rs[r]; q < C.rs[r + 1]; q++) {
+                const int f = C.rc[q];
+                if (is_forb(C, f)) continue;
                 int o = 0;
-                for (int t = P.col_start[f]; t < P.col_start[f + 1]; t++) {
-                    const int rr = P.col_rows[t];
-                    o += (int)((NODE(L, P.off_s + (rr >> 6), i) >> (rr & 63)) & 1);
+                for (int t = C.cs[f]; t < C.cs[f + 1]; t++) {
+                    const int rr = C.cr[t];
+                    o += (int)((syn(C, rr >> 6) >> (rr & 63)) & 1);
                 }
                 m = o > m ? o : m;
             }
@@ -247,9 +269,9 @@ __device__ bool cover_prune(const GmDev &P, const Level &L, uint32_t i, int R, i
     for (int d = 0; d < nd; d++) {
         const int r = drow[d];
         const double y = dy[d];
-        for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
-            const int f = P.row_cols[q];
-            if (is_forb(P, L, i, f)) continue;
+        for (int q = C.rs[r]; q < C.rs[r + 1]; q++) {
+            const int f = C.rc[q];
+            if (is_forb(C, f)) continue;
             if (stamp[f] != id) {
                 stamp[f] = id;
                 load[f] = y;
@@ -259,10 +281,10 @@ __device__ bool cover_prune(const GmDev &P, const Level &L, uint32_t i, int R, i
         }
     }
     // rows with the fewest free columns first, ties in row order
-    int *av = P.cv_av + (size_t)slot * kCvMax;
-    int *ord = P.cv_ord + (size_t)slot * kCvMax;
+    int *av = P.cv_av + (size_t)slot * P.cvmax;
+    int *ord = P.cv_ord + (size_t)slot * P.cvmax;
     for (int d = 0; d < nd; d++) {
-        av[d] = free_in_row(P, L, i, drow[d]);
+        av[d] = free_in_row(C, drow[d]);
         int j = d;
         while (j > 0 && av[ord[j - 1]] > av[d]) {
             ord[j] = ord[j - 1];
@@ -273,16 +295,16 @@ __device__ bool cover_prune(const GmDev &P, const Level &L, uint32_t i, int R, i
     for (int t = 0; t < nd; t++) {
         const int r = drow[ord[t]];
         double slackv = 1.0;
-        for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
-            const int f = P.row_cols[q];
-            if (is_forb(P, L, i, f)) continue;
+        for (int q = C.rs[r]; q < C.rs[r + 1]; q++) {
+            const int f = C.rc[q];
+            if (is_forb(C, f)) continue;
             const double s = 1.0 - load[f];
             if (s < slackv) slackv = s;
         }
         if (slackv <= 1e-12) continue;
-        for (int q = P.row_start[r]; q < P.row_start[r + 1]; q++) {
-            const int f = P.row_cols[q];
-            if (!is_forb(P, L, i, f)) load[f] += slackv;
+        for (int q = C.rs[r]; q < C.rs[r + 1]; q++) {
+            const int f = C.rc[q];
+            if (!is_forb(C, f)) load[f] += slackv;
         }
         sum += slackv;
         if (sum > lim) {
@@ -293,75 +315,128 @@ __device__ bool cover_prune(const GmDev &P, const Level &L, uint32_t i, int R, i
     return false;
 }
 
-__device__ __forceinline__ void set_bit(const GmDev &P, const Level &L, uint32_t j, int off, int b) {
-    NODE(L, off + (b >> 6), j) |= 1ull << (b & 63);
+// Does column f (joining a path of length a) close kernel set s: |path + f| meets s in exactly floor(|s|/2)?
+__device__ bool closes_set(const GmDev &P, const Level &in, uint32_t i, int a, int f, int s) {
+    const int b0 = P.ks_start[s], b1 = P.ks_start[s + 1];
+    int cnt = 0;
+    for (int t = b0; t < b1; t++) {
+        const int g = P.ks_cols[t];
+        bool in_path = g == f;
+        for (int p = 0; p < a && !in_path; p++) in_path = path_at(P, in, i, p) == g;
+        cnt += in_path;
+    }
+    return cnt == (b1 - b0) / 2;
 }
 
 // Child of node i (depth a) by column f into slot j of `out`.  pos >= 0: the general branch, whose child forbids
-// the pattern-admissible columns of row piv up to position pos, the exclusion partners of f and the kernel sets f
-// closes; pos < 0: a leaf-probe hit, which forbids f only (it is completed, never expanded).
-__device__ void write_child(const GmDev &P, const Level &in, uint32_t i, int a, const Level &out, uint32_t j, int f,
-                            int how, int piv, int pos) {
-    uint64_t hdr = NODE(in, P.off_hdr, i);
-    if (how == 2) hdr += 1;
-    NODE(out, P.off_hdr, j) = hdr;
-    for (int w = P.off_path; w < P.off_s; w++) NODE(out, w, j) = NODE(in, w, i);
-    {
-        uint64_t &pw = NODE(out, P.off_path + (a >> 2), j);
-        const int sh = (a & 3) * 16;
-        pw = (pw & ~(0xFFFFull << sh)) | ((uint64_t)f << sh);
+// the pattern-admissible columns of row piv before position pos, f itself, the exclusion partners of f and the
+// kernel sets f closes; pos < 0: a leaf-probe hit, which forbids f only (it is completed, never expanded).  Each
+// word is assembled in registers and stored once.
+__device__ void write_child(const GmDev &P, const Level &in, const Ctx &C, int a, const Level &out, uint32_t j,
+                            int f, int how, int piv, int pos, int nwords) {
+    const uint32_t i = C.i;
+    NODE(out, P.off_hdr, j) = NODE(in, P.off_hdr, i) + (how == 2 ? 1 : 0);
+    for (int w = 0; w < P.off_s - P.off_path; w++) {
+        uint64_t v = NODE(in, P.off_path + w, i);
+        if (w == (a >> 2)) {
+            const int sh = (a & 3) * 16;
+            v = (v & ~(0xFFFFull << sh)) | ((uint64_t)f << sh);
+        }
+        NODE(out, P.off_path + w, j) = v;
     }
-    for (int w = 0; w < P.W; w++) NODE(out, P.off_s + w, j) = NODE(in, P.off_s + w, i) ^ P.col[(size_t)f * P.W + w];
+    for (int w = 0; w < P.W; w++) NODE(out, P.off_s + w, j) = syn(C, w) ^ P.col[(size_t)f * P.W + w];
     for (int t = 0; t < P.ntab; t++) {
         NODE(out, P.off_zh + t, j) = NODE(in, P.off_zh + t, i) ^ P.tzh[t][f];
         NODE(out, P.off_zl + t, j) = NODE(in, P.off_zl + t, i) ^ P.tzl[t][f];
     }
     NODE(out, P.off_lg, j) = NODE(in, P.off_lg, i) ^ P.lg[f];
-    for (int w = P.off_forb; w < P.off_used; w++) NODE(out, w, j) = NODE(in, w, i);
-    set_bit(P, out, j, P.off_forb, f);
-    if (pos >= 0) {
-        for (int q = P.row_start[piv]; q < pos; q++) {
-            const int g = P.row_cols[q];
-            if (!is_forb(P, in, i, g) && pat_how(P, in, i, g)) set_bit(P, out, j, P.off_forb, g);
-        }
-        if (P.ex_start)
-            for (int q = P.ex_start[f]; q < P.ex_start[f + 1]; q++) set_bit(P, out, j, P.off_forb, P.ex_cols[q]);
-        for (int q = P.nks ? P.kc_start[f] : 0; P.nks && q < P.kc_start[f + 1]; q++) {
-            const int s = P.kc_ids[q];
-            const int b0 = P.ks_start[s], b1 = P.ks_start[s + 1];
-            int cnt = 0;
-            for (int t = b0; t < b1; t++) {
-                const int g = P.ks_cols[t];
-                bool in_path = g == f;
-                for (int p = 0; p < a && !in_path; p++) in_path = path_at(P, in, i, p) == g;
-                cnt += in_path;
+    const int Nw = P.off_used - P.off_forb;
+    for (int w = 0; w < Nw; w++) {
+        uint64_t v = forb_word(C, w);
+        if ((f >> 6) == w) v |= 1ull << (f & 63);
+        if (pos >= 0) {
+            for (int q = C.rs[piv]; q < pos; q++) {
+                const int g = C.rc[q];
+                if ((g >> 6) == w && !is_forb(C, g) && pat_how(P, in, i, g)) v |= 1ull << (g & 63);
             }
-            if (cnt == (b1 - b0) / 2)
-                for (int t = b0; t < b1; t++) set_bit(P, out, j, P.off_forb, P.ks_cols[t]);
+            if (P.ex_start)
+                for (int q = P.ex_start[f]; q < P.ex_start[f + 1]; q++) {
+                    const int g = P.ex_cols[q];
+                    if ((g >> 6) == w) v |= 1ull << (g & 63);
+                }
+            for (int q = P.nks ? P.kc_start[f] : 0; P.nks && q < P.kc_start[f + 1]; q++) {
+                const int s = P.kc_ids[q];
+                if (!closes_set(P, in, i, a, f, s)) continue;
+                for (int t = P.ks_start[s]; t < P.ks_start[s + 1]; t++) {
+                    const int g = P.ks_cols[t];
+                    if ((g >> 6) == w) v |= 1ull << (g & 63);
+                }
+            }
         }
+        NODE(out, P.off_forb + w, j) = v;
+    }
+    for (int w = P.off_used; w < nwords; w++) {
+        uint64_t v = NODE(in, w, i);
+        if (how == 1 && P.cls && (P.cls[f] >> 6) == w - P.off_used) v |= 1ull << (P.cls[f] & 63);
+        NODE(out, w, j) = v;
     }
 }
 
-__device__ __forceinline__ void copy_used(const GmDev &P, const Level &in, uint32_t i, const Level &out, uint32_t j,
-                                          int nwords, int f, int how) {
-    for (int w = P.off_used; w < nwords; w++) NODE(out, w, j) = NODE(in, w, i);
-    if (how == 1 && P.cls) set_bit(P, out, j, P.off_used, P.cls[f]);
-}
-
 // One launch: nodes [first, first + n) of depth a (buffer `in`) are visited; children go to `out` (depth a + 1).
+// Dynamic LDS: the incidence arrays when P.inc_lds, then the visited nodes' syndrome and forbidden words (word w of
+// thread t at w * kBlock + t) when P.stage.
 __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level in, uint32_t first, uint32_t n,
                                                       int a, const Level out, int nwords, GmFlags *fl,
                                                       unsigned long long *stats) {
+    extern __shared__ uint64_t lds_dyn[];
     __shared__ unsigned cnt[S_N];
     __shared__ unsigned nchild_sh[kBlock];
     __shared__ unsigned base_sh;
     const int tid = threadIdx.x;
     if (tid < S_N) cnt[tid] = 0;
     if (blockIdx.x == 0 && tid == 0) atomicSub(&fl->count[a], n);
-    __syncthreads();
 
     const uint32_t t = blockIdx.x * kBlock + tid;
     const uint32_t i = first + t;
+    const bool live = t < n;
+    Ctx C;
+    C.i = i;
+    C.rs = P.row_start;
+    C.rc = P.row_cols;
+    C.cs = P.col_start;
+    C.cr = P.col_rows;
+    uint64_t *stage = lds_dyn;
+    if (P.inc_lds) {
+        int *inc = reinterpret_cast<int *>(lds_dyn);
+        const int nnz = P.nnz;
+        for (int q = tid; q <= P.M; q += kBlock) inc[q] = P.row_start[q];
+        for (int q = tid; q < nnz; q += kBlock) inc[P.M + 1 + q] = P.row_cols[q];
+        for (int q = tid; q <= P.N; q += kBlock) inc[P.M + 1 + nnz + q] = P.col_start[q];
+        for (int q = tid; q < nnz; q += kBlock) inc[P.M + P.N + 2 + nnz + q] = P.col_rows[q];
+        C.rs = inc;
+        C.rc = inc + P.M + 1;
+        C.cs = inc + P.M + 1 + nnz;
+        C.cr = inc + P.M + P.N + 2 + nnz;
+        stage = lds_dyn + P.inc_words;
+    }
+    const int Nw = P.off_used - P.off_forb;
+    if (P.stage) {
+        if (live) {
+            for (int w = 0; w < P.W; w++) stage[w * kBlock + tid] = NODE(in, P.off_s + w, i);
+            for (int w = 0; w < Nw; w++) stage[(P.W + w) * kBlock + tid] = NODE(in, P.off_forb + w, i);
+        }
+        C.s = stage + tid;
+        C.ss = kBlock;
+        C.fb = stage + P.W * kBlock + tid;
+        C.fs = kBlock;
+    } else {
+        C.s = in.base + (size_t)P.off_s * in.cap + i;
+        C.ss = in.cap;
+        C.fb = in.base + (size_t)P.off_forb * in.cap + i;
+        C.fs = in.cap;
+    }
+    __syncthreads();
+
     unsigned c_lookups = 0, c_hits = 0, c_false = 0, c_tasks = 0, c_cpr = 0, c_apr = 0, c_cc = 0, c_na = 0,
              c_na1 = 0;
     int piv = -1;          // >= 0: general branch over row piv
@@ -369,10 +444,10 @@ __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level
     const int R = P.K - a;
     const bool leaf = P.main_tab >= 0 && R - 1 <= P.tab_k[P.main_tab];
 
-    if (t < n && !stopped(fl)) {
+    if (live && !stopped(fl)) {
         c_na = 1;
         int pc = 0;
-        for (int w = 0; w < P.W; w++) pc += __popcll(NODE(in, P.off_s + w, i));
+        for (int w = 0; w < P.W; w++) pc += __popcll(syn(C, w));
         bool go = true;
         if (pc == 0) {
             if (sol_nonzero(P, in, i, a, nullptr, 0, NODE(in, P.off_lg, i)) != (bool)P.kernel_mode)
@@ -382,7 +457,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level
             go = false;
         } else if (P.cover && pc > R && 4 * pc >= P.cover * R && (P.main_tab < 0 || R > P.tab_k[P.main_tab])) {
             c_cc = 1;
-            if (cover_prune(P, in, i, R, pc, t, c_apr)) {
+            if (cover_prune(P, C, R, pc, t, c_apr)) {
                 c_cpr = 1;
                 go = false;
             }
@@ -397,7 +472,7 @@ __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level
                 go = false;
             } else if (tb == P.main_tab) {
                 // the remainder exists up to a fingerprint collision: derive it, with the full budget R
-                if (complete(P, in, i, a, R, fl) > 0) c_hits++;
+                if (complete(P, in, C, a, R, fl) > 0) c_hits++;
                 else c_false++;
                 go = false;
             }
@@ -406,11 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level
         if (go) {
             int best = INT_MAX;
             for (int w = 0; w < P.W; w++) {
-                uint64_t x = NODE(in, P.off_s + w, i);
+                uint64_t x = syn(C, w);
                 while (x) {
                     const int r = w * 64 + __builtin_ctzll(x);
                     x &= x - 1;
-                    const int av = free_in_row(P, in, i, r);
+                    const int av = free_in_row(C, r);
                     if (av < best) {
                         best = av;
                         piv = r;
@@ -422,9 +497,9 @@ __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level
         if (piv >= 0 && leaf) {
             // children decided by one probe each: store only the hits
             const int mt = P.main_tab;
-            for (int q = P.row_start[piv]; q < P.row_start[piv + 1]; q++) {
-                const int f = P.row_cols[q];
-                if (is_forb(P, in, i, f)) continue;
+            for (int q = C.rs[piv]; q < C.rs[piv + 1]; q++) {
+                const int f = C.rc[q];
+                if (is_forb(C, f)) continue;
                 const int how = pat_how(P, in, i, f);
                 if (!how) continue;
                 const uint64_t zh = NODE(in, P.off_zh + mt, i) ^ P.tzh[mt][f];
@@ -440,14 +515,13 @@ __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level
                     __hip_atomic_store(&fl->overflow, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     continue;
                 }
-                write_child(P, in, i, a, out, j, f, how, piv, -1);
-                copy_used(P, in, i, out, j, nwords, f, how);
+                write_child(P, in, C, a, out, j, f, how, piv, -1, nwords);
             }
             piv = -1;
         } else if (piv >= 0) {
-            for (int q = P.row_start[piv]; q < P.row_start[piv + 1]; q++) {
-                const int f = P.row_cols[q];
-                if (!is_forb(P, in, i, f) && pat_how(P, in, i, f)) nchild++;
+            for (int q = C.rs[piv]; q < C.rs[piv + 1]; q++) {
+                const int f = C.rc[q];
+                if (!is_forb(C, f) && pat_how(P, in, i, f)) nchild++;
             }
         }
     }
@@ -466,17 +540,16 @@ __global__ __launch_bounds__(kBlock) void k_gm_expand(const GmDev P, const Level
     __syncthreads();
     if (nchild) {
         uint32_t j = base_sh + nchild_sh[tid];
-        for (int q = P.row_start[piv]; q < P.row_start[piv + 1]; q++) {
-            const int f = P.row_cols[q];
-            if (is_forb(P, in, i, f)) continue;
+        for (int q = C.rs[piv]; q < C.rs[piv + 1]; q++) {
+            const int f = C.rc[q];
+            if (is_forb(C, f)) continue;
             const int how = pat_how(P, in, i, f);
             if (!how) continue;
             if (j >= out.cap) {
                 __hip_atomic_store(&fl->overflow, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
-            write_child(P, in, i, a, out, j, f, how, piv, q);
-            copy_used(P, in, i, out, j, nwords, f, how);
+            write_child(P, in, C, a, out, j, f, how, piv, q, nwords);
             j++;
         }
     }
@@ -523,37 +596,63 @@ struct Options {
     const int *cls = nullptr;
     const uint8_t *onpat = nullptr;
     int nclass = 0, slack = 0;
+    int chunk = 0;                 // nodes per launch; 0: the default
 };
 thread_local Options g_opt;
+thread_local int g_last_launches = 0, g_last_chunk = 0;
 
-// device copies of the run's host arrays, freed together
-struct DevArena {
-    std::vector<void *> blocks;
-    ~DevArena() {
-        for (void *p : blocks) (void)hipFree(p);
-    }
-    template <typename T>
-    int put(const T *h, size_t n, const T **out) {
-        *out = nullptr;
-        if (!h || !n) return TDP_OK;
-        void *d = nullptr;
-        TDP_HIP_CHECK(hipMalloc(&d, n * sizeof(T)));
-        blocks.push_back(d);
-        TDP_HIP_CHECK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
-        *out = static_cast<const T *>(d);
-        return TDP_OK;
-    }
-    int alloc(size_t bytes, void **out) {
-        *out = nullptr;
-        if (hipMalloc(out, bytes ? bytes : 8) != hipSuccess) {
-            *out = nullptr;
-            set_error("device allocation of %zu bytes for the search failed", bytes);
+// Device buffers of the searches, kept per device between calls and grown when a search needs more (the levels'
+// node buffers and the dual-ascent scratch run to hundreds of MB; allocating them per call would cost more than a
+// small search).  One search at a time per device: the cache's mutex is held for the whole call.
+enum {
+    B_COL, B_LG, B_FULL, B_RS, B_RC, B_CS, B_CR, B_TZH, B_TZL = B_TZH + kTmax, B_EXS = B_TZL + kTmax, B_EXC, B_KSS,
+    B_KSC, B_KCS, B_KCI, B_CLS, B_ONP, B_INV, B_LOAD, B_STAMP, B_CVID, B_DROW, B_DY, B_AV, B_ORD, B_FLAGS, B_STATS,
+    B_SOL, B_LEVEL, B_COUNT = B_LEVEL + kDmax + 2
+};
+
+struct GmCache {
+    std::mutex mu;
+    void *ptr[B_COUNT] = {};
+    size_t cap[B_COUNT] = {};
+    GmFlags *host_flags = nullptr;
+    int stamp_n = -1;              // column count of the stamps' slot layout (slot * N + column)
+};
+constexpr int kMaxDevices = 64;
+GmCache g_cache[kMaxDevices];
+
+// at least `bytes` of device memory in slot b; *fresh: newly allocated (contents undefined)
+int cache_buf(GmCache &c, int b, size_t bytes, void **out, bool *fresh = nullptr) {
+    if (fresh) *fresh = false;
+    if (bytes == 0) bytes = 8;
+    if (c.cap[b] < bytes) {
+        const size_t want = std::max(bytes, c.cap[b] + c.cap[b] / 2);
+        if (c.ptr[b]) (void)hipFree(c.ptr[b]);
+        c.ptr[b] = nullptr;
+        c.cap[b] = 0;
+        if (hipMalloc(&c.ptr[b], want) != hipSuccess) {
+            c.ptr[b] = nullptr;
+            (void)hipGetLastError();
+            set_error("device allocation of %zu bytes for the search failed", want);
             return TDP_ENOMEM;
         }
-        blocks.push_back(*out);
-        return TDP_OK;
+        c.cap[b] = want;
+        if (fresh) *fresh = true;
     }
-};
+    *out = c.ptr[b];
+    return TDP_OK;
+}
+
+template <typename T>
+int cache_put(GmCache &c, int b, const T *h, size_t n, const T **out) {
+    *out = nullptr;
+    if (!h || !n) return TDP_OK;
+    void *d = nullptr;
+    int rc = cache_buf(c, b, n * sizeof(T), &d);
+    if (rc != TDP_OK) return rc;
+    TDP_HIP_CHECK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+    *out = static_cast<const T *>(d);
+    return TDP_OK;
+}
 
 #define GM_TRY(expr)                 \
     do {                             \
@@ -595,7 +694,7 @@ void tdp_gm_set_pattern(const int *cls, const uint8_t *onpat, int nclass, int sl
     g_opt.slack = slack;
 }
 
-int tdp_gm_num_threads(void) { return 65536; }
+int tdp_gm_num_threads(void) { return 1 << 19; }
 
 int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint64_t *lg, const int *row_start,
                   const int *row_cols, const int *col_start, const int *col_rows, int ntab, void **tabs,
@@ -614,6 +713,7 @@ int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint
         return TDP_OK;
     }
     const Options opt = g_opt;
+    g_last_launches = 0;
     TDP_REQUIRE(W >= 1 && N >= 1 && M >= 1 && M <= 64 * W && N < 65536, "tdp_gm_search: W %d N %d M %d", W, N, M);
     TDP_REQUIRE(ntab >= 0 && ntab <= kTmax && main_tab < ntab && K >= 0 && n_anchor >= 0,
                 "tdp_gm_search: %d tables (main %d), K %d", ntab, main_tab, K);
@@ -653,17 +753,25 @@ int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint
     P.sol_cap = sol_cap;
     P.sol_stride = sol_stride;
 
-    DevArena D;
+    int dev = 0;
+    TDP_HIP_CHECK(hipGetDevice(&dev));
+    TDP_REQUIRE(dev >= 0 && dev < kMaxDevices, "tdp_gm_search: device %d beyond %d", dev, kMaxDevices);
+    GmCache &G = g_cache[dev];
+    std::lock_guard<std::mutex> hold(G.mu);
+
     const size_t nnz = (size_t)row_start[M];
-    GM_TRY(D.put(col, (size_t)N * W, &P.col));
+    TDP_REQUIRE((size_t)col_start[N] == nnz, "tdp_gm_search: row and column incidence disagree");
+    P.nnz = (int)nnz;
+    GM_TRY(cache_put(G, B_COL, col, (size_t)N * W, &P.col));
     std::vector<uint64_t> zero_lg;
     if (!lg) zero_lg.assign(N, 0);
-    GM_TRY(D.put(lg ? lg : zero_lg.data(), (size_t)N, &P.lg));
-    if (opt.full_lg) GM_TRY(D.put(opt.full_lg, (size_t)N * opt.lw, &P.full_lg));
-    GM_TRY(D.put(row_start, (size_t)M + 1, &P.row_start));
-    GM_TRY(D.put(row_cols, nnz ? nnz : 1, &P.row_cols));
-    GM_TRY(D.put(col_start, (size_t)N + 1, &P.col_start));
-    GM_TRY(D.put(col_rows, (size_t)col_start[N] ? (size_t)col_start[N] : 1, &P.col_rows));
+    GM_TRY(cache_put(G, B_LG, lg ? lg : zero_lg.data(), (size_t)N, &P.lg));
+    if (opt.full_lg) GM_TRY(cache_put(G, B_FULL, opt.full_lg, (size_t)N * opt.lw, &P.full_lg));
+    const int one = 0;
+    GM_TRY(cache_put(G, B_RS, row_start, (size_t)M + 1, &P.row_start));
+    GM_TRY(cache_put(G, B_RC, nnz ? row_cols : &one, nnz ? nnz : 1, &P.row_cols));
+    GM_TRY(cache_put(G, B_CS, col_start, (size_t)N + 1, &P.col_start));
+    GM_TRY(cache_put(G, B_CR, nnz ? col_rows : &one, nnz ? nnz : 1, &P.col_rows));
     for (int t = 0; t < ntab; t++) {
         DevTable *tb = static_cast<DevTable *>(tabs[t]);
         TDP_REQUIRE(tb != nullptr, "tdp_gm_search: table %d is null", t);
@@ -672,67 +780,89 @@ int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint
         P.tab_k[t] = tb->k;
         TDP_REQUIRE(t != main_tab || tb->k <= kCompleteMax, "tdp_gm_search: exact table depth %d above %d", tb->k,
                     kCompleteMax);
-        GM_TRY(D.put(tzh[t], (size_t)N, &P.tzh[t]));
-        GM_TRY(D.put(tzl[t], (size_t)N, &P.tzl[t]));
+        GM_TRY(cache_put(G, B_TZH + t, tzh[t], (size_t)N, &P.tzh[t]));
+        GM_TRY(cache_put(G, B_TZL + t, tzl[t], (size_t)N, &P.tzl[t]));
     }
     if (ex_start) {
-        GM_TRY(D.put(ex_start, (size_t)N + 1, &P.ex_start));
-        GM_TRY(D.put(ex_cols, (size_t)std::max(ex_start[N], 1), &P.ex_cols));
+        GM_TRY(cache_put(G, B_EXS, ex_start, (size_t)N + 1, &P.ex_start));
+        GM_TRY(cache_put(G, B_EXC, ex_cols, (size_t)std::max(ex_start[N], 1), &P.ex_cols));
     }
     if (opt.nks > 0) {
-        GM_TRY(D.put(opt.ks_start, (size_t)opt.nks + 1, &P.ks_start));
-        GM_TRY(D.put(opt.ks_cols, (size_t)std::max(opt.ks_start[opt.nks], 1), &P.ks_cols));
-        GM_TRY(D.put(opt.kc_start, (size_t)N + 1, &P.kc_start));
-        GM_TRY(D.put(opt.kc_ids, (size_t)std::max(opt.kc_start[N], 1), &P.kc_ids));
+        GM_TRY(cache_put(G, B_KSS, opt.ks_start, (size_t)opt.nks + 1, &P.ks_start));
+        GM_TRY(cache_put(G, B_KSC, opt.ks_cols, (size_t)std::max(opt.ks_start[opt.nks], 1), &P.ks_cols));
+        GM_TRY(cache_put(G, B_KCS, opt.kc_start, (size_t)N + 1, &P.kc_start));
+        GM_TRY(cache_put(G, B_KCI, opt.kc_ids, (size_t)std::max(opt.kc_start[N], 1), &P.kc_ids));
     }
     if (opt.cls) {
-        GM_TRY(D.put(opt.cls, (size_t)N, &P.cls));
+        for (int j = 0; j < N; j++)
+            TDP_REQUIRE(opt.cls[j] >= -1 && opt.cls[j] < opt.nclass, "tdp_gm_search: class %d of column %d",
+                        opt.cls[j], j);
+        GM_TRY(cache_put(G, B_CLS, opt.cls, (size_t)N, &P.cls));
         std::vector<uint8_t> on(std::max(opt.nclass, 1), 0);
         for (int c = 0; c < opt.nclass; c++) on[c] = opt.onpat[c];
-        GM_TRY(D.put(on.data(), on.size(), &P.onpat));
+        GM_TRY(cache_put(G, B_ONP, on.data(), on.size(), &P.onpat));
     }
     std::vector<double> inv(kInvMax + 1, 0.0);
     for (int m = 1; m <= kInvMax; m++) inv[m] = 1.0 / m;
-    GM_TRY(D.put(inv.data(), inv.size(), &P.inv));
+    GM_TRY(cache_put(G, B_INV, inv.data(), inv.size(), &P.inv));
+
+    // LDS: the incidence arrays and the visited nodes' syndrome and forbidden words, when they fit
+    const size_t inc_ints = (size_t)M + N + 2 + 2 * nnz;
+    P.inc_lds = inc_ints * 4 <= (size_t)kIncLdsBytes;
+    P.inc_words = P.inc_lds ? (int)((inc_ints + 1) / 2) : 0;
+    P.stage = W + Nw <= kStageWords;
+    const size_t lds_bytes = ((size_t)P.inc_words + (P.stage ? (size_t)(W + Nw) * kBlock : 0)) * 8;
 
     int deg = 1;
     for (int r = 0; r < M; r++) deg = std::max(deg, row_start[r + 1] - row_start[r]);
     // chunk: nodes per launch.  Buffers: depth 1 holds the anchors, deeper depths chunk x degree nodes each.
-    uint32_t chunk = 65536;
+    // A node's visit is a long chain of dependent loads (tens of microseconds), so a launch takes about that long
+    // whatever its size until the device is full: the default chunk is about the device's resident threads.
+    uint32_t chunk = opt.chunk > 0 ? (uint32_t)std::max(opt.chunk, kBlock) : (1u << 19);
     const size_t node_bytes = (size_t)nwords * 8;
-    const size_t level_budget = (size_t)24 << 30;
-    while (chunk > 256 && (size_t)KL * chunk * deg * node_bytes > level_budget) chunk /= 2;
+    const size_t level_budget = (size_t)32 << 30;
+    while (chunk > (uint32_t)kBlock && (size_t)KL * chunk * deg * node_bytes > level_budget) chunk /= 2;
+    P.cvmax = std::min(kCvMax, std::max(1, wmax * KL));
     if (P.cover && P.ascent) {
-        const size_t per_slot = (size_t)N * 12 + (size_t)kCvMax * 20 + 4;
-        while (chunk > 256 && chunk * per_slot > ((size_t)2 << 30)) chunk /= 2;
+        const size_t per_slot = (size_t)N * 12 + (size_t)P.cvmax * 20 + 4;
+        while (chunk > (uint32_t)kBlock && chunk * per_slot > ((size_t)8 << 30)) chunk /= 2;
         void *p = nullptr;
-        GM_TRY(D.alloc((size_t)chunk * N * 8, &p));
+        bool f1 = false, f2 = false;
+        GM_TRY(cache_buf(G, B_LOAD, (size_t)chunk * N * 8, &p));
         P.cv_load = static_cast<double *>(p);
-        GM_TRY(D.alloc((size_t)chunk * N * 4, &p));
+        GM_TRY(cache_buf(G, B_STAMP, (size_t)chunk * N * 4, &p, &f1));
         P.cv_stamp = static_cast<unsigned *>(p);
-        TDP_HIP_CHECK(hipMemset(p, 0, (size_t)chunk * N * 4));
-        GM_TRY(D.alloc((size_t)chunk * 4, &p));
+        GM_TRY(cache_buf(G, B_CVID, (size_t)chunk * 4, &p, &f2));
         P.cv_id = static_cast<unsigned *>(p);
-        TDP_HIP_CHECK(hipMemset(p, 0, (size_t)chunk * 4));
-        GM_TRY(D.alloc((size_t)chunk * kCvMax * 4, &p));
+        // a slot's stamps are compared with that slot's own running id: they start together at zero, and again
+        // whenever the slot layout changes with N (a region would otherwise hold another slot's ids)
+        if (f1 || f2 || G.stamp_n != N) {
+            TDP_HIP_CHECK(hipMemset(G.ptr[B_STAMP], 0, G.cap[B_STAMP]));
+            TDP_HIP_CHECK(hipMemset(G.ptr[B_CVID], 0, G.cap[B_CVID]));
+            G.stamp_n = N;
+        }
+        GM_TRY(cache_buf(G, B_DROW, (size_t)chunk * P.cvmax * 4, &p));
         P.cv_drow = static_cast<int *>(p);
-        GM_TRY(D.alloc((size_t)chunk * kCvMax * 8, &p));
+        GM_TRY(cache_buf(G, B_DY, (size_t)chunk * P.cvmax * 8, &p));
         P.cv_dy = static_cast<double *>(p);
-        GM_TRY(D.alloc((size_t)chunk * kCvMax * 4, &p));
+        GM_TRY(cache_buf(G, B_AV, (size_t)chunk * P.cvmax * 4, &p));
         P.cv_av = static_cast<int *>(p);
-        GM_TRY(D.alloc((size_t)chunk * kCvMax * 4, &p));
+        GM_TRY(cache_buf(G, B_ORD, (size_t)chunk * P.cvmax * 4, &p));
         P.cv_ord = static_cast<int *>(p);
     }
     void *pv = nullptr;
-    GM_TRY(D.alloc(sizeof(GmFlags), &pv));
+    GM_TRY(cache_buf(G, B_FLAGS, sizeof(GmFlags), &pv));
     GmFlags *d_fl = static_cast<GmFlags *>(pv);
     const int nstats = kStatBase + S_N;
-    GM_TRY(D.alloc(nstats * 8, &pv));
+    GM_TRY(cache_buf(G, B_STATS, nstats * 8, &pv));
     unsigned long long *d_stats = static_cast<unsigned long long *>(pv);
     if (sol_cap > 0) {
-        GM_TRY(D.alloc((size_t)sol_cap * sol_stride * 4, &pv));
+        GM_TRY(cache_buf(G, B_SOL, (size_t)sol_cap * sol_stride * 4, &pv));
         P.sol = static_cast<int *>(pv);
     }
+    if (!G.host_flags)
+        TDP_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&G.host_flags), sizeof(GmFlags), hipHostMallocDefault));
+    GmFlags *h_fl = G.host_flags;
     k_gm_init<<<1, 128>>>(d_fl, d_stats, nstats);
     TDP_LAUNCH_CHECK();
 
@@ -772,7 +902,7 @@ int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint
             }
         }
         void *p = nullptr;
-        GM_TRY(D.alloc(h.size() * 8, &p));
+        GM_TRY(cache_buf(G, B_LEVEL + 1, h.size() * 8, &p));
         TDP_HIP_CHECK(hipMemcpy(p, h.data(), h.size() * 8, hipMemcpyHostToDevice));
         lev[1] = Level{static_cast<uint64_t *>(p), cap};
         count[1] = cap;
@@ -786,19 +916,13 @@ int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint
         }
         if (!lev[d].base) {
             void *p = nullptr;
-            GM_TRY(D.alloc((size_t)deep_cap * node_bytes, &p));
+            GM_TRY(cache_buf(G, B_LEVEL + d, (size_t)deep_cap * node_bytes, &p));
             lev[d] = Level{static_cast<uint64_t *>(p), deep_cap};
         }
         *out = lev[d];
         return TDP_OK;
     };
 
-    GmFlags *h_fl = nullptr;
-    TDP_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&h_fl), sizeof(GmFlags), hipHostMallocDefault));
-    struct PinnedFree {
-        GmFlags *p;
-        ~PinnedFree() { (void)hipHostFree(p); }
-    } pin{h_fl};
     bool timed_out = false, overflow = false, stop = false;
     const double deadline = time_limit > 0 ? time_limit : 0.0;
     while (true) {
@@ -826,10 +950,12 @@ int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint
         Level out{};
         GM_TRY(level(src + 1, &out));
         const uint32_t first = count[src] - n;
-        k_gm_expand<<<(n + kBlock - 1) / kBlock, kBlock>>>(P, lev[src], first, n, src, out, nwords, d_fl, d_stats);
+        k_gm_expand<<<(n + kBlock - 1) / kBlock, kBlock, lds_bytes>>>(P, lev[src], first, n, src, out, nwords, d_fl,
+                                                                      d_stats);
         TDP_LAUNCH_CHECK();
         TDP_HIP_CHECK(hipMemcpyAsync(h_fl, d_fl, sizeof(GmFlags), hipMemcpyDeviceToHost, 0));
         TDP_HIP_CHECK(hipStreamSynchronize(0));
+        g_last_launches++;
         if (h_fl->overflow) {
             overflow = true;
             break;
@@ -859,7 +985,27 @@ int tdp_gm_search(int W, int N, int M, int wmax, const uint64_t *col, const uint
         TDP_HIP_CHECK(hipMemcpy(sol_buf, P.sol, (size_t)*nsol * sol_stride * 4, hipMemcpyDeviceToHost));
     stop = stop || h_fl->stop;
     *status = timed_out ? 2 : (stop ? 1 : 0);
+    g_last_chunk = (int)chunk;
     return TDP_OK;
+}
+
+void tdp_gm_set_chunk(int nodes) { g_opt.chunk = nodes; }
+
+int tdp_gm_last_launches(void) { return g_last_launches; }
+
+void tdp_gm_release(void) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return;
+    GmCache &G = g_cache[dev];
+    std::lock_guard<std::mutex> hold(G.mu);
+    for (int b = 0; b < B_COUNT; b++) {
+        if (G.ptr[b]) (void)hipFree(G.ptr[b]);
+        G.ptr[b] = nullptr;
+        G.cap[b] = 0;
+    }
+    if (G.host_flags) (void)hipHostFree(G.host_flags);
+    G.host_flags = nullptr;
+    G.stamp_n = -1;
 }
 
 }  // extern "C"

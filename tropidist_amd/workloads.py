@@ -5,6 +5,9 @@
   single_large()      600 detectors, 6000 columns (2-6 detectors in a 25-row window), 4 logical rows of density
                       0.05, costs 1-3: 2^26-entry tables
   sparse_qc()         two-block quasi-cyclic code, l = m = 12 (288 columns)
+  bivariate_bicycle_288()  the [[288,12,18]] bivariate bicycle code (l = m = 12, A = x^3 + y^2 + y^7,
+                      B = y^3 + x + x^2), Z-basis code-capacity problem, with its two translation orbits as ranks
+                      and anchors
   batch_scan(n)       n banded models, 200-1200 detectors, 5-15 faults per detector, width 10-22, half unit and half
                       costs 1-3
   serving_mix(n)      the request mix of the serving loop (70% dense width 10-20, 20% width 21-25, 10% sparse QC)
@@ -27,6 +30,16 @@ def single_large(seed=2000, W=25):
 
 def sparse_qc(seed=3000, l=12, m=12):
     return problem.quasi_cyclic(l, m, seed=seed, name="cfg3_qc")
+
+
+def bivariate_bicycle_288():
+    """(problem, rank, anchors): columns 0-143 form the first block and 144-287 the second; the l x m translations
+    act transitively on each, so one anchor per block with rank = block index covers every solution."""
+    hx, hz, n = problem.two_block_matrices(12, 12, [(3, 0), (0, 2), (0, 7)], [(0, 3), (1, 0), (2, 0)])
+    p = problem.css_problem(hx, hz, n, "bb288")
+    if p.ncols != 288:
+        raise ValueError("bb288: expected 288 distinct columns, got %d" % p.ncols)
+    return p, [j // 144 for j in range(288)], [0, 144]
 
 
 def banded_instance(rng, seed, width, weighted, nobs=2):
