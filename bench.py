@@ -537,6 +537,73 @@ def bench_tree(args):
                                "sample": "the whole query: the reference's tree_dp.c joins (single-threaded) in the "
                                          "same postorder schedule", "seconds": cs, "distance": cv,
                                "matches_gpu": cv == value}
+    out["large"] = bench_tree_large(args)
+    return out
+
+
+def bench_tree_large(args):
+    """Tree joins at the scale of the single large instance's merges: banded R=150 N=600 W=13, balanced tree,
+    parents of up to 2^26 entries and joins of up to 2^39 pairs.  The widest joins run LDS-tiled (k_tree_join_tiled);
+    the bound is the LDS reads, 4 bytes per pair (two dwords per two pairs), against 256 CUs x 128 B/clk x 2.4 GHz.
+    The CPU figure is the reference's tree_dp.c on one join of the same tree (the whole query would take about half an
+    hour on one core)."""
+    import numpy as np
+    import oracle
+    from tropidist_amd import _lib, kernel, problem, trellis, tree_dp
+    t = _lib.torch()
+    p = problem.banded(R=150, N=600, kmin=2, kmax=5, W=13, nobs=1, rho=0.1, seed=10)
+    cols = [h | ((l & 1) << p.nrows) for h, l in zip(p.h, p.l)]
+    prog = tree_dp.CompiledTreeDP(cols, _balanced(0, len(cols)), max_compile_bytes=None)
+    target = 1 << p.nrows
+    lim = dict(max_join_work=1 << 44, max_memory_bytes=1 << 38, max_seconds=600.0)
+    value, stats = prog.run(target, **lim)
+    t.cuda.synchronize()
+    t1 = time.perf_counter()
+    value, stats = prog.run(target, **lim)
+    t.cuda.synchronize()
+    dt = time.perf_counter() - t1
+    pairs = stats["join_work"]
+    dense = kernel.run_plan(trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows))[0]
+    lds_peak = 256 * 128 * 2.4e9
+    out = {"workload": "banded R=150 N=600 k=2-5 W=13, 1 logical row, balanced tree over the natural order, seed 10",
+           "distance": value, "dense_distance": dense, "identical": value == dense, "join_pairs": int(pairs),
+           "max_join_dimension": stats["max_join_dimension"],
+           "max_parent_entries": 1 << max(j.parent_rank for j in prog.joins.values()), "dtype": stats["dtype"],
+           "seconds": dt, "pairs_per_s": pairs / dt,
+           "roofline": {"bound": "lds", "achieved": 4 * pairs / dt / 1e9, "peak": lds_peak / 1e9, "unit": "GB/s",
+                        "frac": 4 * pairs / dt / lds_peak}}
+    if not args.no_cpu and oracle.have_ref():
+        # one join of about 2^30 pairs, on the host tables of its real size (random finite costs)
+        j = min((j for j in prog.joins.values() if j.lmaps), key=lambda j: abs(j.join_dimension - 30))
+        rng = np.random.default_rng(1)
+        left = rng.integers(0, 20, size=1 << j.left_rank).astype(np.uint16)
+        right = rng.integers(0, 20, size=1 << j.right_rank).astype(np.uint16)
+        want = np.full(1 << j.parent_rank, prog.infinity, dtype=np.uint16)
+        lm, rm = np.array(j.lmaps, dtype=np.uint64), np.array(j.rmaps, dtype=np.uint64)
+        c0 = time.perf_counter()
+        oracle.tree_join(2, left, right, want, lm, rm, j.parent_rank, j.join_dimension, 0, 1 << j.join_dimension,
+                         prog.infinity, impl="ref")
+        cs = time.perf_counter() - c0
+        L = _lib.lib()
+        dl, dr = (t.from_numpy(x.view(np.uint8).copy()).cuda() for x in (left, right))
+        do = t.from_numpy(np.full(1 << j.parent_rank, prog.infinity, dtype=np.uint16).view(np.uint8).copy()).cuda()
+        _lib.check(L.tdp_tree_join(2, dl.data_ptr(), dr.data_ptr(), do.data_ptr(), lm.ctypes.data, rm.ctypes.data,
+                                   j.parent_rank, j.join_dimension, 0, 1 << j.join_dimension, prog.infinity,
+                                   _lib.stream_handle()), "tdp_tree_join")
+        t.cuda.synchronize()
+        g0 = time.perf_counter()
+        _lib.check(L.tdp_tree_join(2, dl.data_ptr(), dr.data_ptr(), do.data_ptr(), lm.ctypes.data, rm.ctypes.data,
+                                   j.parent_rank, j.join_dimension, 0, 1 << j.join_dimension, prog.infinity,
+                                   _lib.stream_handle()), "tdp_tree_join")
+        t.cuda.synchronize()
+        gs = time.perf_counter() - g0
+        same = bool((do.cpu().numpy().view(np.uint16) == want).all())
+        out["cpu_baseline"] = {"value": (1 << j.join_dimension) / cs, "unit": "join pairs/s", "cores": 1,
+                               "kind": "reference",
+                               "sample": "one join of the tree (J=%d, parents 2^%d, children 2^%d and 2^%d) by the "
+                                         "reference's tree_join_u16 on random costs 0-19" % (
+                                             j.join_dimension, j.parent_rank, j.left_rank, j.right_rank),
+                               "seconds": cs, "gpu_seconds_same_join": gs, "matches_gpu": same}
     return out
 
 

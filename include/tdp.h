@@ -193,6 +193,15 @@ int tdp_tree_join_u64(const void *d_left, const void *d_right, void *d_out, cons
                       unsigned parent_bits, unsigned generators, uint64_t start, uint64_t stop, uint64_t infinity,
                       void *stream);
 
+/* The tiled form tdp_tree_join gives a full join of 1- or 2-byte items (infinity <= 32767, 4..14 kernel generators
+ * with injective left and right maps, some reuse): *tiles = the number of tiles (0 when the join runs on the
+ * gather kernel instead), args = the tile description (tdp_tree_tile_args_bytes() bytes; layout in
+ * csrc/tdp_tree.hip TileArgs).  The launches keep at least 2^10 tiles (min_tiles_log = 10); smaller values give the
+ * plan of smaller joins.  Host only, for checking the plan; no counterpart in the reference. */
+int tdp_tree_tile_plan(const uint64_t *lm, const uint64_t *rm, unsigned parent_bits, unsigned generators,
+                       int itemsize, uint64_t infinity, int min_tiles_log, void *args, uint64_t *tiles);
+int tdp_tree_tile_args_bytes(void);
+
 /* Postorder arena compaction: move `bytes` toward a lower address, source and destination may overlap.
  * Replaces tree_move (cdist/tree_dp.c:46-51). */
 int tdp_tree_move(void *d_dst, const void *d_src, size_t bytes, void *stream);
@@ -288,7 +297,11 @@ int tdp_sp_hist(const void *d_ents, uint64_t n, uint64_t *d_hist, void *stream);
  * counts of each direction and changed counts of the frontier (N + 1 each, may be NULL), the witness's positions
  * (room for K + 1), and the result record; a state budget or deadline stops the query with that status.  K <= 254 for
  * one sweep, K <= 508 for the meet in the middle (a(c) <= 254 and K - a(c) <= 254).  The
- * reference's Sweep/query run this in Python around one C call per column (cdist/sparse_dp.py:621-1095). */
+ * reference's Sweep/query run this in Python around one C call per column (cdist/sparse_dp.py:621-1095).
+ * dev_alloc / dev_free (optional, both or neither): the caller's device allocator for every buffer of the query,
+ * called on the calling thread with alloc_ctx; dev_alloc returns NULL when it has no memory (the query then fails
+ * with TDP_ENOMEM).  A buffer is freed only after the work on `stream` that used it has completed.  Without them
+ * the library allocates stream-ordered from the device's default pool. */
 #define TDP_SQ_FORWARD 0
 #define TDP_SQ_MITM 1
 #define TDP_SQ_PROFILE_FW 2
@@ -309,6 +322,9 @@ typedef struct {
     const int64_t *bnd_w[2];
     const int64_t *bnd_extra[2];
     uint64_t *hist;
+    void *(*dev_alloc)(uint64_t bytes, void *ctx);
+    void (*dev_free)(void *ptr, void *ctx);
+    void *alloc_ctx;
 } tdp_sq_spec_t;
 typedef struct {
     int32_t status, fail_cut, value, join_cut, nwitness, pad;

@@ -129,3 +129,69 @@ def test_typed_join_entry_points_equal_oracle(itemsize):
         t.cuda.synchronize()
         got = do.cpu().numpy().view(dt)
         assert (got == want).all(), (itemsize, trial)
+
+
+@pytest.mark.parametrize("itemsize", [1, 2])
+def test_tiled_full_joins_equal_oracle(itemsize):
+    """Full joins that take the LDS-tiled kernel (parent ranks 12-16, 4-10 kernel generators, parent directions
+    that shift the kernel index on one or both sides) against the oracle's Gray-code join, with the infinity in the
+    tables; the plan must be the tiled one for most of them."""
+    import numpy as np
+    from test_tree_tiles import oracle_rank, plan, xor_of
+    from tropidist_amd import _lib
+    print("case tree tiled joins u%d" % (8 * itemsize), flush=True)
+    t = _lib.torch()
+    L = _lib.lib()
+    dt = np.uint8 if itemsize == 1 else np.uint16
+    rng = np.random.default_rng(70 + itemsize)
+    tiled = 0
+    for trial in range(16):
+        kr = int(rng.integers(4, 11))
+        pr = int(rng.integers(12, 17))
+        lr, rr = kr + int(rng.integers(0, 5)), kr + int(rng.integers(0, 5))
+        while True:
+            A = [int(x) for x in rng.integers(1, 1 << lr, size=kr)]
+            B = [int(x) for x in rng.integers(1, 1 << rr, size=kr)]
+            if oracle_rank(A) == kr and oracle_rank(B) == kr:
+                break
+        lm = [int(x) for x in rng.integers(0, 1 << lr, size=pr)] + A
+        rm = [int(x) for x in rng.integers(0, 1 << rr, size=pr)] + B
+        for i in range(pr):
+            r = rng.random()
+            if r < 0.4:
+                lm[i] = xor_of(A, int(rng.integers(0, 1 << kr)))
+            if r > 0.6 or r < 0.2:
+                rm[i] = xor_of(B, int(rng.integers(0, 1 << kr)))
+        J = pr + kr
+        inf = int(rng.choice([60, 200 if itemsize == 1 else 6001]))
+        tiled += plan(lm, rm, pr, J, itemsize, inf, min_tiles_log=10)[1] > 0
+        left = np.where(rng.random(1 << lr) < 0.1, inf, rng.integers(0, inf, size=1 << lr)).astype(dt)
+        right = np.where(rng.random(1 << rr) < 0.1, inf, rng.integers(0, inf, size=1 << rr)).astype(dt)
+        lmv, rmv = np.array(lm, dtype=np.uint64), np.array(rm, dtype=np.uint64)
+        want = np.full(1 << pr, inf, dtype=dt)
+        oracle.tree_join(itemsize, left, right, want, lmv, rmv, pr, J, 0, 1 << J, inf)
+        dl, dr = (t.from_numpy(x.view(np.uint8).copy()).cuda() for x in (left, right))
+        do = t.from_numpy(np.full(1 << pr, inf, dtype=dt).view(np.uint8).copy()).cuda()
+        _lib.check(L.tdp_tree_join(itemsize, dl.data_ptr(), dr.data_ptr(), do.data_ptr(), lmv.ctypes.data,
+                                   rmv.ctypes.data, pr, J, 0, 1 << J, inf, _lib.stream_handle()), "tdp_tree_join")
+        t.cuda.synchronize()
+        got = do.cpu().numpy().view(dt)
+        assert (got == want).all(), (itemsize, trial, int((got != want).sum()))
+    assert tiled >= 8, tiled
+
+
+def test_bench_tree_instance_with_tiled_joins():
+    """The tree instance of the benchmark (banded R=75 N=300 W=10, balanced tree): its widest joins take the tiled
+    kernel, and the distance equals the trellis DP's and the oracle's CPU run of the same program."""
+    print("case tree bench instance tiled", flush=True)
+    p = problem.banded(R=75, N=300, kmin=2, kmax=5, W=10, nobs=1, rho=0.1, seed=10)
+    aug = [h | ((l & 1) << p.nrows) for h, l in zip(p.h, p.l)]
+    program = CompiledTreeDP(aug, balanced(list(range(p.ncols))))
+    from test_tree_tiles import plan
+    ntiled = sum(plan(list(j.lmaps), list(j.rmaps), j.parent_rank, j.join_dimension, program.itemsize,
+                      program.infinity, min_tiles_log=10)[1] > 0 for j in program.joins.values() if j.lmaps)
+    assert ntiled >= 10, ntiled
+    lim = dict(max_join_work=1 << 40, max_memory_bytes=1 << 36, max_seconds=600.0)
+    got, _ = program.run(1 << p.nrows, **lim)
+    dense = kernel.run_plan(trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows))[0]
+    assert got == dense == oracle.tree_run(program, 1 << p.nrows)

@@ -50,6 +50,8 @@ SIGNATURES = {
     "tdp_witness_workspace_bytes": (ci, [i64, ci, ctypes.POINTER(u64)]),
     "tdp_run_plan_witness": (ci, [vp, i64, ci, vp, u64, ctypes.POINTER(ci), vp, ctypes.POINTER(i64), vp]),
     "tdp_tree_join": (ci, [ci, vp, vp, vp, vp, vp, ctypes.c_uint, ctypes.c_uint, u64, u64, u64, vp]),
+    "tdp_tree_tile_plan": (ci, [vp, vp, ctypes.c_uint, ctypes.c_uint, ci, u64, ci, vp, vp]),
+    "tdp_tree_tile_args_bytes": (ci, []),
     "tdp_tree_join_u8": (ci, [vp, vp, vp, vp, vp, ctypes.c_uint, ctypes.c_uint, u64, u64, u64, vp]),
     "tdp_tree_join_u16": (ci, [vp, vp, vp, vp, vp, ctypes.c_uint, ctypes.c_uint, u64, u64, u64, vp]),
     "tdp_tree_join_u32": (ci, [vp, vp, vp, vp, vp, ctypes.c_uint, ctypes.c_uint, u64, u64, u64, vp]),
@@ -172,3 +174,42 @@ def stream_handle(stream=None):
 
 def dptr(tensor):
     return ctypes.c_void_p(tensor.data_ptr())
+
+
+ALLOC_FN = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p)
+FREE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p)
+
+
+class TorchDeviceAllocator:
+    """Device buffers for a native call, taken from torch's caching allocator on the stream the call runs on, as a
+    pair of C callbacks (alloc(bytes, ctx) -> pointer or NULL, free(pointer, ctx)).  Memory a call gives back stays in
+    torch's cache for the next one instead of going back to the driver.  Use as a context manager around the call:
+    on exit every buffer the call did not free is released.  The callbacks run on the calling thread."""
+
+    def __init__(self, stream=None):
+        t = torch()
+        self._stream = stream if stream is not None else t.cuda.current_stream()
+        self._live = {}
+        self.alloc_fn = ALLOC_FN(self._alloc)
+        self.free_fn = FREE_FN(self._free)
+
+    def _alloc(self, nbytes, ctx):
+        t = torch()
+        try:
+            with t.cuda.stream(self._stream):
+                buf = t.empty(max(1, int(nbytes)), dtype=t.uint8, device="cuda")
+        except Exception:                       # out of memory: the native call reports it
+            return None
+        p = buf.data_ptr()
+        self._live[p] = buf
+        return p
+
+    def _free(self, p, ctx):
+        self._live.pop(p, None)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self._live.clear()
+        return False

@@ -275,10 +275,11 @@ def _certify_wide(problems, orders, by_problem, pis, budget, stream=None):
 
 
 def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=None, stream=None,
-                    wide_bits=WIDE_BITS, wide_concurrent=True, wide_budget=WIDE_BUDGET):
+                    wide_bits=WIDE_BITS, wide_concurrent=True, wide_budget=WIDE_BUDGET, wide_stream=None):
     """The device part of certify_batch on plans made by plan_problems.  Problems with a plan of 2^wide_bits
     entries or more run on the multi-program fused executor (on a stream of their own, alongside the persistent
-    launches, when wide_concurrent); wide_bits = None keeps every problem in the persistent launches."""
+    launches, when wide_concurrent: wide_stream, or a stream from torch's pool); wide_bits = None keeps every problem
+    in the persistent launches."""
     by_problem = {}
     for plan, (pi, i) in zip(flat, owner):
         by_problem.setdefault(pi, []).append((i, plan))
@@ -293,7 +294,7 @@ def certify_planned(problems, orders, flat, owner, workgroups=None, intervals=No
 
         def worker():
             try:
-                st = t.cuda.Stream()
+                st = wide_stream if wide_stream is not None else t.cuda.Stream()
                 with t.cuda.stream(st):
                     result.update(_certify_wide(problems, orders, by_problem, sorted(wide), wide_budget, st))
             except BaseException as e:          # re-raised by the caller

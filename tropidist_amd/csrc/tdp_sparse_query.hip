@@ -207,11 +207,36 @@ struct SqRun {
 
     explicit SqRun(const tdp_sq_spec_t &s, hipStream_t stream) : S(s), st(stream), t0(Clock::now()) {}
 
+    // Device memory of the query: from the caller's allocator when it gave one (a buffer goes back to it only once
+    // the stream is idle), else stream-ordered from the device's default pool.
+    int dalloc(void **p, uint64_t bytes) {
+        *p = nullptr;
+        if (S.dev_alloc) {
+            *p = S.dev_alloc(bytes, S.alloc_ctx);
+            if (!*p) {
+                set_error("sparse query: the caller's allocator has no %llu bytes", (unsigned long long)bytes);
+                return TDP_ENOMEM;
+            }
+            return TDP_OK;
+        }
+        TDP_HIP_CHECK(hipMallocAsync(p, bytes, st));
+        return TDP_OK;
+    }
+    void dfree(void *p) {
+        if (!p) return;
+        if (S.dev_free) {
+            (void)hipStreamSynchronize(st);
+            S.dev_free(p, S.alloc_ctx);
+        } else {
+            (void)hipFreeAsync(p, st);
+        }
+    }
     int alloc(Tab &t, uint64_t nents, uint64_t cap) {
         t.cap = cap;
         const uint64_t eb = up256(16 * std::max<uint64_t>(nents, 1));
         t.bytes = eb + 4 * cap;
-        TDP_HIP_CHECK(hipMallocAsync(&t.mem, t.bytes, st));
+        int rc = dalloc(&t.mem, t.bytes);
+        if (rc) return rc;
         t.ents = static_cast<Ent *>(t.mem);
         t.slots = reinterpret_cast<uint32_t *>(static_cast<uint8_t *>(t.mem) + eb);
         held += t.bytes;
@@ -220,7 +245,7 @@ struct SqRun {
     }
     void drop(Tab &t) {
         if (t.mem) {
-            (void)hipFreeAsync(t.mem, st);
+            dfree(t.mem);
             held -= t.bytes;
         }
         t = Tab();
@@ -421,7 +446,10 @@ static int walk_back(SqRun &R, int dir, const int32_t *caps, bool use_h, Kept &k
     int c = cut;
     void *dev = nullptr;
     const uint64_t dev_bytes = up256(sizeof(SqRef) * (N + 1)) + 64 + up256(4ull * (S.K + 2) + 8);
-    TDP_HIP_CHECK(hipMallocAsync(&dev, dev_bytes, R.st));
+    {
+        int rc0 = R.dalloc(&dev, dev_bytes);
+        if (rc0) return rc0;
+    }
     SqRef *d_refs = static_cast<SqRef *>(dev);
     unsigned long long *d_io = reinterpret_cast<unsigned long long *>(static_cast<uint8_t *>(dev) +
                                                                       up256(sizeof(SqRef) * (N + 1)));
@@ -476,7 +504,7 @@ static int walk_back(SqRun &R, int dir, const int32_t *caps, bool use_h, Kept &k
             ok = io[0] == 0 && io[1] == 0 && io[2] == 0;
         }
     }
-    (void)hipFreeAsync(dev, R.st);
+    R.dfree(dev);
     return rc;
 }
 
@@ -520,8 +548,16 @@ int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_coun
     res->join_cut = -1;
     hipStream_t st = as_stream(stream);
     SqRun R(S, st);
-    TDP_HIP_CHECK(hipMallocAsync((void **)&R.d_cnt, 8 * 272, st));
-    TDP_HIP_CHECK(hipHostMalloc((void **)&R.h_cnt, 8 * 272, hipHostMallocDefault));
+    TDP_REQUIRE(!S.dev_alloc == !S.dev_free, "tdp_sq_run: dev_alloc and dev_free come together");
+    {
+        int rc0 = R.dalloc((void **)&R.d_cnt, 8 * 272);
+        if (rc0) return rc0;
+    }
+    if (hipHostMalloc((void **)&R.h_cnt, 8 * 272, hipHostMallocDefault) != hipSuccess) {
+        R.dfree(R.d_cnt);
+        set_error("tdp_sq_run: pinned host counters");
+        return TDP_EHIP;
+    }
     // device copies of what kernels read: the column words (traceback) and the lookahead classes (steps)
     void *dconst = nullptr;
     uint64_t nclass[2] = {0, 0};
@@ -532,7 +568,12 @@ int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_coun
     const uint64_t prm_bytes = up256(80ull * std::max(N, 1));
     const uint64_t mb[2] = {up256(16 * std::max<uint64_t>(nclass[0], 1)), up256(16 * std::max<uint64_t>(nclass[1], 1))};
     const uint64_t wb[2] = {up256(8 * std::max<uint64_t>(nclass[0], 1)), up256(8 * std::max<uint64_t>(nclass[1], 1))};
-    TDP_HIP_CHECK(hipMallocAsync(&dconst, prm_bytes + mb[0] + mb[1] + wb[0] + wb[1], st));
+    if (R.dalloc(&dconst, prm_bytes + mb[0] + mb[1] + wb[0] + wb[1])) {
+        (void)hipStreamSynchronize(st);
+        R.dfree(R.d_cnt);
+        (void)hipHostFree(R.h_cnt);
+        return TDP_ENOMEM;
+    }
     {
         uint8_t *q = static_cast<uint8_t *>(dconst);
         R.d_prm = reinterpret_cast<uint64_t *>(q);
@@ -554,13 +595,16 @@ int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_coun
     std::vector<Tab> bcache, fcache, frontier_tabs;
     std::vector<int32_t> used;
     int rc = TDP_OK;
+    void *chmem = nullptr;                     // the forward frontier's changed-state buffer
     auto finish = [&](int code) {
+        R.dfree(chmem);
+        chmem = nullptr;
         for (Tab &t : bw.at) R.drop(t);
         for (Tab &t : fw.at) R.drop(t);
         for (Tab &t : bcache) R.drop(t);
         for (Tab &t : fcache) R.drop(t);
-        (void)hipFreeAsync(R.d_cnt, st);
-        (void)hipFreeAsync(dconst, st);
+        R.dfree(R.d_cnt);
+        R.dfree(dconst);
         (void)hipStreamSynchronize(st);
         (void)hipHostFree(R.h_cnt);
         res->status = R.status;
@@ -635,7 +679,6 @@ int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_coun
     uint64_t best_lo = 0, best_hi = 0, joined = 0;
     Ent *ch = nullptr;
     uint64_t chcap = 0;
-    void *chmem = nullptr;
     auto join = [&](const Ent *f, uint64_t nf, int c) -> int {
         if (!nf) return TDP_OK;
         Tab b;
@@ -665,9 +708,10 @@ int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_coun
         const int c = p + 1;
         const uint64_t need = 2 * std::max<uint64_t>(cur.n, 1);
         if (chcap < need) {
-            if (chmem) (void)hipFreeAsync(chmem, st);
+            R.dfree(chmem);
+            chmem = nullptr;
             chcap = std::max<uint64_t>(need, 2 * chcap);
-            rc = hipMallocAsync(&chmem, 16 * chcap, st);
+            rc = R.dalloc(&chmem, 16 * chcap);
             if (rc) {
                 set_error("sparse query: changed-state buffer of %llu entries", (unsigned long long)chcap);
                 return finish(TDP_EHIP);
@@ -719,7 +763,8 @@ int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_coun
         if (R.check_budget(nxt.n, nch, c)) break;
     }
     if (!cur_kept) R.drop(cur);
-    if (chmem) (void)hipFreeAsync(chmem, st);
+    R.dfree(chmem);
+    chmem = nullptr;
     res->joined = joined;
     res->forward_seconds = R.elapsed() - res->backward_seconds;
     if (R.status != TDP_SQ_DONE) return finish(TDP_OK);

@@ -96,9 +96,14 @@ class Server:
         self.done = []
         _lib.lib()
         t = _lib.torch()
+        # Each dense lane keeps two streams for its life: its persistent launches, and the wide problems' rounds
+        # that run beside them.  torch's caching allocator keeps freed blocks per stream, so fixed streams let a
+        # lane's tables, workspaces and witness arenas reuse its own cached blocks instead of mapping new device
+        # memory for every batch (and, once the device is full of blocks cached for streams no longer used,
+        # unmapping them again).
         self.threads = []
         for k in range(dense_lanes):
-            th = threading.Thread(target=self._dense_lane, args=(t.cuda.Stream(),), daemon=True)
+            th = threading.Thread(target=self._dense_lane, args=(t.cuda.Stream(), t.cuda.Stream()), daemon=True)
             th.start()
             self.threads.append(th)
         sst = t.cuda.Stream()
@@ -145,6 +150,10 @@ class Server:
             self.cv.notify_all()
         self.single_thread.join()
         self.sparse_engine.close()
+        # the blocks cached for this server's streams are of no use to later streams: give them back
+        t = _lib.torch()
+        t.cuda.synchronize()
+        t.cuda.empty_cache()
 
     # ------------------------------------------------------------------ lanes
     def _finish(self, tickets, results=None, error=None):
@@ -171,7 +180,7 @@ class Server:
             tk.footprint = _dense_footprint([p for p in flat if p.feasible])
         return tk
 
-    def _dense_lane(self, stream):
+    def _dense_lane(self, stream, wide_stream):
         t = _lib.torch()
         while True:
             with self.cv:
@@ -217,7 +226,8 @@ class Server:
                     owner += [(k, i) for (_, i) in o]
                 try:
                     with t.cuda.stream(stream):
-                        recs = batch.certify_planned(probs, orders, flat, owner, stream=stream)
+                        recs = batch.certify_planned(probs, orders, flat, owner, stream=stream,
+                                                     wide_stream=wide_stream)
                     self._finish(take, recs)
                 except Exception as exc:          # delivered to every waiter of the failed batch
                     self._finish(take, error=exc)

@@ -492,7 +492,8 @@ class _Spec(ctypes.Structure):
         ("max_states", ctypes.c_uint64), ("max_seconds", ctypes.c_double), ("keep_bytes", ctypes.c_uint64),
         ("prm", ctypes.c_void_p), ("tm", ctypes.c_void_p), ("split", ctypes.c_void_p)] + [
         (k, ctypes.c_void_p * 2) for k in ("bnd_ncls", "bnd_off", "bnd_mask", "bnd_w", "bnd_extra")] + [
-        ("hist", ctypes.c_void_p)]
+        ("hist", ctypes.c_void_p), ("dev_alloc", _lib.ALLOC_FN), ("dev_free", _lib.FREE_FN),
+        ("alloc_ctx", ctypes.c_void_p)]
 
 
 class _Result(ctypes.Structure):
@@ -542,8 +543,13 @@ def _native(tr, K, mode, split=None, use_h=False, witness=False, lower=0, every=
     ch = np.zeros(N + 1, dtype=np.uint64)
     wit = np.zeros(K + 2, dtype=np.int32)
     res = _Result()
-    _lib.check(_lib.lib().tdp_sq_run(ctypes.byref(spec), fw.ctypes.data, bw.ctypes.data, ch.ctypes.data,
-                                     wit.ctypes.data, ctypes.byref(res), _lib.stream_handle(stream)), "tdp_sq_run")
+    # the query's tables come from torch's caching allocator: returning tens of GB to the driver after every query
+    # and mapping them again in the next cost seconds
+    with _lib.TorchDeviceAllocator(stream) as arena:
+        spec.dev_alloc, spec.dev_free = arena.alloc_fn, arena.free_fn
+        rc = _lib.lib().tdp_sq_run(ctypes.byref(spec), fw.ctypes.data, bw.ctypes.data, ch.ctypes.data,
+                                   wit.ctypes.data, ctypes.byref(res), _lib.stream_handle(stream))
+    _lib.check(rc, "tdp_sq_run")
     if res.status == SQ_STATES:
         raise BudgetExceeded("%d states at cut %d" % (res.peak_states, res.fail_cut), "states")
     if res.status == SQ_DEADLINE:
