@@ -53,8 +53,13 @@ typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 // table pointers from memory, where the compiler could not tell them from generic pointers).
 typedef __attribute__((address_space(1))) uint32_t gu32;
 typedef __attribute__((address_space(1))) uint8_t gu8;
+// NT: a nontemporal load (the input of a pass is read once; on rounds of large tables this leaves more of the
+// memory-side cache to the outputs the next round reads).
+template <bool NT = false>
 __device__ __forceinline__ uint32_t gload32(const uint8_t *base, uint64_t off) {
-    return *reinterpret_cast<const gu32 *>(reinterpret_cast<uintptr_t>(base) + off);
+    const gu32 *p = reinterpret_cast<const gu32 *>(reinterpret_cast<uintptr_t>(base) + off);
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
 }
 __device__ __forceinline__ void gstore32(uint8_t *base, uint64_t off, uint32_t v) {
     *reinterpret_cast<gu32 *>(reinterpret_cast<uintptr_t>(base) + off) = v;
@@ -141,38 +146,47 @@ __device__ __forceinline__ IDX map_bits(const PassArgs &a, int m, uint64_t bits,
 // FREE_NEW: some new row is a free variable, so whether an element starts at 255 varies across lanes; otherwise it
 // depends on the element only and the test is uniform.
 // FREE_NEW_ < 0: read from the descriptor at run time (the multi-program kernel, whose jobs differ).
-template <int D, bool BYTE, int FREE_NEW_, typename IDX>
-__device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
-                                          uint64_t block_bits, uint32_t lane, const PassArgs &a) {
+template <int D, int FREE_NEW_, typename IDX, bool NT = false>
+__device__ __forceinline__ void cube_load(const uint8_t *__restrict__ in, uint64_t block_bits, uint32_t lane,
+                                          const PassArgs &a, uint32_t (&d)[1 << D]) {
     const bool FREE_NEW = FREE_NEW_ < 0 ? a.nruns[MAP_NEW] > 0 : FREE_NEW_ != 0;
     const IDX bin = map_bits<IDX>(a, MAP_IN, block_bits, false) | map_bits<IDX>(a, MAP_IN, lane, true);
-    const IDX bout = map_bits<IDX>(a, MAP_OUT, block_bits, false) | map_bits<IDX>(a, MAP_OUT, lane, true);
-    const uint32_t xs = map_bits<uint32_t>(a, MAP_SEL, block_bits, false) | map_bits<uint32_t>(a, MAP_SEL, lane, true);
     uint32_t xn = 0;
     if (FREE_NEW) xn = map_bits<uint32_t>(a, MAP_NEW, block_bits, false) | map_bits<uint32_t>(a, MAP_NEW, lane, true);
     constexpr int Q = 1 << D;
-    u16x2 lo[Q], hi[Q];
     // uniform per-element deltas, each the previous one (q with its lowest bit cleared) XOR one direction's:
-    // one scalar XOR per element and quantity.  The twist selectors are affine in sigma:
-    // sel_lo(s) = sel_lo(0) ^ s * 0x10001, likewise sel_hi.
+    // one scalar XOR per element and quantity
     IDX cin[Q];
-    uint32_t cnew[Q], csig[Q];
+    uint32_t cnew[Q];
     cin[0] = 0;
     cnew[0] = 0;
-    csig[0] = 0;
 #pragma unroll
     for (int q = 1; q < Q; q++) {
         const int j = __builtin_ctz(q), p = q & (q - 1);
         cin[q] = cin[p] ^ (IDX)a.din[2 + j];
         cnew[q] = cnew[p] ^ a.dnew[2 + j];
-        csig[q] = csig[p] ^ (a.lamw[j] & 3u) * 0x10001u;
     }
 #pragma unroll
+    for (int q = 0; q < Q; q++) d[q] = !(xn ^ cnew[q]) ? gload32<NT>(in, bin ^ cin[q]) : 0xFFFFFFFFu;
+}
+
+// The butterflies and stores of a loaded cube.  The twist selectors are affine in sigma:
+// sel_lo(s) = sel_lo(0) ^ s * 0x10001, likewise sel_hi.
+template <int D, bool BYTE, typename IDX>
+__device__ __forceinline__ void cube_finish(uint8_t *__restrict__ out, uint64_t block_bits, uint32_t lane,
+                                            const PassArgs &a, const uint32_t (&d)[1 << D]) {
+    const IDX bout = map_bits<IDX>(a, MAP_OUT, block_bits, false) | map_bits<IDX>(a, MAP_OUT, lane, true);
+    const uint32_t xs = map_bits<uint32_t>(a, MAP_SEL, block_bits, false) | map_bits<uint32_t>(a, MAP_SEL, lane, true);
+    constexpr int Q = 1 << D;
+    u16x2 lo[Q], hi[Q];
+    uint32_t csig[Q];
+    csig[0] = 0;
+#pragma unroll
+    for (int q = 1; q < Q; q++) csig[q] = csig[q & (q - 1)] ^ (a.lamw[__builtin_ctz(q)] & 3u) * 0x10001u;
+#pragma unroll
     for (int q = 0; q < Q; q++) {
-        uint32_t d = 0xFFFFFFFFu;
-        if (!(xn ^ cnew[q])) d = gload32(in, bin ^ cin[q]);
-        lo[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_lo(0) ^ csig[q]));
-        hi[q] = as_u16x2(__builtin_amdgcn_perm(d, d, sel_hi(0) ^ csig[q]));
+        lo[q] = as_u16x2(__builtin_amdgcn_perm(d[q], d[q], sel_lo(0) ^ csig[q]));
+        hi[q] = as_u16x2(__builtin_amdgcn_perm(d[q], d[q], sel_hi(0) ^ csig[q]));
     }
     DirLoop<D, 0>::run(lo, hi, a);
 #pragma unroll
@@ -215,6 +229,18 @@ __device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_
             }
         }
     }
+}
+
+// IDX: uint32_t when both tables hold at most 2^32 entries (byte offsets from a scalar base), else uint64_t.
+// FREE_NEW: some new row is a free variable, so whether an element starts at 255 varies across lanes; otherwise it
+// depends on the element only and the test is uniform.
+// FREE_NEW_ < 0: read from the descriptor at run time (the multi-program kernel, whose jobs differ).
+template <int D, bool BYTE, int FREE_NEW_, typename IDX, bool NT = false>
+__device__ __forceinline__ void pass_cube(const uint8_t *__restrict__ in, uint8_t *__restrict__ out,
+                                          uint64_t block_bits, uint32_t lane, const PassArgs &a) {
+    uint32_t d[1 << D];
+    cube_load<D, FREE_NEW_, IDX, NT>(in, block_bits, lane, a, d);
+    cube_finish<D, BYTE, IDX>(out, block_bits, lane, a, d);
 }
 
 // Six waves per SIMD: the cube of a D = 5 pass then fits 80 VGPRs without spilling (the compiler's own choice, 88,
@@ -499,6 +525,8 @@ __device__ void tiny_ops(const OpJob &J) {
 
 // The same bound spills 16 bytes per lane here; the headline step of four programs is still 2% faster with it than
 // at five waves.
+// NT: nontemporal input loads (rounds larger than the memory-side cache, kNtBytes).
+template <bool NT>
 __global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass_multi(const PassArgs *__restrict__ descs,
                                                            const Job *__restrict__ jobs, uint32_t njobs) {
     __shared__ uint32_t s_count;
@@ -521,18 +549,18 @@ __global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass_multi(const Pas
     const uint64_t bb = local * kPassBlock;
     // one compiled shape per (D, store mode); the branch is uniform over the block
     switch (a.D * 2 + (a.byte_store ? 1 : 0)) {
-        case 10: pass_cube<5, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 11: pass_cube<5, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 8: pass_cube<4, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 9: pass_cube<4, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 6: pass_cube<3, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 7: pass_cube<3, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 4: pass_cube<2, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 5: pass_cube<2, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 2: pass_cube<1, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 3: pass_cube<1, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 0: pass_cube<0, false, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
-        default: pass_cube<0, true, -1, uint32_t>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 10: pass_cube<5, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 11: pass_cube<5, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 8: pass_cube<4, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 9: pass_cube<4, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 6: pass_cube<3, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 7: pass_cube<3, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 4: pass_cube<2, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 5: pass_cube<2, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 2: pass_cube<1, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 3: pass_cube<1, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        case 0: pass_cube<0, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
+        default: pass_cube<0, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
     }
 }
 
@@ -788,7 +816,12 @@ struct Action {
     uint8_t *cur, *other; // kind 0: tables at that point
     uint64_t n;           // kind 0: entries at that point
     uint64_t nblocks;     // kind 2: blocks of the round
+    bool nt;              // kind 2: nontemporal input loads (the round moves at least kNtBytes)
 };
+// Rounds whose inputs and outputs together exceed the 256 MB memory-side cache by half or more read their inputs
+// nontemporally: on the headline step that raised the rounds of 4 x 2^25 -> 2^26 and 4 x 2^26 entries from 0.52
+// to 0.54 of the HBM peak, while rounds that fit the cache (up to 4 x 2^25 -> 2^25) ran slower with it.
+constexpr uint64_t kNtBytes = 3ull << 27;
 
 struct Schedule {
     std::vector<uint32_t> desc_base;      // per run: index of its first pass descriptor in the upload
@@ -882,6 +915,7 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
     while (true) {
         const size_t j0 = S.jobs.size();
         uint64_t nblocks = 0;
+        uint64_t round_bytes = 0;
         for (int r = 0; r < nruns; r++) {
             const tdp_run_t &R = runs[r];
             // per-op work on a tiny table is this round's job of the program (its next pass waits a round)
@@ -968,6 +1002,7 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
             }
             J.nthreads = 1ull << p.nfree;
             J.desc = desc_base[r] + (uint32_t)e.a;
+            round_bytes += (1ull << p.b_in) + (1ull << p.b_out);
             const uint64_t nb = (J.nthreads + kPassBlock - 1) / kPassBlock;
             TDP_REQUIRE(nblocks + nb < (1ull << 31), "round of more than 2^31 blocks");
             J.first_block = (uint32_t)nblocks;
@@ -980,6 +1015,7 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
         act.a = (int64_t)j0;
         act.b = (int64_t)(S.jobs.size() - j0);
         act.nblocks = nblocks;
+        act.nt = round_bytes >= kNtBytes;
         S.actions.push_back(act);
     }
     auto up = [](uint64_t x) { return (x + 255) / 256 * 256; };
@@ -1222,7 +1258,10 @@ static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t wo
                 evs.push_back(e1);
                 TDP_HIP_CHECK(hipEventRecord(e0, st));
             }
-            k_pass_multi<<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, (uint32_t)act.b);
+if (act.nt)
+                k_pass_multi<true><<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, (uint32_t)act.b);
+            else
+                k_pass_multi<false><<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, (uint32_t)act.b);
             TDP_LAUNCH_CHECK();
             if (timed) TDP_HIP_CHECK(hipEventRecord(evs.back(), st));
         }
