@@ -19,14 +19,17 @@ Beside it (each can be skipped):
             against the reference's tree_dp.c joins on the host;
   tables    the meet-in-the-middle lookup table of every column subset of size <= 3 of the l = m = 12 code: build
             time per entry and batched lookups per second, against the reference's gmitm.c build on the host;
-  serving   the serving mix (70% dense width 10-20, 20% width 21-25, 10% sparse MITM on quasi-cyclic codes):
-            capacity with every request submitted at once, then a Poisson stream at a quarter of it, p50 / p99 latency,
-            against a stratified plain-GPU sample;
+  serving   the serving mix of 10,000 requests (70% dense width 10-20, 20% width 21-25, 10% sparse MITM with
+            K <= 10 on quasi-cyclic codes): capacity with every request submitted at once, then the first 2,000 as a
+            Poisson stream at half of it, p50 / p99 latency, against a stratified plain-GPU sample;
+  cpu_configs  the reference's kernels on the host: the plain dense DP on one thread (R=48 N=300 W=10), and
+            certificates per second of the batched scan and the serving mix on all host cores;
   cpu_baseline  the reference's own viterbi.c (compiled from its sources, OpenMP on this host) on observable 0.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--no-cpu] [--skip-batched] [--skip-sparse]
-                    [--skip-tree] [--skip-tables] [--skip-serving] [--serving-requests 600] [--serving-max-K 8]
-                    [--serving-max-states 30000000] [--batch-instances 512] [--sparse-K 12]
+                    [--skip-tree] [--skip-tables] [--skip-serving] [--serving-requests 10000] [--serving-max-K 10]
+                    [--serving-max-states 4000000] [--serving-poisson-requests 2000] [--batch-instances 512]
+                    [--sparse-K 12]
 """
 import argparse
 import json
@@ -607,6 +610,152 @@ def bench_tree_large(args):
     return out
 
 
+def _release_cache():
+    """Hand torch's cached device blocks back between sections: each section starts from the same free memory
+    (the sparse queries alone leave tens of GB cached), and the library's own driver allocations see all of it."""
+    import torch
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
+def host_cpu(threads=None):
+    """The host the CPU figures ran on: logical CPUs (os.cpu_count, as nproc without affinity), the CPUs this process
+    may use, the model line of /proc/cpuinfo, and the OpenMP threads of the reference's kernels."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)), "model": model,
+            "omp_threads": threads}
+
+
+def _omp_threads(n):
+    """Set the OpenMP thread count of the reference's kernels (one libgomp in the process); returns the old one."""
+    import ctypes
+    gomp = ctypes.CDLL("libgomp.so.1")
+    old = gomp.omp_get_max_threads()
+    gomp.omp_set_num_threads(int(n))
+    return old
+
+
+def _ref_values(p):
+    """Per-observable values of p by the reference's viterbi.c, driven op by op (the plans of this library's
+    planner, whose peak and work equal the reference planner's on the recorded fixtures)."""
+    import oracle
+    from tropidist_amd import auto, trellis
+    order = auto.planning_order(p)
+    plans = trellis.build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], order, p.nrows, p.w)
+    return [oracle.run_plan_c(pl, "ref") if pl.feasible else None for pl in plans]
+
+
+def bench_cpu_configs(args, scan, requests):
+    """The CPU figures of the reference's own kernels beside the GPU ones, for the configurations whose GPU figures
+    are reported elsewhere: the plain dense Rank DP on one thread (R=48 N=300 W=10), and certificates per second of
+    the batched scan and of the serving mix on all host cores (dense requests by viterbi.c with OpenMP, sparse ones
+    by sparse_dp.c under the same sweep logic).  Every CPU sample is bounded to a few seconds per class; values only
+    (the reference's traceback is a numpy replay several times slower than its DP, so these rates favour the CPU)."""
+    import oracle
+    from tropidist_amd import kernel, ordering, problem, sparse_dp, trellis
+    if not oracle.have_ref():
+        return {"skipped": "oracle/_ref not built"}
+    threads = oracle.rlib("viterbi").dp_num_threads()
+    out = {"host": host_cpu(threads)}
+    # plain dense Rank DP, the reference's CPU path on one thread
+    p = problem.banded(R=48, N=300, kmin=1, kmax=4, W=10, nobs=1, rho=0.1, seed=1000)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+    gv = kernel.run_plan(plan)[0]
+    n, t0 = 0, time.perf_counter()
+    while n < 20 or time.perf_counter() - t0 < 0.5:
+        kernel.run_plan(plan)
+        n += 1
+    gpu_s = (time.perf_counter() - t0) / n
+    old = _omp_threads(1)
+    try:
+        m, t0 = 0, time.perf_counter()
+        while m < 5 or time.perf_counter() - t0 < 2.0:
+            cv = oracle.run_plan_c(plan, "ref")
+            m += 1
+        cpu_s = (time.perf_counter() - t0) / m
+    finally:
+        _omp_threads(old)
+    out["plain_dense_one_thread"] = {
+        "workload": "banded R=48 N=300 k=1-4 W=10, 1 logical row rho=0.1, unit costs, uint8, seed 1000",
+        "peak_bits": plan.peak_bits, "work": plan.work(), "distance": gv, "identical": gv == cv,
+        "gpu_states_per_s": plan.work() / gpu_s, "gpu_seconds": gpu_s,
+        "cpu_baseline": {"value": plan.work() / cpu_s, "unit": "states/s", "cores": 1, "kind": "reference",
+                         "sample": "%d runs of the whole plan through viterbi.c on one OpenMP thread" % m,
+                         "seconds": cpu_s}}
+
+    def dense_sample(problems, limit_s):
+        done, spent = 0, 0.0
+        for q in problems:
+            t1 = time.perf_counter()
+            _ref_values(q)
+            spent += time.perf_counter() - t1
+            done += 1
+            if spent > limit_s:
+                break
+        return done, spent
+
+    # the batched scan: instances spread evenly through the 512
+    if scan:
+        pick = [scan[(j * len(scan)) // 24] for j in range(24)]
+        done, spent = dense_sample(pick, 8.0)
+        out["batched_scan"] = {"cpu_baseline": {
+            "value": done / spent, "unit": "certs/s (values of every observable)", "cores": threads,
+            "kind": "reference", "sample": "%d of the 512 scan instances, evenly spaced, one at a time through "
+                                           "viterbi.c (OpenMP)" % done, "seconds": spent}}
+    # the serving mix: each class sampled evenly, weighted by its share
+    if requests:
+        classes = {}
+        for kind, q, K in requests:
+            c = "sparse" if kind == "sparse" else ("dense_wide" if getattr(q, "band_width", 0) >= 21 else "dense_narrow")
+            classes.setdefault(c, []).append((q, K))
+        per, notes = {}, {}
+        for c, items in sorted(classes.items()):
+            pick = [items[(j * len(items)) // min(12, len(items))] for j in range(min(12, len(items)))]
+            if c != "sparse":
+                done, spent = dense_sample([q for q, _ in pick], 6.0)
+                per[c] = spent / done
+                notes[c] = "%d requests" % done
+                continue
+            done = spent = skipped = 0
+            for q, K in pick:
+                order = ordering.greedy_order_native(q.h, q.nrows, q.words())
+                trs = [sparse_dp.Trellis.for_query(q, i, order) for i in range(q.nobs)]
+                if any(t.feasible and t.nslots > 56 for t in trs):
+                    skipped += 1                # beyond the reference tables' 56 row slots
+                    continue
+                t1 = time.perf_counter()
+                best = None
+                for tr in trs:
+                    if not tr.feasible:
+                        continue
+                    k = K if best is None else best - 1
+                    if k < 1:
+                        break
+                    r = sparse_dp.query(tr, k, mode="mitm", use_h=False, witness=False,
+                                        backend=oracle.RefTables(tr))
+                    if r["value"] is not None:
+                        best = r["value"]
+                spent += time.perf_counter() - t1
+                done += 1
+                if spent > 8.0:
+                    break
+            per[c] = spent / max(1, done)
+            notes[c] = "%d requests through sparse_dp.c (one thread); %d of the sample need more than the " \
+                       "reference's 56 row slots and cannot run there" % (done, skipped)
+        total = sum(len(v) for v in classes.values())
+        rate = 1.0 / sum(len(classes[c]) / total * per[c] for c in per)
+        out["serving_mix"] = {"cpu_baseline": {
+            "value": rate, "unit": "certs/s (values)", "cores": threads, "kind": "reference",
+            "sample": "; ".join("%s: %s, %.1f ms each" % (c, notes[c], 1e3 * per[c]) for c in sorted(per)),
+            "class_share": {c: round(len(v) / total, 3) for c, v in sorted(classes.items())}}}
+    return out
+
+
 def bench_sparse(args):
     """cfg3: the QC code l = m = 12 at threshold K (its 77 row slots exceed the reference's 56, so the reference
     cannot run it); beside it, QC l = m = 9 (50 slots) at K = 12 on the GPU and on the reference's sparse_dp.c."""
@@ -684,7 +833,10 @@ def bench_serving(args, requests):
     print("bench: serving capacity %.0f/s over %d requests (%.1f s)" % (capacity, len(reqs), time.time() - T_START),
           file=sys.stderr, flush=True)
     rate = args.serving_rate if args.serving_rate else args.serving_load * capacity
-    tickets, wall = _serve_run(serve, reqs, rate, 7, args.serving_max_states)
+    # the Poisson stream: the first --serving-poisson-requests of the same mix (all of them when 0), so that the
+    # stream at a fraction of the capacity stays within the run's time
+    npois = args.serving_poisson_requests or len(reqs)
+    tickets, wall = _serve_run(serve, reqs[:npois], rate, 7, args.serving_max_states)
     print("bench: serving Poisson stream at %.0f/s done (%.1f s)" % (rate, time.time() - T_START), file=sys.stderr,
           flush=True)
     lat = [tk.latency for tk in tickets]
@@ -692,6 +844,8 @@ def bench_serving(args, requests):
     ok = all(tk.result["witness_ok"] for tk in tickets if tk.error is None and tk.result["distance"] is not None)
     same = all(a.error is None and b.error is None and a.result["distance"] == b.result["distance"]
                for a, b in zip(cap_tickets, tickets) if a.error is None and b.error is None)
+    cap_lat = [tk.latency for tk in cap_tickets]
+    cap_errs = [tk for tk in cap_tickets if tk.error is not None]
     dl = [tk.latency for tk in tickets if tk.kind == "dense"]
     sl = [tk.latency for tk in tickets if tk.kind == "sparse"]
     # plain GPU: a sample of the same requests answered one at a time, synchronously, by the single-problem paths
@@ -722,19 +876,21 @@ def bench_serving(args, requests):
     ns = sum(len(v) for v in per_class.values())
     plain = 1.0 / sum(len(members[c]) / len(reqs) * (sum(v) / len(v)) for c, v in per_class.items())
     return {"workload": "%d requests: 70%% dense width 10-20, 20%% width 21-25, 10%% sparse MITM K 4-%d on QC codes "
-                        "l,m 6-15 (tables of at most %d states); capacity with all submitted at once, then Poisson "
-                        "arrivals at %.0f/s (%.0f%% of capacity)" % (len(reqs), args.serving_max_K,
-                                                                     args.serving_max_states, rate,
-                                                                     100 * rate / capacity),
+                        "l,m 6-15 (tables of at most %d states); capacity with all submitted at once, then the first "
+                        "%d of them as Poisson arrivals at %.0f/s (%.0f%% of capacity)" % (
+                            len(reqs), args.serving_max_K, args.serving_max_states, len(tickets), rate,
+                            100 * rate / capacity),
             "capacity_certs_per_s": capacity, "capacity_wall_seconds": cap_wall, "rate": rate,
+            "capacity_p50_s": serve.percentile(cap_lat, 50), "capacity_p99_s": serve.percentile(cap_lat, 99),
+            "capacity_errors": len(cap_errs), "poisson_requests": len(tickets),
             "plain_gpu_certs_per_s": plain, "plain_sample": ns, "capacity_vs_plain": capacity / plain,
             "plain_ms_per_request": {c: round(1e3 * sum(v) / len(v), 2) for c, v in sorted(per_class.items())},
             "class_share": {c: round(len(v) / len(reqs), 3) for c, v in sorted(members.items())},
-            "certs_per_s": len(reqs) / wall, "wall_seconds": wall, "p50_s": serve.percentile(lat, 50),
+            "certs_per_s": len(tickets) / wall, "wall_seconds": wall, "p50_s": serve.percentile(lat, 50),
             "p99_s": serve.percentile(lat, 99), "dense_p50_s": serve.percentile(dl, 50),
             "dense_p99_s": serve.percentile(dl, 99), "sparse_p50_s": serve.percentile(sl, 50),
             "sparse_p99_s": serve.percentile(sl, 99), "errors": len(errs),
-            "certified_per_s": (len(reqs) - len(errs)) / wall, "sparse_max_states": args.serving_max_states,
+            "certified_per_s": (len(tickets) - len(errs)) / wall, "sparse_max_states": args.serving_max_states,
             "error_kinds": sorted({type(tk.error).__name__ + ": " + str(tk.error)[:80] for tk in errs}),
             "witnesses_ok": ok, "same_distances_both_runs": same}
 
@@ -866,16 +1022,16 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=512,
                     help="problems of the scan certified one at a time for the plain-GPU rate (default: all)")
     ap.add_argument("--sparse-K", type=int, default=12)
-    ap.add_argument("--serving-requests", type=int, default=600)
+    ap.add_argument("--serving-requests", type=int, default=10000)
+    ap.add_argument("--serving-poisson-requests", type=int, default=2000,
+                    help="requests of the Poisson stream, the first ones of the mix (0: all)")
     ap.add_argument("--serving-rate", type=float, default=None,
                     help="Poisson arrival rate (default: --serving-load times the measured capacity)")
-    ap.add_argument("--serving-load", type=float, default=0.25,
-                    help="Poisson rate as a fraction of the closed-loop capacity (the closed loop runs batches of "
-                         "hundreds; trickling arrivals make small ones, and about a quarter of that rate is "
-                         "sustained)")
+    ap.add_argument("--serving-load", type=float, default=0.5,
+                    help="Poisson rate as a fraction of the closed-loop capacity")
     ap.add_argument("--serving-plain-sample", type=int, default=60)
-    ap.add_argument("--serving-max-K", type=int, default=8)
-    ap.add_argument("--serving-max-states", type=int, default=30_000_000,
+    ap.add_argument("--serving-max-K", type=int, default=10)
+    ap.add_argument("--serving-max-states", type=int, default=4_000_000,
                     help="state budget of every sparse table, on the served and the one-at-a-time path alike (a "
                          "request past it is answered BudgetExceeded, not certified)")
     ap.add_argument("--profile-pass", type=int, default=0, metavar="N",
@@ -925,12 +1081,15 @@ def main():
     if not args.skip_batched:
         out["batched"] = bench_batched(args, inputs["scan"])
         progress("batched done")
+        _release_cache()
     if not args.skip_sparse:
         out["sparse"] = bench_sparse(args)
         progress("sparse done")
+        _release_cache()
     if not args.skip_tree:
         out["tree"] = bench_tree(args)
         progress("tree done")
+        _release_cache()
     if not args.skip_tables:
         out["tables"] = bench_tables(args)
         progress("tables done")
@@ -940,6 +1099,9 @@ def main():
     if not args.skip_serving:
         out["serving"] = bench_serving(args, inputs["requests"])
         progress("serving done")
+    if not args.no_cpu:
+        out["cpu_configs"] = bench_cpu_configs(args, inputs.get("scan"), inputs.get("requests"))
+        progress("CPU baselines done")
     if not args.no_cpu and not args.skip_dense:
         import oracle
         if oracle.have_ref():
@@ -952,7 +1114,8 @@ def main():
                                    "kind": "reference",
                                    "sample": "observable 0 of the same instance (work %d states): the reference's "
                                              "viterbi.c (OpenMP) driven op by op" % plans[0].work(),
-                                   "seconds": cpu_s, "distance": cv, "matches_gpu": cv == values[0]}
+                                   "seconds": cpu_s, "distance": cv, "matches_gpu": cv == values[0],
+                                   "host": host_cpu(threads)}
     print(json.dumps(out))
 
 

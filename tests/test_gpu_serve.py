@@ -93,3 +93,79 @@ def test_unplannable_dense_request_fails_alone():
             tickets[3].wait(timeout=300)
     finally:
         srv.close()
+
+
+def _cfg5_requests(n, seed=5000):
+    """The first n requests of the serving configuration's mix (dense widths 10-25, sparse MITM with K 4-10 on
+    quasi-cyclic codes with l, m in 6-15), built as the benchmark builds them."""
+    out = []
+    for kind, a in workloads.serving_mix(n, seed=seed, K=(4, 10)):
+        if kind == "sparse":
+            out.append(("sparse", problem.quasi_cyclic(a["l"], a["m"], seed=a["seed"]), a["K"]))
+        else:
+            rng = np.random.default_rng(a["seed"])
+            out.append(("dense", workloads.banded_instance(rng, a["seed"], a["width"], a["weighted"]), None))
+    return out
+
+
+def test_serving_configuration_shape_alone_equals_mix_and_oracle():
+    """Requests of the serving configuration itself: served together, each equals its one-at-a-time answer (the
+    same error where a sparse table passes the state budget), every witness checks against H and L, and the
+    oracle's CPU kernels agree where they run in seconds: dense plans of work <= 2e8 (or.run_ops, every
+    observable), sparse requests of at most 56 row slots and 150 columns (the oracle's tables, the same sweeps)."""
+    import oracle
+    from tropidist_amd import ordering, sparse_dp, trellis
+    print("case serve cfg5 shape", flush=True)
+    reqs = _cfg5_requests(90)
+    assert {k for k, _, _ in reqs} == {"dense", "sparse"}
+    states = 4_000_000
+    srv = serve.Server(memory_bytes=48 << 30, dense_lanes=3, sparse_arena_bytes=16 << 30, sparse_max_states=states)
+    try:
+        tickets = [srv.submit_dense(p) if kind == "dense" else srv.submit_sparse(p, K) for kind, p, K in reqs]
+        for tk in tickets:
+            assert tk._event.wait(600), "request not answered"
+    finally:
+        srv.close()
+    checked = {"dense_oracle": 0, "sparse_oracle": 0}
+    for (kind, p, K), tk in zip(reqs, tickets):
+        if kind == "dense":
+            alone = auto.certify(p)
+            assert tk.error is None, tk.error
+            rec = tk.result
+            assert rec["distance"] == alone["distance"] and rec["per_observable"] == alone["per_observable"]
+            if rec["distance"] is not None:
+                assert rec["witness_ok"] and verify_witness(p, rec["witness"], rec["distance"], rec["observable"])
+            order = auto.planning_order(p)
+            plans = trellis.build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], order, p.nrows, p.w)
+            if all(pl.work() <= 2e8 for pl in plans if pl.feasible):
+                want = [oracle.run_plan_c(pl, "oracle") if pl.feasible else None for pl in plans]
+                assert want == rec["per_observable"]
+                checked["dense_oracle"] += 1
+            continue
+        try:
+            alone = serve.sparse_certify(p, K, max_states=states)
+        except (sparse_dp.BudgetExceeded, ValueError) as exc:
+            # both paths refuse the request: a table past the state budget, or more row slots than a state holds
+            assert type(tk.error) is type(exc), (tk.error, exc)
+            continue
+        assert tk.error is None, tk.error
+        assert tk.result["distance"] == alone["distance"]
+        if alone["distance"] is not None:
+            assert tk.result["witness_ok"] and alone["witness_ok"]
+        order = ordering.greedy_order_native(p.h, p.nrows, p.words())
+        trs = [sparse_dp.Trellis.for_query(p, i, order) for i in range(p.nobs)]
+        if p.ncols <= 150 and all(t.nslots <= 56 for t in trs if t.feasible):
+            best = None
+            for tr in trs:
+                if not tr.feasible:
+                    continue
+                k = K if best is None else best - 1
+                if k < 1:
+                    break
+                r = sparse_dp.query(tr, k, mode="mitm", use_h=False, witness=False, backend=oracle.OracleTables(tr))
+                if r["value"] is not None:
+                    best = r["value"]
+            assert best == alone["distance"]
+            checked["sparse_oracle"] += 1
+    print("  oracle-checked", checked, flush=True)
+    assert checked["dense_oracle"] >= 30

@@ -161,6 +161,19 @@ def torch():
     return _torch
 
 
+def retry_after_release(fn):
+    """fn(), and once more after torch's cached device blocks are handed back when it raises MemoryError: the
+    library's own allocations (lookup tables, the search's node buffers, the sparse engine's arena) come from the
+    driver, which does not see memory that torch holds cached for later tensors."""
+    try:
+        return fn()
+    except MemoryError:
+        t = torch()
+        t.cuda.synchronize()
+        t.cuda.empty_cache()
+        return fn()
+
+
 def device_empty(nbytes, dtype=None):
     t = torch()
     return t.empty(max(1, int(nbytes)), dtype=dtype or t.uint8, device="cuda")

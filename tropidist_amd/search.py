@@ -274,8 +274,9 @@ class Searcher:
         a = self.anchors if anchors is None else np.ascontiguousarray(anchors, dtype=np.int32)
         rk = self.rank if rank is None else np.ascontiguousarray(rank, dtype=np.int32)
         use_ex = exclusions and self.ex_start is not None
-        return self._call(K, self.tables, self.main, rk, a, use_ex, enum_all, task_depth, time_limit, sol_cap,
-                          use_kernel=use_kernel, cover=cover, pattern=pattern)
+        return _lib.retry_after_release(lambda: self._call(K, self.tables, self.main, rk, a, use_ex, enum_all,
+                                                           task_depth, time_limit, sol_cap, use_kernel=use_kernel,
+                                                           cover=cover, pattern=pattern))
 
 
 def verify(p, cols):

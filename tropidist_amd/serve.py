@@ -78,7 +78,7 @@ def _dense_footprint(plans):
 
 class Server:
     def __init__(self, memory_bytes=64 << 30, dense_lanes=2, max_batch=1024, sparse_arena_bytes=4 << 30,
-                 sparse_max_states=30_000_000, sparse_admit=0.5, sparse_max_jobs=4096):
+                 sparse_max_states=30_000_000, sparse_admit=0.5, sparse_max_jobs=4096, prewarm=True):
         if sparse_arena_bytes >= memory_bytes:
             raise ValueError("the sparse arena (%d bytes) must leave room for dense requests in %d bytes"
                              % (sparse_arena_bytes, memory_bytes))
@@ -98,12 +98,21 @@ class Server:
         t = _lib.torch()
         # Each dense lane keeps two streams for its life: its persistent launches, and the wide problems' rounds
         # that run beside them.  torch's caching allocator keeps freed blocks per stream, so fixed streams let a
-        # lane's tables, workspaces and witness arenas reuse its own cached blocks instead of mapping new device
-        # memory for every batch (and, once the device is full of blocks cached for streams no longer used,
-        # unmapping them again).
+        # lane's tables, workspaces and witness arenas reuse its own cached blocks.  prewarm: one segment per stream
+        # (the lane's share of the dense budget for the wide stream, a quarter of that for the other) is taken and
+        # given back here, so that those allocations split cached memory instead of calling hipMalloc while requests
+        # run (a traced serving run showed single hipMalloc calls of 2 s, during which the other lanes' launches
+        # waited).
         self.threads = []
+        share = self.budget // max(1, dense_lanes)
         for k in range(dense_lanes):
-            th = threading.Thread(target=self._dense_lane, args=(t.cuda.Stream(), t.cuda.Stream()), daemon=True)
+            st_main, st_wide = t.cuda.Stream(), t.cuda.Stream()
+            if prewarm:
+                for st, nbytes in ((st_wide, share), (st_main, share // 4)):
+                    with t.cuda.stream(st):
+                        block = t.empty(max(1, nbytes), dtype=t.uint8, device="cuda")
+                        del block
+            th = threading.Thread(target=self._dense_lane, args=(st_main, st_wide), daemon=True)
             th.start()
             self.threads.append(th)
         sst = t.cuda.Stream()

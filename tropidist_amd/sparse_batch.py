@@ -85,9 +85,13 @@ class Engine:
         self._sh = _lib.stream_handle(stream)
         self.max_states = max_states
         self.max_jobs = max_jobs
-        self.h = self.L.tdp_spb_create(arena_bytes, max_jobs, self._sh)
-        if not self.h:
-            raise MemoryError(self.L.tdp_last_error().decode(errors="replace"))
+        def create():
+            h = self.L.tdp_spb_create(arena_bytes, max_jobs, self._sh)
+            if not h:
+                raise MemoryError(self.L.tdp_last_error().decode(errors="replace"))
+            return h
+
+        self.h = _lib.retry_after_release(create)
         self.active = 0
         self._ids = (ctypes.c_int64 * 4096)()
 

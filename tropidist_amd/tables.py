@@ -66,9 +66,13 @@ class Table:
         self.log2slots = table_log2slots(p.ncols, k, load)
         L = _lib.lib()
         self.L = L
-        self.handle = L.tdp_tab_create(self.log2slots, 1 if self.wide else 0)
-        if not self.handle:
-            raise MemoryError("table with 2^%d slots: %s" % (self.log2slots, L.tdp_last_error().decode()))
+        def create():
+            h = L.tdp_tab_create(self.log2slots, 1 if self.wide else 0)
+            if not h:
+                raise MemoryError("table with 2^%d slots: %s" % (self.log2slots, L.tdp_last_error().decode()))
+            return h
+
+        self.handle = _lib.retry_after_release(create)
         t = time.time()
         _lib.check(L.tdp_tab_build(self.handle, p.ncols, self.zh.ctypes.data, self.zl.ctypes.data,
                                    self.lg.ctypes.data, k), "tdp_tab_build")
