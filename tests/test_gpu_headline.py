@@ -49,3 +49,22 @@ def test_weighted_four_observable_banded_at_2_22():
     want = [_reference_value(pl) for pl in plans]
     assert got == want
     assert [kernel.run_plan(pl, executor="per-op")[0] for pl in plans] == want
+
+
+def test_witness_of_the_single_large_instance_at_2_26():
+    """The benchmark's single large instance (banded R=600 N=6000 W=25, costs 1-3, four logical rows, seed 2000):
+    the minimising observable's fused witness run at 2^26 entries gives the step's distance, and its columns form a
+    solution of that cost (H e = 0, the observable flipped) -- the certificate the reference's dp_witness gives."""
+    from tropidist_amd import auto, workloads
+    from tropidist_amd.problem import verify_witness
+    p = workloads.single_large(W=25)
+    order = list(range(p.ncols))
+    plans = trellis.build_plans_native(p.h, p.l, [1 << i for i in range(p.nobs)], order, p.nrows, p.w)
+    feas = [(i, pl) for i, pl in enumerate(plans) if pl.feasible]
+    values = fused.run_many([pl for _, pl in feas])
+    best = min(values)
+    obs, plan = feas[values.index(best)]
+    assert plan.peak_bits == 26
+    print("case witness of the single large instance, observable", obs, "peak 2^26", flush=True)
+    v, cols = auto.witness_plan(plan, order)
+    assert v == best and verify_witness(p, cols, best, obs) and p.cost(cols) == best

@@ -20,8 +20,11 @@ at the last step (one at least when nothing runs).  A request one of whose queri
 other queries and goes back to the head of the queue while others run.  A request that does not fit alone, or has
 more queries than the engine has slots, goes to a lane of its own with its own stream, where the one-at-a-time path
 (sparse_certify, checkpointed sweeps) answers it while the engine keeps stepping the others.
-Every sparse request deepens through sparse_batch.threshold_ladder(K) and every table is held to
-`sparse_max_states` states on both paths; a request past that budget is answered with sparse_dp.BudgetExceeded.
+A sparse request sweeps at its threshold K, as the one-at-a-time path does (sparse_deepen=True deepens through
+sparse_batch.threshold_ladder(K) first: on the full serving mix with K <= 10 that took 33.7 s for its 1,000 sparse
+requests against 22.9 s without, and let two requests finish below the state budget that the one-at-a-time path
+refuses).  Every table is held to `sparse_max_states` states on both paths; a request past that budget is answered
+with sparse_dp.BudgetExceeded.
 
 The result of a request does not depend on what it was batched with: every dense plan runs alone in its workgroup on
 its own tables, every sparse query on its own tables and counters (tests/test_gpu_serve.py checks each result alone
@@ -78,12 +81,14 @@ def _dense_footprint(plans):
 
 class Server:
     def __init__(self, memory_bytes=64 << 30, dense_lanes=2, max_batch=1024, sparse_arena_bytes=4 << 30,
-                 sparse_max_states=30_000_000, sparse_admit=0.5, sparse_max_jobs=4096, prewarm=True):
+                 sparse_max_states=30_000_000, sparse_admit=0.5, sparse_max_jobs=4096, prewarm=True,
+                 sparse_deepen=False):
         if sparse_arena_bytes >= memory_bytes:
             raise ValueError("the sparse arena (%d bytes) must leave room for dense requests in %d bytes"
                              % (sparse_arena_bytes, memory_bytes))
         self.budget = memory_bytes - sparse_arena_bytes
         self.sparse_admit = sparse_admit        # admit sparse requests while the arena is less full than this
+        self.sparse_deepen = sparse_deepen
         self.in_use = 0
         self.max_batch = max_batch
         self.idle_lanes = 0                     # dense lanes waiting for requests
@@ -286,7 +291,7 @@ class Server:
             now = time.perf_counter()
             for tk in new:                      # host planning (C++ trellis words) as the requests arrive
                 tk.t_start = now
-                r = sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk)
+                r = sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk, deepen=self.sparse_deepen)
                 try:
                     r.plan()
                     if len(r.trs) > eng.max_jobs:   # more queries than the engine has slots: one at a time

@@ -192,19 +192,20 @@ def threshold_ladder(K, first=4, step=2):
 
 class Request:
     """A whole sparse request: the least weight <= K over every observable of a problem (serve.sparse_certify's
-    record).  The request deepens through threshold_ladder(K); at each threshold all feasible observables run at
-    once, and the first threshold with a solution ends it: the distance is the least value and the observable the
+    record).  The request deepens through threshold_ladder(K) (deepen=False: one sweep at K); at each threshold all
+    feasible observables run at once, and the first threshold with a solution ends it: the distance is the least
+    value and the observable the
     first reaching it -- what the sequential loop with thresholds best - 1 finds.  After a threshold without a
     solution, finish() sets `pending` and the caller admits the request again for the next one."""
     __slots__ = ("problem", "K", "order", "trs", "ids", "record", "error", "tag", "left", "ladder", "stage",
                  "pending", "results")
 
-    def __init__(self, problem, K, order=None, tag=None):
+    def __init__(self, problem, K, order=None, tag=None, deepen=True):
         self.problem, self.K, self.order, self.tag = problem, K, order, tag
         self.trs, self.ids = [], []
         self.record = self.error = None
         self.left = 0
-        self.ladder, self.stage, self.pending = threshold_ladder(K), 0, False
+        self.ladder, self.stage, self.pending = (threshold_ladder(K) if deepen else [K]), 0, False
         self.results = {}
 
     @property
