@@ -184,7 +184,7 @@ def _bare_server(engine):
     srv.cv = threading.Condition()
     srv.sparse_q, srv.dense_q, srv.single_q = collections.deque(), collections.deque(), collections.deque()
     srv.sparse_failed, srv.closed, srv.done, srv.in_use = None, False, [], 0
-    srv.sparse_engine, srv.sparse_admit = engine, 0.5
+    srv.sparse_engine, srv.sparse_admit, srv.sparse_deepen = engine, 0.5, False
     return srv
 
 
