@@ -152,6 +152,8 @@ def test_serving_configuration_shape_alone_equals_mix_and_oracle():
         assert tk.result["distance"] == alone["distance"]
         if alone["distance"] is not None:
             assert tk.result["witness_ok"] and alone["witness_ok"]
+        if alone["path"] == "search":            # a trellis beyond the device states: no sparse tables to compare
+            continue
         order = ordering.greedy_order_native(p.h, p.nrows, p.words())
         trs = [sparse_dp.Trellis.for_query(p, i, order) for i in range(p.nobs)]
         if p.ncols <= 150 and all(t.nslots <= 56 for t in trs if t.feasible):
@@ -169,3 +171,36 @@ def test_serving_configuration_shape_alone_equals_mix_and_oracle():
             checked["sparse_oracle"] += 1
     print("  oracle-checked", checked, flush=True)
     assert checked["dense_oracle"] >= 30
+
+
+WIDE_REQUESTS = [(13, 15, 5076, 5), (15, 11, 6440, 9), (15, 15, 6809, 10)]   # trellises of 122-130 row slots
+
+
+def test_requests_beyond_the_state_width_go_to_the_search():
+    """Sparse requests of the serving mix whose trellises need more row slots than a device state holds (122-130 >
+    120) are answered by the weight-bounded search on the GPU (gm_search's route), alone and through the server
+    alike: the least weight <= K with a checked witness, and None below it."""
+    from tropidist_amd import ordering, sparse_dp
+    print("case serve wide-trellis requests", flush=True)
+    probs = [(problem.quasi_cyclic(l, m, seed=s), K) for l, m, s, K in WIDE_REQUESTS]
+    alone = []
+    for p, K in probs:
+        order = ordering.greedy_order_native(p.h, p.nrows, p.words())
+        with pytest.raises(ValueError):
+            [sparse_dp.Trellis.for_query(p, i, order) for i in range(p.nobs)]
+        rec = serve.sparse_certify(p, K)
+        assert rec["path"] == "search" and rec["witness_ok"]
+        if rec["distance"] is not None:
+            assert rec["distance"] <= K and len(rec["witness"]) == rec["distance"]
+            below = serve.sparse_certify(p, rec["distance"] - 1)
+            assert below["distance"] is None
+        alone.append(rec)
+    srv = serve.Server(memory_bytes=8 << 30, dense_lanes=1, sparse_arena_bytes=1 << 30)
+    try:
+        tickets = [srv.submit_sparse(p, K) for p, K in probs]
+        got = [tk.wait(timeout=300) for tk in tickets]
+    finally:
+        srv.close()
+    for a, g in zip(alone, got):
+        assert (g["distance"], g["observable"], g["path"]) == (a["distance"], a["observable"], "search")
+        assert g["witness_ok"]

@@ -19,7 +19,8 @@ them per step.  Requests are admitted while the arena is less than `sparse_admit
 at the last step (one at least when nothing runs).  A request one of whose queries finds the arena full stops its
 other queries and goes back to the head of the queue while others run.  A request that does not fit alone, or has
 more queries than the engine has slots, goes to a lane of its own with its own stream, where the one-at-a-time path
-(sparse_certify, checkpointed sweeps) answers it while the engine keeps stepping the others.
+(sparse_certify, checkpointed sweeps) answers it while the engine keeps stepping the others; so does a request whose
+trellis needs more row slots than a device state holds, which sparse_certify answers by the weight-bounded search.
 A sparse request sweeps at its threshold K, as the one-at-a-time path does (sparse_deepen=True deepens through
 sparse_batch.threshold_ladder(K) first: on the full serving mix with K <= 10 that took 33.7 s for its 1,000 sparse
 requests against 22.9 s without, and let two requests finish below the state budget that the one-at-a-time path
@@ -293,7 +294,13 @@ class Server:
                 tk.t_start = now
                 r = sparse_batch.Request(tk.problem, tk.K, tk.order, tag=tk, deepen=self.sparse_deepen)
                 try:
-                    r.plan()
+                    try:
+                        r.plan()
+                    except ValueError as exc:
+                        if not wide_trellis(exc):
+                            raise
+                        self._to_single(r)          # beyond the device states' row slots: the search answers it
+                        continue
                     if len(r.trs) > eng.max_jobs:   # more queries than the engine has slots: one at a time
                         self._to_single(r)
                         continue
@@ -398,6 +405,32 @@ def claim_count(waiting, idle_lanes, max_batch, share_above=64):
     return min(max_batch, n)
 
 
+def wide_trellis(exc):
+    """Is this the error of a trellis with more row slots than a device state holds?"""
+    return isinstance(exc, ValueError) and "bits per state" in str(exc)
+
+
+def search_certify(p, K, order=None, k_table=2):
+    """The record of sparse_certify for a problem whose trellis is too wide for the sparse states: the least weight
+    <= K of a solution over all observables by the weight-bounded search on the GPU (search.distance, thresholds 1..K
+    on label-free tables, the gm_search route of the reference's pipeline for wide codes).  The observable is the
+    lowest one the witness flips."""
+    from . import search
+    from .problem import verify_witness
+    r = search.distance(p, k_table=k_table, kmax=K, start=1, labelfree=True)
+    if r["distance"] is None or not r["solutions"]:
+        return {"distance": None, "witness": None, "observable": None, "witness_ok": True, "K": K, "path": "search",
+                "peak_states": 0}
+    cols = sorted(r["solutions"][0])
+    lg = 0
+    for j in cols:
+        lg ^= p.l[j]
+    obs = (lg & -lg).bit_length() - 1
+    v = r["distance"]
+    return {"distance": v, "witness": cols, "observable": obs, "K": K,
+            "witness_ok": verify_witness(p, cols, v, obs), "path": "search", "peak_states": 0}
+
+
 def sparse_certify(p, K, order=None, max_states=30_000_000, deepen=False):
     """Least weight <= K of a solution over all observables by the sparse MITM DP: {"distance", "witness",
     "observable", "witness_ok"}.  One query at a time on the single-query path: the observables are swept in turn,
@@ -405,11 +438,17 @@ def sparse_certify(p, K, order=None, max_states=30_000_000, deepen=False):
     default split).  deepen: first through the lower thresholds of sparse_batch.threshold_ladder(K), as the
     engine's requests do (the same result; one query at a time it is slower on the serving mix, whose codes mostly
     have distances above K).  A table beyond max_states raises sparse_dp.BudgetExceeded (the request is answered
-    with that error)."""
+    with that error).  A problem whose trellis needs more row slots than a device state holds is answered by the
+    weight-bounded search instead (search_certify, path "search")."""
     from .ordering import greedy_order_native
     from .problem import verify_witness
     order = order if order is not None else greedy_order_native(p.h, p.nrows, p.words())
-    trs = [sparse_dp.Trellis.for_query(p, i, order) for i in range(p.nobs)]
+    try:
+        trs = [sparse_dp.Trellis.for_query(p, i, order) for i in range(p.nobs)]
+    except ValueError as exc:
+        if not wide_trellis(exc):
+            raise
+        return search_certify(p, K, order)
     best, peak = None, 0
     for kt in (sparse_batch.threshold_ladder(K) if deepen else [K]):
         for i, tr in enumerate(trs):
