@@ -281,6 +281,38 @@ int tdp_sp_join(const void *d_f, uint64_t nf, const void *d_bents, const uint32_
 /* d_hist[v] = number of states of value v (256 device counters).  Replaces sp_hist (cdist/sparse_dp.c:238-242). */
 int tdp_sp_hist(const void *d_ents, uint64_t n, uint64_t *d_hist, void *stream);
 
+/* ------------------------------------------------------------------ dense tables of the sparse sweeps' fallback
+ * A dense table holds every state of a cut's active rows: d_T[i] (uint8, 255 = absent, 2^w entries, 4-byte aligned)
+ * for the compact index i of a state, bit k of i = the state's bit at slot slots[k] (w <= 40 slots, ascending,
+ * host array).  The Dense class of cdist/sparse_dp.py:221-289. */
+
+/* d_T = 255 everywhere, then d_T[compact(key)] = value for each of the n states (their bits lie in the slots).
+ * The sparse-to-dense switch of cdist/sparse_dp.py:698-705. */
+int tdp_dn_from_sparse(const void *d_ents, uint64_t n, const uint8_t *slots, int w, uint8_t *d_T, void *stream);
+
+/* The entries of d_T other than 255 as states into d_out (capacity cap, any order); device counters [0] = their
+ * number, [1] = 1 if it exceeded cap.  The dense-to-sparse switch of cdist/sparse_dp.py:694-697. */
+int tdp_dn_to_sparse(const uint8_t *d_T, int w, const uint8_t *slots, void *d_out, uint64_t cap,
+                     unsigned long long *d_counters, void *stream);
+
+/* One column step of a dense table (cdist/sparse_dp.py:680-693): the rows starting at the column take new bits of
+ * the "mid" space (w_mid = w_old + nstart bits; their half of it absent), T'[x] = min(T[x], T[x ^ cm] + 1) there,
+ * the rows ending at the column are fixed at their target bit and removed, and states above cap, or (use_h) whose
+ * value plus ceil((extra + sum_c cls_w[c] |(i ^ tmask) & cls_mask[c]|) / 720720) exceeds K, are dropped.
+ * start_pos: mid positions of the starting rows, descending; end_pos / end_bit: mid positions of the ending rows,
+ * ascending, and their targets; cls_mask (ncls <= 64) and tmask in the new table's compact space (host arrays).
+ * d_U: 2^(w_mid - nend) bytes; *d_count (device) = its entries other than 255. */
+int tdp_dn_step(const uint8_t *d_T, int w_old, uint8_t *d_U, int w_mid, const uint8_t *start_pos, int nstart,
+                const uint8_t *end_pos, const uint8_t *end_bit, int nend, uint64_t cm, int cap, int K, int ncls,
+                const uint64_t *cls_mask, const int64_t *cls_w, int64_t extra, uint64_t tmask, int use_h,
+                unsigned long long *d_count, void *stream);
+
+/* tdp_sp_join against a dense backward table: d_best = {total, key hi, key lo, forward value} of the least
+ * value(f) + T[compact(key(f) ^ tmask)] with the least forward key.  NumpyTables.join over a Dense table
+ * (cdist/sparse_dp.py:420-444). */
+int tdp_dn_join(const void *d_f, uint64_t nf, const uint8_t *d_T, int w, const uint8_t *slots, const uint64_t *tmask,
+                unsigned long long *d_best, void *stream);
+
 /* ------------------------------------------------------------------ one sparse query, natively
  * A whole query of cdist/sparse_dp.py:985-1095 (mode TDP_SQ_MITM: backward sweep, forward frontier with joins at
  * every cut, traceback) or one forward sweep (TDP_SQ_FORWARD, weights <= K, the value of state 0 at cut N), run by

@@ -329,14 +329,16 @@ class OracleTables:
 
     @staticmethod
     def size(tab):
-        return tab.size
+        return tab.count() if isinstance(tab, OracleDense) else tab.size
 
     @staticmethod
     def nbytes(tab):
-        return tab.nbytes
+        return tab.T.nbytes if isinstance(tab, OracleDense) else tab.nbytes
 
     @staticmethod
     def arrays(tab):
+        if isinstance(tab, OracleDense):
+            tab = tab.to_sparse()
         return tab & np.uint64(KEYMASK56), (tab >> np.uint64(56)).astype(np.uint8)
 
     @staticmethod
@@ -346,6 +348,8 @@ class OracleTables:
 
     @staticmethod
     def lookup(tab, s):
+        if isinstance(tab, OracleDense):
+            return tab.lookup(s)
         keys = tab & np.uint64(KEYMASK56)
         i = int(np.searchsorted(keys, np.uint64(s)))
         if i < keys.size and int(keys[i]) == s:
@@ -385,7 +389,7 @@ class OracleTables:
 
     @staticmethod
     def persistent(tab):
-        return tab.copy()
+        return OracleDense(tab.slots, tab.T.copy()) if isinstance(tab, OracleDense) else tab.copy()
 
     @staticmethod
     def truncate(tab, below):
@@ -394,6 +398,8 @@ class OracleTables:
     def join(self, ch, btab, tmask):
         if not ch.size:
             return None
+        if isinstance(btab, OracleDense):
+            return btab.join(ch, tmask)
         fb, bb = ctypes.c_uint64(0), ctypes.c_uint64(0)
         tot = self.joinf(ptr(ch), ch.size, ptr(btab), btab.size, tmask, ctypes.byref(fb), ctypes.byref(bb))
         if tot < 0:
@@ -403,6 +409,111 @@ class OracleTables:
     @staticmethod
     def hist(tab):
         return np.bincount((tab >> np.uint64(56)).astype(np.int64), minlength=256)
+
+
+    # ---- the dense form (cdist/sparse_dp.py:221-289, 680-707)
+    @staticmethod
+    def is_dense(tab):
+        return isinstance(tab, OracleDense)
+
+    @staticmethod
+    def to_dense(tab, slots):
+        D = OracleDense(slots, np.full(1 << len(slots), INF, dtype=np.uint8))
+        keys, vals = tab & np.uint64(KEYMASK56), (tab >> np.uint64(56)).astype(np.uint8)
+        D.T[D.compact(keys)] = vals
+        return D
+
+    @staticmethod
+    def from_dense(D):
+        return OracleTables.from_arrays(*D.arrays())
+
+    def dense_step(self, D, starts, ends, upd, cap, K, tmask, hd):
+        D = OracleDense(D.slots, D.T.copy())
+        for s, _ in sorted(starts):
+            D.insert_slot(s)
+        cm = D.cmask(upd)
+        if cm:
+            olib().or_dp_update(ptr(D.T), D.T.size, cm)
+        for s, t in ends:
+            D.select_slot(s, t)
+        D.prune(cap, K, tmask, hd)
+        return D
+
+
+class OracleDense:
+    """A dense table of the sweeps' fallback, restated from cdist/sparse_dp.py:221-289: T[i] for the compact index
+    i of a state (bit k of i = the state's bit at slots[k], slots ascending), INF = absent; the update is
+    liboracle's restatement of viterbi.c's dp_update."""
+
+    def __init__(self, slots, T):
+        self.slots, self.T = list(slots), T
+
+    def compact(self, keys):
+        keys = np.asarray(keys, dtype=np.uint64)
+        out = np.zeros(keys.shape, dtype=np.int64)
+        for k, s in enumerate(self.slots):
+            out |= ((keys >> np.uint64(s)) & np.uint64(1)).astype(np.int64) << k
+        return out
+
+    def expand(self, idx):
+        out = np.zeros(idx.shape, dtype=np.uint64)
+        for k, s in enumerate(self.slots):
+            out |= ((idx >> k) & 1).astype(np.uint64) << np.uint64(s)
+        return out
+
+    def cmask(self, mask):
+        return sum(1 << k for k, s in enumerate(self.slots) if (int(mask) >> s) & 1)
+
+    def count(self):
+        return int(np.count_nonzero(self.T != INF))
+
+    def arrays(self):
+        idx = np.flatnonzero(self.T != INF)
+        return self.expand(idx), self.T[idx]
+
+    def to_sparse(self):
+        return OracleTables.from_arrays(*self.arrays())
+
+    def lookup(self, s):
+        if int(s) & ~sum(1 << x for x in self.slots):
+            return INF
+        return int(self.T[int(self.compact(np.array([s], dtype=np.uint64))[0])])
+
+    def insert_slot(self, slot):                 # cdist/sparse_dp.py:262-267
+        pos = sum(1 for x in self.slots if x < slot)
+        A = self.T.reshape(-1, 1 << pos)
+        self.T = np.concatenate([A, np.full_like(A, INF)], axis=1).reshape(-1)
+        self.slots.insert(pos, slot)
+
+    def select_slot(self, slot, target):         # cdist/sparse_dp.py:269-273
+        pos = self.slots.index(slot)
+        self.T = np.ascontiguousarray(self.T.reshape(-1, 2, 1 << pos)[:, target, :]).reshape(-1)
+        self.slots.pop(pos)
+
+    def prune(self, cap, K, tmask, hd):          # cdist/sparse_dp.py:275-289
+        T = self.T
+        T[(T > cap) & (T != INF)] = INF
+        if hd is None:
+            return
+        masks, weights, extra = hd
+        u = np.arange(T.size, dtype=np.uint64) ^ np.uint64(self.cmask(tmask))
+        tot = np.full(T.size, int(extra), dtype=np.int64)
+        for m, w in zip(masks, weights):
+            tot += np.int64(int(w)) * np.bitwise_count(u & np.uint64(self.cmask(m))).astype(np.int64)
+        h = (tot + (720720 - 1)) // 720720
+        T[(T.astype(np.int64) + h > K) & (T != INF)] = INF
+
+    def join(self, ch, tmask):                   # cdist/sparse_dp.py:420-444 over a dense table
+        ck, cv = ch & np.uint64(KEYMASK56), (ch >> np.uint64(56)).astype(np.int64)
+        q = ck ^ np.uint64(tmask)
+        bv = np.array([self.lookup(int(x)) for x in q], dtype=np.int64)
+        hit = bv != INF
+        if not hit.any():
+            return None
+        tot = cv[hit] + bv[hit]
+        i = int(np.argmin(tot))                  # ch is sorted by key: the least key among the least totals
+        fi = int(np.flatnonzero(hit)[i])
+        return int(tot[i]), int(ck[fi]), int(cv[fi]), int(q[fi])
 
 
 class RefTables(OracleTables):

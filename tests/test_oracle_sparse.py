@@ -210,3 +210,77 @@ def test_request_deepening_equals_brute_force():
             assert len(eng.released) == len(eng.q)        # every query released once
             checked += 1
     assert checked > 100
+
+
+def _cut_tables(tr, direction, K, use_h, dense, be, dense_max=1 << 30):
+    """{cut: sorted (key, value) pairs} of one whole sweep keeping weights <= K, and the sweep."""
+    sw = sparse_dp.Sweep(tr, direction, K, K, use_h, backend=be, dense=dense, dense_max=dense_max)
+    out = {}
+
+    def keep(c, tab, changed, extra):
+        keys, vals = sw.be.arrays(tab)
+        out[c] = sorted(zip((int(k) for k in keys), (int(v) for v in vals)))
+
+    sw.run(on_cut=keep)
+    return out, sw
+
+
+def test_dense_fallback_keeps_the_sparse_tables():
+    """The dense form changes the representation only: a sweep with the fallback (every table dense from the first
+    step at 1e-9, switching back and forth at 0.25) holds, at every cut, exactly the states and values of the plain
+    sparse sweep, which test_step_and_join_equal_reference_c and the brute-force tests pin.  dense_max = 8 makes
+    the wider cuts switch back to sparse tables and the narrower ones dense again."""
+    rng = random.Random(11)
+    dense_cuts = 0
+    for _ in range(40):
+        p = random_problem(rng)
+        order = list(range(p.ncols))
+        rng.shuffle(order)
+        for obs in range(p.nobs):
+            tr = sparse_dp.Trellis.for_query(p, obs, order)
+            if not tr.feasible:
+                continue
+            for direction in (+1, -1):
+                K = rng.randint(1, p.ncols)
+                use_h = rng.random() < 0.5
+                plain, _ = _cut_tables(tr, direction, K, use_h, None, oracle.OracleTables(tr))
+                for dense, dmax in ((1e-9, 1 << 30), (0.25, 1 << 30), (1e-9, 8)):
+                    got, sw = _cut_tables(tr, direction, K, use_h, dense, oracle.OracleTables(tr), dmax)
+                    assert got == plain, (direction, K, use_h, dense, dmax)
+                    dense_cuts += sw.dense_cuts
+    assert dense_cuts > 100
+
+
+def test_dense_fallback_queries_against_brute_force():
+    """tests/test_sparse_dp.py:49-74 and 146-155 with the fallback: forward sweeps dense from the first step, and
+    meets in the middle whose backward sweep goes dense, answer every threshold as brute force does."""
+    rng = random.Random(3)
+    checked = dense_cuts = 0
+    for _ in range(40):
+        p = random_problem(rng)
+        order = list(range(p.ncols))
+        rng.shuffle(order)
+        for obs in range(p.nobs):
+            tr = sparse_dp.Trellis.for_query(p, obs, order)
+            if not tr.feasible:
+                continue
+            truth = brute(p, obs)
+            for K in range(1, p.ncols + 2):
+                want = truth if truth <= K else None
+                for mode, split, dense in (("forward", None, 1e-9), ("mitm", K // 2 + 1, 0.25), ("mitm", 1, 1e-9)):
+                    r = sparse_dp.query(tr, K, mode=mode, split=split, use_h=rng.random() < 0.5, dense=dense,
+                                        backend=oracle.OracleTables(tr), every=rng.choice([1, 2, 5]))
+                    assert r["value"] == want, (mode, split, dense, K, r["value"], want)
+                    if want is not None:
+                        check_witness(p, r["witness"], want, obs)
+                    checked += 1
+                    dense_cuts += r["dense_cuts"]
+    assert checked > 500 and dense_cuts > 1000
+
+
+def test_dense_fallback_at_the_8bit_boundary():
+    """tests/test_false_bounds.py:213-225 with dense=0.25: K = 254 forward and 256 in the middle stay exact."""
+    for n, mode in ((254, "forward"), (256, "mitm")):
+        tr = sparse_dp.Trellis.for_query(problem.chain(n), 0, list(range(n)))
+        r = sparse_dp.query(tr, n, mode=mode, backend=oracle.OracleTables(tr), use_h=False, dense=0.25)
+        assert r["value"] == n and r["witness"] == list(range(n)) and r["dense_cuts"] > 0

@@ -74,6 +74,10 @@ SIGNATURES = {
     "tdp_sp_truncate": (ci, [vp, u64, ci, vp, vp, vp]),
     "tdp_sp_join": (ci, [vp, u64, vp, vp, u64, vp, vp, vp]),
     "tdp_sp_hist": (ci, [vp, u64, vp, vp]),
+    "tdp_dn_from_sparse": (ci, [vp, u64, vp, ci, vp, vp]),
+    "tdp_dn_to_sparse": (ci, [vp, ci, vp, vp, u64, vp, vp]),
+    "tdp_dn_step": (ci, [vp, ci, vp, ci, vp, ci, vp, vp, ci, u64, ci, ci, ci, vp, vp, i64, u64, ci, vp, vp]),
+    "tdp_dn_join": (ci, [vp, u64, vp, ci, vp, vp, vp, vp]),
     "tdp_tab_create": (vp, [ci, ci]),
     "tdp_tab_free": (None, [vp]),
     "tdp_tab_count": (i64, [vp]),

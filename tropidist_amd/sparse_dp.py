@@ -18,10 +18,13 @@ How it runs here:
                 reference's backend interface, e.g. a CPU table set used as a checker) runs through Sweep below.
   Sweep         the sweep of one direction over any backend object, stepping tables one column at a time, with
                 checkpoints and a traceback through recomputed stretches.
+  dense=f       (the sweeps that keep whole tables, as the reference's) a table holding more than the fraction f
+                of its cut's 2^width states (2^width <= dense_max) turns into a DenseTab, stepped by one gather
+                kernel per column (tdp_dn_step) until it holds fewer than f / 4 of them; query(dense=...) runs its
+                sweeps this way, through Sweep, on the device tables.
   split choice  probe sweeps (tdp_sq_run's profile mode) give the states of each weight at every cut; the split
                 minimises a cost model of the native runner (which keeps backward tables instead of recomputing
                 them when they fit its budget).
-There is no dense fallback: dense=... raises ValueError.
 """
 from __future__ import annotations
 
@@ -41,6 +44,7 @@ MAX_SLOTS = 120                  # key bits of a device state: 64 + 56
 M64 = (1 << 64) - 1
 U64 = np.uint64
 KEEP_BYTES = 48 << 30            # device bytes of tables a native query keeps in full before checkpointing only
+DENSE_MAX_BITS = 40              # the widest dense table (tdp_dn_*)
 
 
 def _lo_hi(x):
@@ -163,6 +167,35 @@ class DevTable:
         return self.buf.data_ptr()
 
 
+class DenseTab:
+    """A table in dense form (the sweeps' fallback, cdist/sparse_dp.py:221-289): T[i] on the device (uint8, 255 =
+    absent, 2^w entries) for the compact index i of a state, bit k of i = the state's bit at slots[k] (ascending);
+    n = the states present."""
+    __slots__ = ("T", "slots", "n", "mask")
+
+    def __init__(self, T, slots, n):
+        self.T, self.slots, self.n = T, list(slots), int(n)
+        self.mask = sum(1 << s for s in self.slots)
+
+    @property
+    def w(self):
+        return len(self.slots)
+
+    def compact(self, s):
+        """The compact index of state s (its bits outside the slots are ignored)."""
+        return _compact(s, self.slots)
+
+    def slot_array(self):
+        return np.array(self.slots or [0], dtype=np.uint8)
+
+
+def _compact(x, slots):
+    i = 0
+    for k, s in enumerate(slots):
+        i |= ((x >> s) & 1) << k
+    return i
+
+
 def _pair_array(x):
     return (ctypes.c_uint64 * 2)(x & M64, x >> 64)
 
@@ -213,10 +246,12 @@ class HipTables:
 
     @staticmethod
     def nbytes(tab):
-        return 16 * tab.n
+        return (1 << tab.w) if isinstance(tab, DenseTab) else 16 * tab.n
 
     def arrays(self, tab):
         """(keys, vals): keys as uint64 when they all fit 64 bits, else Python ints (object array)."""
+        if isinstance(tab, DenseTab):
+            tab = self.from_dense(tab)
         if not tab.n:
             return np.zeros(0, dtype=U64), np.zeros(0, dtype=np.uint8)
         raw = tab.buf[:16 * tab.n].cpu().numpy().view(U64).reshape(-1, 2)
@@ -240,6 +275,15 @@ class HipTables:
         m = len(keys)
         if not m or not tab.n:
             return np.full(m, INF, dtype=np.uint8)
+        if isinstance(tab, DenseTab):
+            inside = [not (int(k) & ~tab.mask) for k in keys]
+            idx = [tab.compact(int(k)) for k, ok in zip(keys, inside) if ok]
+            out = np.full(m, INF, dtype=np.uint8)
+            if idx:
+                t = _lib.torch()
+                got = tab.T[t.tensor(idx, dtype=t.int64, device=tab.T.device)].cpu().numpy()
+                out[np.flatnonzero(inside)] = got
+            return out
         self.index(tab)
         q = np.array([_lo_hi(int(k)) for k in keys], dtype=U64)
         dq = _lib.torch().from_numpy(q.view(np.uint8).reshape(-1)).cuda()
@@ -295,7 +339,10 @@ class HipTables:
         return ch.n
 
     def persistent(self, tab):
-        """The table in a buffer of its own size (step outputs are sized for twice their input)."""
+        """The table in a buffer of its own size (step outputs are sized for twice their input; a dense step writes
+        a new table, so a dense one is kept as it is)."""
+        if isinstance(tab, DenseTab):
+            return tab
         if tab.buf.numel() > 16 * max(1, tab.n) * 5 // 4 + 16:
             buf = self._buffer(tab.n)
             buf[:16 * tab.n].copy_(tab.buf[:16 * tab.n])
@@ -313,14 +360,83 @@ class HipTables:
         and backward states b = f ^ tmask (ties: the least forward key), or None."""
         if not ch.n or not btab.n:
             return None
+        if isinstance(btab, DenseTab):
+            sl = btab.slot_array()
+            _lib.check(self.L.tdp_dn_join(ch.ptr(), ch.n, btab.T.data_ptr(), btab.w, sl.ctypes.data,
+                                          _pair_array(tmask), self.counters.data_ptr(), self._sh), "tdp_dn_join")
+            return self._join_result(tmask)
         self.index(btab)
         _lib.check(self.L.tdp_sp_join(ch.ptr(), ch.n, btab.ptr(), btab.slots.data_ptr(), btab.capacity,
                                       _pair_array(tmask), self.counters.data_ptr(), self._sh), "tdp_sp_join")
+        return self._join_result(tmask)
+
+    def _join_result(self, tmask):
         tot, hi, lo, fval = self._read(4)
         if tot == M64:
             return None
         fkey = _join(lo, hi)
         return int(tot), fkey, int(fval), fkey ^ tmask
+
+    # ---- dense form (cdist/sparse_dp.py:221-289, 680-707)
+    @staticmethod
+    def is_dense(tab):
+        return isinstance(tab, DenseTab)
+
+    def to_dense(self, tab, slots):
+        """The sparse table `tab` as a DenseTab over the ascending `slots` (its states' bits lie in them)."""
+        dt = DenseTab(_lib.device_empty(max(4, 1 << len(slots))), slots, tab.n)
+        sl = dt.slot_array()                          # held until the call returns
+        _lib.check(self.L.tdp_dn_from_sparse(tab.ptr(), tab.n, sl.ctypes.data, dt.w, dt.T.data_ptr(),
+                                             self._sh), "tdp_dn_from_sparse")
+        return dt
+
+    def from_dense(self, dt):
+        """The DenseTab's states as a sparse table."""
+        out = self._buffer(dt.n)
+        sl = dt.slot_array()
+        _lib.check(self.L.tdp_dn_to_sparse(dt.T.data_ptr(), dt.w, sl.ctypes.data, out.data_ptr(),
+                                           max(1, dt.n), self.counters.data_ptr(), self._sh), "tdp_dn_to_sparse")
+        n, over = self._read(2)
+        if over or n != dt.n:
+            raise RuntimeError("dense table holds %d states, %d expected" % (n, dt.n))
+        return DevTable(out, n)
+
+    def dense_step(self, dt, starts, ends, upd, cap, K, tmask, hd):
+        """One column step of a dense table: rows `starts` ((slot, target) pairs) get new bits, absent where set;
+        T'[x] = min(T[x], T[x ^ upd] + 1); rows `ends` are fixed at their target and removed; states above cap or,
+        with hd = (masks, weights, extra), whose value plus the lookahead bound exceeds K are dropped."""
+        old = dt.slots
+        new_rows = sorted(s for s, _ in starts)
+        if set(new_rows) & set(old):
+            raise AssertionError("a starting row's slot is already active")
+        mid = sorted(old + new_rows)
+        pos = {s: k for k, s in enumerate(mid)}
+        ends = sorted(ends)
+        if any(s not in pos for s, _ in ends):
+            raise AssertionError("an ending row's slot is not active")
+        gone = {s for s, _ in ends}
+        new = [s for s in mid if s not in gone]
+        start_pos = np.array(sorted((pos[s] for s in new_rows), reverse=True) or [0], dtype=np.uint8)
+        end_pos = np.array([pos[s] for s, _ in ends] or [0], dtype=np.uint8)
+        end_bit = np.array([t for _, t in ends] or [0], dtype=np.uint8)
+        ncls, masks, weights, extra, ct = 0, np.zeros(1, dtype=U64), np.zeros(1, dtype=np.int64), 0, 0
+        if hd is not None:
+            by_mask = {}
+            for m, wt in zip(hd[0], hd[1]):
+                cm = _compact(int(m), new)
+                if cm:
+                    by_mask[cm] = by_mask.get(cm, 0) + int(wt)
+            ncls, extra, ct = len(by_mask), int(hd[2]), _compact(int(tmask), new)
+            if by_mask:
+                masks = np.array(list(by_mask), dtype=U64)
+                weights = np.array(list(by_mask.values()), dtype=np.int64)
+        U = _lib.device_empty(max(4, 1 << len(new)))
+        _lib.check(self.L.tdp_dn_step(dt.T.data_ptr(), dt.w, U.data_ptr(), len(mid), start_pos.ctypes.data,
+                                      len(new_rows), end_pos.ctypes.data, end_bit.ctypes.data, len(ends),
+                                      _compact(int(upd), mid), cap, K, ncls, masks.ctypes.data, weights.ctypes.data,
+                                      extra, ct, int(hd is not None), self.counters.data_ptr(), self._sh),
+                   "tdp_dn_step")
+        return DenseTab(U, new, self._read(1)[0])
 
     def hist(self, tab):
         h = _lib.device_empty(8 * 256)
@@ -362,13 +478,16 @@ class Sweep:
 
     def __init__(self, tr, direction, K, cap=None, use_h=True, every=None, frontier=False, backend="hip",
                  dense=None, dense_max=1 << 30, max_states=None, deadline=None, caps=None):
-        if dense is not None and not frontier:
-            raise ValueError("the GPU tables have no dense fallback (dense=None)")
         self.tr, self.direction, self.K = tr, direction, K
         self.caps = [min(int(x), K) for x in caps] if caps is not None else [min(cap, K)] * (tr.N + 1)
         _check_caps(self.caps, direction)
         self.use_h, self.frontier = use_h, frontier
         self.be = BACKENDS[backend](tr) if isinstance(backend, str) else backend
+        # the dense form is for the sweeps that keep whole tables (the frontier's changed states stay sparse)
+        self.dense = dense if dense and not frontier else None
+        self.dense_max = min(int(dense_max), 1 << DENSE_MAX_BITS)
+        if self.dense is not None and not hasattr(self.be, "to_dense"):
+            raise ValueError("backend %r has no dense form" % getattr(self.be, "name", self.be))
         self.hd = self._bounds() if use_h else None
         self.every = every or max(1, math.isqrt(tr.N))
         self.max_states, self.deadline = max_states, deadline
@@ -399,11 +518,28 @@ class Sweep:
             cut, sel = p, (tr.bsel_mask[p], tr.bsel_bits[p])
         self.steps_done += 1
         cap = self.caps[p] if self.frontier else self.caps[cut]
+        dense_in = self.dense is not None and self.be.is_dense(tab)
+        if dense_in and (1 << tr.width[cut]) > self.dense_max:
+            tab, dense_in = self.be.from_dense(tab), False          # too wide to stay dense
+        if dense_in:
+            fwd = self.direction > 0
+            hd = tr.bound_data(fwd)[cut] if self.use_h else None
+            tab = self.be.dense_step(tab, tr.starts_at[p] if fwd else tr.ends_at[p],
+                                     tr.ends_at[p] if fwd else tr.starts_at[p], tr.col[p], cap, self.K,
+                                     tr.tmask[cut], hd)
+            if self.be.size(tab) < self.dense / 4 * (1 << len(tab.slots)):
+                tab = self.be.from_dense(tab)
+            return cut, tab, None, None
         keep = cap if self.frontier else None
         tab, changed = self.be.step(tab, tr.col[p], sel[0], sel[1], cap, self.K, tr.tmask[cut],
                                     self.hd[cut] if self.hd else None, keep, want_changed)
         if self.frontier and self.caps[cut] < cap:
             return cut, self.be.truncate(tab, self.caps[cut]), changed, tab
+        if self.dense is not None:
+            full = 1 << tr.width[cut]
+            if self.be.size(tab) > self.dense * full and full <= self.dense_max:
+                tab = self.be.to_dense(tab, [s for s in range(tr.active[cut].bit_length())
+                                             if (tr.active[cut] >> s) & 1])
         return cut, tab, changed, None
 
     def _steps(self, cut, tab, stop, want_changed=False):
@@ -431,6 +567,8 @@ class Sweep:
             nc = self.be.changed_size(changed) if changed is not None else 0
             self.counts[c], self.changed_counts[c] = n, nc
             self.peak = max(self.peak, n, nc)
+            if self.dense is not None and self.be.is_dense(tab):
+                self.dense_cuts += 1
             if self.is_ckpt(c):
                 self.ckpt[c] = self.be.persistent(tab)
                 kept_bytes += self.be.nbytes(tab)
@@ -719,8 +857,6 @@ def query(tr, K, mode="mitm", split=None, use_h=True, witness=True, lower=0, eve
     K - a(c) and a(c) fall along the cuts while a solution's left weight rises by steps of one."""
     if mode not in ("forward", "mitm"):
         raise ValueError("mode must be 'forward' or 'mitm'")
-    if dense is not None:
-        raise ValueError("the GPU tables have no dense fallback (dense=None)")
     t0 = time.time()
     bname = backend if isinstance(backend, str) else backend.name
     rec = {"K": K, "mode": mode, "use_h": bool(use_h), "value": None, "witness": None, "width": max(tr.width),
@@ -738,10 +874,10 @@ def query(tr, K, mode="mitm", split=None, use_h=True, witness=True, lower=0, eve
             log("K=%d split a=%s: backward sweep (weights <= %s)%s" % (
                 K, _runs(sched), _runs([K - a for a in sched]),
                 "" if predicted is None else ", predicted work %.3g" % predicted))
-    if isinstance(backend, str) and backend in NATIVE:
+    if isinstance(backend, str) and backend in NATIVE and dense is None:
         return _native_query(tr, K, mode, sched, use_h, witness, lower, every, max_states, deadline, rec, t0)
     return _sweep_query(tr, K, mode, sched, use_h, witness, lower, every, backend, max_states, deadline, rec, t0,
-                        log)
+                        log, dense, dense_max)
 
 
 def _native_query(tr, K, mode, sched, use_h, witness, lower, every, max_states, deadline, rec, t0):
@@ -766,15 +902,18 @@ def _native_query(tr, K, mode, sched, use_h, witness, lower, every, max_states, 
     return rec
 
 
-def _sweep_query(tr, K, mode, sched, use_h, witness, lower, every, backend, max_states, deadline, rec, t0, log):
-    """query() over a backend object: the sweeps of this module, one column step per backend call."""
+def _sweep_query(tr, K, mode, sched, use_h, witness, lower, every, backend, max_states, deadline, rec, t0, log,
+                 dense=None, dense_max=1 << 30):
+    """query() through the sweeps of this module, one column step per backend call: for a backend object, or for
+    the device tables with the dense fallback (dense=f; the forward sweep, or a meet in the middle's backward one)."""
     def fresh():
-        return backend.__class__(tr)
+        return BACKENDS[backend](tr) if isinstance(backend, str) else backend.__class__(tr)
     if mode == "forward":
-        fw = Sweep(tr, +1, K, K, use_h, every, backend=fresh(), max_states=max_states, deadline=deadline)
+        fw = Sweep(tr, +1, K, K, use_h, every, backend=fresh(), max_states=max_states, deadline=deadline, dense=dense,
+                   dense_max=dense_max)
         fw.run()
         rec.update(forward_counts=fw.counts, peak_states=fw.peak, peak_table_bytes=fw.peak_bytes,
-                   sweep_seconds=time.time() - t0, steps=fw.steps_done, dense_cuts=0)
+                   sweep_seconds=time.time() - t0, steps=fw.steps_done, dense_cuts=fw.dense_cuts)
         v = fw.be.lookup(fw.ckpt[tr.N], 0)
         if v != INF:
             rec["value"] = v
@@ -785,7 +924,7 @@ def _sweep_query(tr, K, mode, sched, use_h, witness, lower, every, backend, max_
         rec["seconds"] = time.time() - t0
         return rec
     bw = Sweep(tr, -1, K, use_h=use_h, every=every, backend=fresh(), max_states=max_states, deadline=deadline,
-               caps=[K - a for a in sched])
+               caps=[K - a for a in sched], dense=dense, dense_max=dense_max)
     fw = Sweep(tr, +1, K, use_h=use_h, every=bw.every, frontier=True, backend=fresh(), max_states=max_states,
                deadline=deadline, caps=sched)
     bw.run()
@@ -811,7 +950,7 @@ def _sweep_query(tr, K, mode, sched, use_h, witness, lower, every, backend, max_
                peak_states=max(fw.peak, bw.peak), peak_table_bytes=fw.peak_bytes + bw.peak_bytes,
                backward_seconds=t1 - t0, forward_seconds=time.time() - t1,
                split=sched[0] if len(set(sched)) == 1 else _runs(sched), split_values=sched,
-               joined=state["joined"], steps=fw.steps_done + bw.steps_done, dense_cuts=0)
+               joined=state["joined"], steps=fw.steps_done + bw.steps_done, dense_cuts=bw.dense_cuts)
     best = state["best"]
     if best is not None and best[0] <= K:
         total, c, fkey, fval, bkey = best
