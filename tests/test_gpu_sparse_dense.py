@@ -23,14 +23,18 @@ class _Tr:
         return []
 
 
-def _random_layout(rng):
-    """Old slots, starting rows, ending rows (targets) of one column, with room in 56 slots."""
-    w = rng.randint(0, 12)
+def _random_layout(rng, w_max=12):
+    """Old slots, starting rows, ending rows (targets) of one column, with room in 56 slots.  One layout in four
+    has up to ten starting and ending rows, so that its index maps have more runs of consecutive bits than the
+    fast kernel takes (the generic kernel runs it)."""
+    many = rng.random() < 0.25
+    w = rng.randint(0, 8 if many else w_max)
     pool = rng.sample(range(56), 56)
     old = sorted(pool[:w])
-    starts = [(s, 0) for s in sorted(pool[w:w + rng.randint(0, 3)])]
+    starts = [(s, 0) for s in sorted(pool[w:w + rng.randint(0, 10 if many else 3)])]
     mid = sorted(old + [s for s, _ in starts])
-    ends = [(s, rng.randint(0, 1)) for s in sorted(rng.sample(mid, rng.randint(0, min(3, len(mid)))))]
+    nend = rng.randint(0, min(10 if many else 3, len(mid)))
+    ends = [(s, rng.randint(0, 1)) for s in sorted(rng.sample(mid, nend))]
     return old, starts, ends, mid
 
 
@@ -40,8 +44,8 @@ def test_dense_step_equals_oracle_on_random_tables():
     nrng = np.random.default_rng(21)
     torch = _lib.torch()
     hip, cpu = sparse_dp.HipTables(_Tr()), oracle.OracleTables(_Tr())
-    for trial in range(300):
-        old, starts, ends, mid = _random_layout(rng)
+    for trial in range(400):
+        old, starts, ends, mid = _random_layout(rng, 22 if trial % 50 == 0 else 12)
         T = nrng.integers(0, 12, size=1 << len(old)).astype(np.uint8)
         T[nrng.random(T.size) < nrng.random()] = 255
         upd = sum(1 << s for s in mid if rng.random() < 0.4)

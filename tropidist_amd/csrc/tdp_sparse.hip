@@ -10,19 +10,28 @@
 //   key k      (if its closing bits match): old value v against the candidate partner + 1 (partner below the cap);
 //   key k^upd  (only when the partner is absent, v below the cap, closing bits match): a new state of value v + 1.
 // Every output key arises exactly once (XOR by upd is a bijection and the filter fixes the cleared bits), so the
-// outputs need no deduplication: they are appended with one atomic per block and counter and indexed afterwards.
+// outputs need no deduplication: they are appended with one atomic per 1024 old entries and indexed afterwards.
+// Counts and least totals are gathered per block before their one atomic: atomics on one cache line serialise at
+// about 12 ns each on MI355X, whatever their kind.
 #include "tdp_sparse.h"
 
 namespace tdp {
 
-// counters: [0] out count, [1] changed count, [2] overflow flag
+// counters: [0] out count | changed count << 32 (split by k_sp_unpack), [2] overflow flag
 __global__ __launch_bounds__(kSpBlock) void k_sp_step(const Ent *__restrict__ old, uint64_t n,
                                                       const uint32_t *__restrict__ slots, uint64_t capmask,
                                                       StepArgs a, BoundArgs b, Ent *__restrict__ out,
                                                       Ent *__restrict__ ch, unsigned long long *counters) {
-    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    StepOut r = sp_step_entry(old, i, n, slots, capmask, a, b);
-    sp_step_emit(r, out, a.out_cap, ch, a.ch_cap, counters);
+    sp_step_tile<false>(old, n, (uint64_t)blockIdx.x * kSpBlock * kSpTile, slots, capmask, a, b, out, ch,
+                        &counters[0], &counters[2], nullptr, 0);
+}
+
+__global__ void k_sp_unpack(unsigned long long *counters) {
+    if (threadIdx.x == 0) {
+        const unsigned long long p = counters[0];
+        counters[0] = p & 0xFFFFFFFFull;
+        counters[1] = p >> 32;
+    }
 }
 
 __global__ __launch_bounds__(kSpBlock) void k_fill_slots(uint32_t *slots, uint64_t cap) {
@@ -46,15 +55,20 @@ __global__ __launch_bounds__(kSpBlock) void k_lookup(const Ent *__restrict__ ent
 
 __global__ __launch_bounds__(kSpBlock) void k_truncate(const Ent *__restrict__ ents, uint64_t n, int below,
                                                        Ent *__restrict__ out, unsigned long long *count) {
-    uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    bool keep = false;
-    Ent x = {0, 0};
-    if (i < n) {
-        x = ents[i];
-        keep = (int)(x.hv >> 56) < below;
+    const uint64_t base = (uint64_t)blockIdx.x * kSpBlock * kSpTile;
+    Ent x[kSpTile];
+    bool keep[kSpTile];
+    uint64_t pos[kSpTile];
+#pragma unroll
+    for (int k = 0; k < kSpTile; k++) {
+        const uint64_t i = base + (uint64_t)k * kSpBlock + threadIdx.x;
+        x[k] = i < n ? ents[i] : Ent{0, 0};
+        keep[k] = i < n && (int)(x[k].hv >> 56) < below;
     }
-    uint64_t p = wave_append(count, keep);
-    if (keep) out[p] = x;
+    block_append_tile(keep, count, pos);
+#pragma unroll
+    for (int k = 0; k < kSpTile; k++)
+        if (keep[k]) out[pos[k]] = x[k];
 }
 
 // join: best[0] = least total; then among those the least key (hi, then lo), and its forward value
@@ -62,16 +76,25 @@ __global__ __launch_bounds__(kSpBlock) void k_join_total(const Ent *__restrict__
                                                          const Ent *__restrict__ bents,
                                                          const uint32_t *__restrict__ bslots, uint64_t capmask,
                                                          uint64_t tm_lo, uint64_t tm_hi, unsigned long long *best) {
-    // one atomic per wave on best[0], which every thread of the launch would otherwise hit
-    const uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    uint32_t t = ~0u;
-    if (i < nf) {
-        Ent x = f[i];
-        uint32_t vb = sp_find(bents, bslots, capmask, x.lo ^ tm_lo, (x.hv & kKeyMask) ^ tm_hi);
-        if (vb != 255u) t = (uint32_t)(x.hv >> 56) + vb;
+    // each thread takes entries base + k * kSpBlock; one atomic per block on best[0]
+    const uint64_t base = (uint64_t)blockIdx.x * kSpBlock * kSpTile;
+    Ent x[kSpTile];
+    bool live[kSpTile];
+    uint32_t vb[kSpTile];
+#pragma unroll
+    for (int k = 0; k < kSpTile; k++) {
+        const uint64_t i = base + (uint64_t)k * kSpBlock + threadIdx.x;
+        live[k] = i < nf;
+        x[k] = live[k] ? f[i] : Ent{0, 0};
     }
-    t = wave_min_u32(t);
-    if (__lane_id() == 0 && t != ~0u) atomicMin(&best[0], (unsigned long long)t);
+    sp_find_tile(bents, bslots, capmask, x, live, tm_lo, tm_hi, vb);
+    uint32_t t = ~0u;
+#pragma unroll
+    for (int k = 0; k < kSpTile; k++) {
+        const uint32_t s = vb[k] != 255u ? (uint32_t)(x[k].hv >> 56) + vb[k] : ~0u;
+        t = s < t ? s : t;
+    }
+    block_min_atomic(t, &best[0]);
 }
 
 __global__ __launch_bounds__(kSpBlock) void k_join_pick(const Ent *__restrict__ f, uint64_t nf,
@@ -113,6 +136,12 @@ static unsigned sgrid(uint64_t items) {
     return (unsigned)(g ? g : 1);
 }
 
+// blocks of the kernels that take kSpTile entries per thread
+static unsigned tgrid(uint64_t items) {
+    uint64_t g = (items + kSpBlock * kSpTile - 1) / (kSpBlock * kSpTile);
+    return (unsigned)(g ? g : 1);
+}
+
 }  // namespace tdp
 
 using namespace tdp;
@@ -139,6 +168,7 @@ int tdp_sp_step(const void *d_old, uint64_t n, const uint32_t *d_slots, uint64_t
                 unsigned long long *d_counters, void *stream) {
     TDP_REQUIRE(capacity && !(capacity & (capacity - 1)), "index capacity %llu", (unsigned long long)capacity);
     TDP_REQUIRE(cap >= 0 && cap <= 254, "cap %d outside 0..254", cap);
+    TDP_REQUIRE(n < (1ull << 31), "sparse step of %llu states (fewer than 2^31)", (unsigned long long)n);
     hipStream_t st = as_stream(stream);
     StepArgs a;
     a.upd_lo = upd[0];
@@ -165,8 +195,10 @@ int tdp_sp_step(const void *d_old, uint64_t n, const uint32_t *d_slots, uint64_t
     k_set_u64<<<1, 64, 0, st>>>(d_counters, 0ull, 3);
     TDP_LAUNCH_CHECK();
     if (n) {
-        k_sp_step<<<sgrid(n), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_old), n, d_slots, capacity - 1, a, b,
+        k_sp_step<<<tgrid(n), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_old), n, d_slots, capacity - 1, a, b,
                                                  static_cast<Ent *>(d_out), static_cast<Ent *>(d_ch), d_counters);
+        TDP_LAUNCH_CHECK();
+        k_sp_unpack<<<1, 64, 0, st>>>(d_counters);
         TDP_LAUNCH_CHECK();
     }
     return TDP_OK;
@@ -188,7 +220,7 @@ int tdp_sp_truncate(const void *d_ents, uint64_t n, int below, void *d_out, unsi
     k_set_u64<<<1, 64, 0, st>>>(d_count, 0ull, 1);
     TDP_LAUNCH_CHECK();
     if (n) {
-        k_truncate<<<sgrid(n), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_ents), n, below,
+        k_truncate<<<tgrid(n), kSpBlock, 0, st>>>(static_cast<const Ent *>(d_ents), n, below,
                                                   static_cast<Ent *>(d_out), d_count);
         TDP_LAUNCH_CHECK();
     }
@@ -204,7 +236,7 @@ int tdp_sp_join(const void *d_f, uint64_t nf, const void *d_bents, const uint32_
     const Ent *f = static_cast<const Ent *>(d_f);
     const Ent *be = static_cast<const Ent *>(d_bents);
     uint64_t tlo = tmask[0], thi = tmask[1] & kKeyMask;
-    k_join_total<<<sgrid(nf), kSpBlock, 0, st>>>(f, nf, be, d_bslots, bcapacity - 1, tlo, thi, d_best);
+    k_join_total<<<tgrid(nf), kSpBlock, 0, st>>>(f, nf, be, d_bslots, bcapacity - 1, tlo, thi, d_best);
     TDP_LAUNCH_CHECK();
     for (int stage = 1; stage <= 3; stage++) {
         k_join_pick<<<sgrid(nf), kSpBlock, 0, st>>>(f, nf, be, d_bslots, bcapacity - 1, tlo, thi, stage, d_best);

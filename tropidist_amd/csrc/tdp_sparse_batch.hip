@@ -35,7 +35,7 @@ struct SpbStep {                      // one job's column step
     Ent *ch;                          // changed states (capacity 2n), or null
     uint64_t upd_lo, upd_hi, sm_lo, sm_hi, sb_lo, sb_hi;
     int32_t cap, kb;                  // candidates from values < cap; the table keeps values < kb
-    unsigned long long *cnt;          // [0] table count, [1] changed count, [2] overflow flag
+    unsigned long long *cnt;          // [0] table count | changed count << 32, [2] overflow flag
     uint64_t pad0, pad1;
 };
 static_assert(sizeof(SpbStep) == 128, "SpbStep layout");
@@ -81,8 +81,8 @@ constexpr int kPrmWords = 10;
 
 // ---------------------------------------------------------------------------------------------- kernels
 // Grids are njobs * per_job blocks; job j owns blocks [j * per_job, (j + 1) * per_job) and walks its entries with a
-// stride of per_job * kSpBlock.  Loop bounds depend on the block only, so every lane of a wave reaches the
-// wave-wide appends together.
+// stride of per_job * kSpBlock (per_job * kSpBlock * kSpTile for the step's tiles).  Loop bounds depend on the block
+// only, so every thread of a block reaches the block-wide appends and reductions together.
 __global__ __launch_bounds__(kSpBlock) void k_spb_step(const SpbStep *__restrict__ jobs, uint32_t per_job) {
     const SpbStep &J = jobs[blockIdx.x / per_job];
     StepArgs a;
@@ -105,11 +105,10 @@ __global__ __launch_bounds__(kSpBlock) void k_spb_step(const SpbStep *__restrict
     b.extra = 0;
     b.tm_lo = b.tm_hi = 0;
     b.K = 0;
-    const uint64_t stride = (uint64_t)per_job * kSpBlock;
-    for (uint64_t base = (uint64_t)(blockIdx.x % per_job) * kSpBlock; base < J.n; base += stride) {
-        StepOut r = sp_step_entry(J.old, base + threadIdx.x, J.n, J.slots, J.capmask, a, b);
-        sp_step_emit(r, J.out, a.out_cap, J.ch, a.ch_cap, J.cnt);
-    }
+    const uint64_t tile = (uint64_t)kSpBlock * kSpTile;
+    for (uint64_t base = (uint64_t)(blockIdx.x % per_job) * tile; base < J.n; base += (uint64_t)per_job * tile)
+        sp_step_tile<false>(J.old, J.n, base, J.slots, J.capmask, a, b, J.out, J.ch, &J.cnt[0], &J.cnt[2], nullptr,
+                            0);
 }
 
 __global__ __launch_bounds__(kSpBlock) void k_spb_fill(const SpbPost *__restrict__ jobs, uint32_t per_job) {
@@ -129,19 +128,14 @@ __global__ __launch_bounds__(kSpBlock) void k_spb_post(const SpbPost *__restrict
     if (x == 0 && threadIdx.x < 3) J.cnt[threadIdx.x] = 0;
     const uint64_t stride = (uint64_t)per_job * kSpBlock;
     for (uint64_t i = (uint64_t)x * kSpBlock + threadIdx.x; i < J.n; i += stride) sp_insert(J.ents, i, J.slots, J.capmask);
-    // the joins: one atomic per wave on the job's least total (the loop bound is uniform over the block, so every
-    // lane reaches the wave minimum)
-    for (uint64_t base = (uint64_t)x * kSpBlock; base < J.jn; base += stride) {
-        const uint64_t i = base + threadIdx.x;
-        uint32_t t = ~0u;
-        if (i < J.jn) {
-            Ent f = J.jf[i];
-            uint32_t vb = sp_find(J.jb, J.jslots, J.jcapmask, f.lo ^ J.tm_lo, (f.hv & kKeyMask) ^ J.tm_hi);
-            if (vb != 255u) t = (uint32_t)(f.hv >> 56) + vb;
-        }
-        t = wave_min_u32(t);
-        if (__lane_id() == 0 && t != ~0u) atomicMin(&J.cnt[3], (unsigned long long)t);
+    // the joins: the block's least total, one atomic per block on the job's least
+    uint32_t t = ~0u;
+    for (uint64_t i = (uint64_t)x * kSpBlock + threadIdx.x; i < J.jn; i += stride) {
+        Ent f = J.jf[i];
+        uint32_t vb = sp_find(J.jb, J.jslots, J.jcapmask, f.lo ^ J.tm_lo, (f.hv & kKeyMask) ^ J.tm_hi);
+        if (vb != 255u) t = min(t, (uint32_t)(f.hv >> 56) + vb);
     }
+    if (J.jn) block_min_atomic(t, &J.cnt[3]);
 }
 
 // One step back along the path (cdist/sparse_dp.py:812-819): the state before the column is s | bits; it stayed if
@@ -462,6 +456,7 @@ public:
             const Tab &cur = fwd ? j->fw[j->p] : j->bw[j->p + 1];
             const uint64_t n = cur.n;
             const uint64_t n1 = std::max<uint64_t>(n, 1);
+            if (n >= (1ull << 31)) { fail(j, TDP_SPB_STATES); continue; }       // the step counts in 32 bits
             Ent *ch = nullptr;
             const int par = (int)(global_steps & 1);
             if (fwd) {
@@ -512,8 +507,8 @@ public:
             if (j->phase != kDone) max_slot = std::max(max_slot, j->slot);
         if (nstep) {
             TDP_HIP_CHECK(hipMemcpyAsync(d_step, h_step, sizeof(SpbStep) * nstep, hipMemcpyHostToDevice, st));
-            uint64_t per = std::min<uint64_t>((maxn + kSpBlock - 1) / kSpBlock,
-                                              std::max<uint64_t>(1, kMaxGridBlocks / nstep));
+            const uint64_t tile = (uint64_t)kSpBlock * kSpTile;
+            uint64_t per = std::min<uint64_t>((maxn + tile - 1) / tile, std::max<uint64_t>(1, kMaxGridBlocks / nstep));
             k_spb_step<<<(unsigned)(nstep * per), kSpBlock, 0, st>>>(d_step, (uint32_t)per);
             TDP_LAUNCH_CHECK();
         }
@@ -548,8 +543,8 @@ public:
                 fail(j, TDP_SPB_FAILED);
                 continue;
             }
-            j->nout = c[0];
-            j->nch = c[1];
+            j->nout = c[0] & 0xFFFFFFFFull;
+            j->nch = c[0] >> 32;
             j->state_steps += j->nout;
             j->peak = std::max<uint64_t>(j->peak, std::max<uint64_t>(j->nout, j->nch));
             if (std::max<uint64_t>(j->nout, j->nch) > j->max_states) { fail(j, TDP_SPB_STATES); continue; }

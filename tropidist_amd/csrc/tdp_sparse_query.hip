@@ -28,58 +28,15 @@ namespace tdp {
 // ---------------------------------------------------------------------------------------------- kernels
 // The step of tdp_sparse.hip's k_sp_step with the new table's index built at append time: every table entry an old
 // entry emits is written at its reserved position p and p is placed in the first empty slot of its probe sequence
-// (output keys are distinct, so no two entries compete for a key).
+// (output keys are distinct, so no two entries compete for a key).  counters[0] = table count | changed count << 32,
+// counters[2] = overflow flag.
 __global__ __launch_bounds__(kSpBlock) void k_sq_step(const Ent *__restrict__ old, uint64_t n,
                                                       const uint32_t *__restrict__ slots, uint64_t capmask,
                                                       StepArgs a, BoundArgs b, Ent *__restrict__ out,
                                                       uint32_t *__restrict__ nslots, uint64_t ncapmask,
                                                       Ent *__restrict__ ch, unsigned long long *counters) {
-    const uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    StepOut r = sp_step_entry(old, i, n, slots, capmask, a, b);
-    constexpr int kWaves = kSpBlock / 64;
-    __shared__ uint32_t s_cnt[2][kWaves];
-    __shared__ unsigned long long s_base[2];
-    const uint32_t lane = __lane_id();
-    const uint32_t w = threadIdx.x >> 6;
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    const uint64_t bo1 = __ballot(r.o1), bo2 = __ballot(r.o2), bc1 = __ballot(r.c1), bc2 = __ballot(r.c2);
-    const uint32_t no1 = (uint32_t)__popcll(bo1), nc1 = (uint32_t)__popcll(bc1);
-    if (lane == 0) {
-        s_cnt[0][w] = no1 + (uint32_t)__popcll(bo2);
-        s_cnt[1][w] = nc1 + (uint32_t)__popcll(bc2);
-    }
-    __syncthreads();
-    if (threadIdx.x < 2) {
-        uint32_t tot = 0;
-        for (int k = 0; k < kWaves; k++) tot += s_cnt[threadIdx.x][k];
-        s_base[threadIdx.x] = tot ? atomicAdd(&counters[threadIdx.x], (unsigned long long)tot) : 0ull;
-    }
-    __syncthreads();
-    uint64_t bo = s_base[0], bc = s_base[1];
-    for (uint32_t k = 0; k < w; k++) {
-        bo += s_cnt[0][k];
-        bc += s_cnt[1][k];
-    }
-    const Ent *emit[2] = {r.o1 ? &r.e1 : nullptr, r.o2 ? &r.e2 : nullptr};
-    const uint64_t pos[2] = {bo + (uint64_t)__popcll(bo1 & lt), bo + no1 + (uint64_t)__popcll(bo2 & lt)};
-    for (int k = 0; k < 2; k++) {
-        if (!emit[k]) continue;
-        if (pos[k] >= a.out_cap) {
-            counters[2] = 1;
-            continue;
-        }
-        out[pos[k]] = *emit[k];
-        uint64_t h = hash_key(emit[k]->lo, emit[k]->hv & kKeyMask) & ncapmask;
-        while (atomicCAS(&nslots[h], kEmpty, (uint32_t)pos[k]) != kEmpty) h = (h + 1) & ncapmask;
-    }
-    if (r.c1) {
-        const uint64_t p = bc + (uint64_t)__popcll(bc1 & lt);
-        if (p < a.ch_cap) ch[p] = r.f1; else counters[2] = 1;
-    }
-    if (r.c2) {
-        const uint64_t p = bc + nc1 + (uint64_t)__popcll(bc2 & lt);
-        if (p < a.ch_cap) ch[p] = r.f2; else counters[2] = 1;
-    }
+    sp_step_tile<true>(old, n, (uint64_t)blockIdx.x * kSpBlock * kSpTile, slots, capmask, a, b, out, ch,
+                       &counters[0], &counters[2], nslots, ncapmask);
 }
 
 __global__ __launch_bounds__(kSpBlock) void k_sq_clear(uint32_t *slots, uint64_t cap) {
@@ -269,6 +226,10 @@ struct SqRun {
         const uint64_t *q = S.prm + (uint64_t)p * 10;
         const int cut = dir > 0 ? p + 1 : p;
         const uint64_t n = old.n, ncap = index_cap(2 * std::max<uint64_t>(n, 1));
+        if (n >= (1ull << 31)) {
+            set_error("sparse query: a table of %llu states (fewer than 2^31 are stepped)", (unsigned long long)n);
+            return TDP_EINVAL;
+        }
         int rc = alloc(out, 2 * n, ncap);
         if (rc) return rc;
         StepArgs a;
@@ -296,8 +257,8 @@ struct SqRun {
         k_sq_zero<<<1, 64, 0, st>>>(d_cnt, 3);
         k_sq_clear<<<grid_for(ncap, 4096), kSpBlock, 0, st>>>(out.slots, ncap);
         if (n)
-            k_sq_step<<<grid_for(n, ~0ull), kSpBlock, 0, st>>>(old.ents, n, old.slots, old.cap - 1, a, b, out.ents,
-                                                              out.slots, ncap - 1, ch, d_cnt);
+            k_sq_step<<<grid_for((n + kSpTile - 1) / kSpTile, ~0ull), kSpBlock, 0, st>>>(
+                old.ents, n, old.slots, old.cap - 1, a, b, out.ents, out.slots, ncap - 1, ch, d_cnt);
         TDP_LAUNCH_CHECK();
         steps++;
         return TDP_OK;
@@ -307,6 +268,15 @@ struct SqRun {
     int read(int nwords) {
         TDP_HIP_CHECK(hipMemcpyAsync(h_cnt, d_cnt, 8ull * nwords, hipMemcpyDeviceToHost, st));
         TDP_HIP_CHECK(hipStreamSynchronize(st));
+        return TDP_OK;
+    }
+
+    // Reads a step's counters: h_cnt[0] table count, [1] changed count, [2] overflow flag.
+    int read_step() {
+        int rc = read(3);
+        if (rc) return rc;
+        h_cnt[1] = h_cnt[0] >> 32;
+        h_cnt[0] &= 0xFFFFFFFFull;
         return TDP_OK;
     }
 
@@ -361,7 +331,7 @@ static int plain_sweep(SqRun &R, int dir, const int32_t *caps, bool use_h, Kept 
         Tab nxt;
         rc = R.step(cur, p, dir, caps[nc], 1 << 30, use_h, nxt, nullptr);
         if (rc) return rc;
-        rc = R.read(3);
+        rc = R.read_step();
         if (rc) return rc;
         if (R.h_cnt[2]) {
             set_error("sparse query: step output past its capacity");
@@ -370,7 +340,7 @@ static int plain_sweep(SqRun &R, int dir, const int32_t *caps, bool use_h, Kept 
         nxt.n = R.h_cnt[0];
         if (hist) {
             k_sq_zero<<<1, 256, 0, R.st>>>(R.d_cnt + 8, 256);
-            k_sq_hist<<<grid_for(nxt.n, 2048), kSpBlock, 0, R.st>>>(nxt.ents, nxt.n, top, R.d_cnt + 8);
+            k_sq_hist<<<grid_for(nxt.n, 256), kSpBlock, 0, R.st>>>(nxt.ents, nxt.n, top, R.d_cnt + 8);
             TDP_LAUNCH_CHECK();
             TDP_HIP_CHECK(hipMemcpyAsync(R.h_cnt + 8, R.d_cnt + 8, 8ull * (top + 1), hipMemcpyDeviceToHost, R.st));
             TDP_HIP_CHECK(hipStreamSynchronize(R.st));
@@ -419,7 +389,7 @@ static int table_at(SqRun &R, int dir, const int32_t *caps, bool use_h, Kept &ke
         Tab nxt;
         int rc = R.step(cur, p, dir, caps[nx], 1 << 30, use_h, nxt, nullptr);
         if (rc) return rc;
-        rc = R.read(3);
+        rc = R.read_step();
         if (rc) return rc;
         nxt.n = R.h_cnt[0];
         cache[nx] = nxt;
@@ -722,7 +692,7 @@ int tdp_sq_run(const tdp_sq_spec_t *spec, uint64_t *fw_counts, uint64_t *bw_coun
         const int a = S.split[p];
         rc = R.step(cur, p, +1, a, a, S.use_h != 0, nxt, ch);
         if (rc) return finish(rc);
-        rc = R.read(3);
+        rc = R.read_step();
         if (rc) return finish(rc);
         if (R.h_cnt[2]) {
             set_error("sparse query: step output past its capacity");
