@@ -779,7 +779,9 @@ def bench_sparse(args):
     p = workloads.sparse_qc()
     order = ordering.greedy_straddle_order(p.h, p.nrows)
     tr = sparse_dp.Trellis.for_query(p, 0, order)
-    sparse_dp.query(tr, 2, mode="mitm", use_h=False, backend="hip")            # warm-up
+    # warm-up at the same threshold: the timed query then finds its tables' memory in torch's caching allocator, as
+    # a served query does (a first query in a fresh process also pays the driver's mapping of tens of GB)
+    sparse_dp.query(tr, args.sparse_K, mode="mitm", use_h=False, backend="hip", max_states=400_000_000)
     t0 = time.perf_counter()
     r = sparse_dp.query(tr, args.sparse_K, mode="mitm", use_h=False, backend="hip", max_states=400_000_000)
     dt = time.perf_counter() - t0
@@ -815,6 +817,112 @@ def bench_sparse(args):
                                 "matches_gpu": rr["value"] == rh["value"]}
     out["reference_sized"] = cmp_
     return out
+
+
+def bench_sparse_dense(args):
+    """The sparse sweeps' dense fallback (query(dense=f); cdist/sparse_dp.py:221-289, 680-707): a forward sweep at
+    K over a banded model of width 24 whose tables fill their 2^width states, plain (sparse tables, the native runner)
+    and with dense = 0.25 (dense tables, one tdp_dn_step launch per column); both must give the same value and
+    per-cut counts.  The roofline of the dense step comes from HIP events around its launches on the sweep's stream:
+    algorithmic bytes per launch = 2^w_old (the old table read once) + 2^w_new (the new one written)."""
+    import ctypes
+
+    import numpy as np
+
+    import oracle
+    from tropidist_amd import _lib, problem, sparse_dp
+    torch = _lib.torch()
+    p = problem.banded(R=300, N=2000, kmin=2, kmax=6, W=24, nobs=1, rho=0.05, seed=7024, name="dense_fallback_w24")
+    tr = sparse_dp.Trellis.for_query(p, 0, list(range(p.ncols)))
+    K = args.dense_fallback_K
+    sparse_dp.query(tr, 4, mode="forward", backend="hip", use_h=False, dense=0.25, witness=False)     # warm-up
+    runs = {}
+    for name, dense in (("plain", None), ("dense", 0.25)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = sparse_dp.query(tr, K, mode="forward", backend="hip", use_h=False, dense=dense, witness=True)
+        torch.cuda.synchronize()
+        runs[name] = (r, time.perf_counter() - t0)
+    plain, dense = runs["plain"][0], runs["dense"][0]
+    if plain["value"] != dense["value"] or plain["forward_counts"] != dense["forward_counts"]:
+        raise AssertionError("dense fallback differs from the sparse sweep")
+    # the same sweep again, recording every dense step's layout; the widest steps are then replayed back to back on
+    # fresh tables between two HIP events (each launch alone would also time the host's enqueue gap)
+    be = sparse_dp.HipTables(tr)
+    launch, rec = be._dense_launch, []
+
+    def recording_launch(*a):
+        nst, nend, ncls = a[5], a[8], a[12]
+        arrays = [np.frombuffer(ctypes.string_at(a[i], max(1, n) * size), dtype=dt).copy()
+                  for i, n, size, dt in ((4, nst, 1, np.uint8), (6, nend, 1, np.uint8), (7, nend, 1, np.uint8),
+                                         (13, ncls, 8, np.uint64), (14, ncls, 8, np.int64))]
+        rec.append((a, arrays))
+        launch(*a)
+
+    be._dense_launch = recording_launch
+    sparse_dp.Sweep(tr, +1, K, K, False, backend=be, dense=0.25).run()
+    torch.cuda.synchronize()
+    top = max(a[3] - a[8] for a, _ in rec)
+    wide = [(a, arr) for a, arr in rec if a[3] - a[8] >= top - 1]
+    T = torch.randint(0, 12, (1 << max(a[1] for a, _ in wide),), dtype=torch.uint8, device="cuda")
+    U = torch.empty(1 << top, dtype=torch.uint8, device="cuda")
+
+    def replay():
+        for a, arr in wide:
+            args = list(a)
+            args[0], args[2] = T.data_ptr(), U.data_ptr()
+            args[4], args[6], args[7], args[13], args[14] = (x.ctypes.data for x in arr)
+            launch(*args)
+
+    replay()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ksec = e0.elapsed_time(e1) / 1e3
+    alg = sum((1 << a[1]) + (1 << (a[3] - a[8])) for a, _ in wide)
+    entries = sum(1 << (a[3] - a[8]) for a, _ in rec)
+    out = {"workload": "banded R=300 N=2000 k=2-6 W=24 rho=0.05 seed 7024 (25 row slots), observable 0, forward sweep "
+                       "at K=%d without lookahead, dense=0.25" % K,
+           "value": dense["value"], "dense_cuts": dense["dense_cuts"], "peak_states": dense["peak_states"],
+           "plain_seconds": runs["plain"][1], "dense_seconds": runs["dense"][1],
+           "speedup_vs_sparse": runs["plain"][1] / runs["dense"][1],
+           "dense_entries_per_s": entries / runs["dense"][1],
+           "roofline": {"bound": "hbm", "kernel": "tdp::k_dn_step_fast (16 or 4 entries per lane where bits 0-3 or "
+                                                   "0-1 pass through)", "achieved": alg / ksec / 1e9, "peak": HBM_PEAK / 1e9,
+                        "unit": "GB/s", "frac": alg / ksec / HBM_PEAK, "launches": len(wide),
+                        "widths": [top - 1, top], "alg_bytes_per_launch": alg / len(wide),
+                        "launch_seconds": ksec / len(wide),
+                        "timing": "the %d widest steps' launches replayed back to back (each call: a counter reset "
+                                  "and the step kernel) on fresh tables, HIP events on the stream" % len(wide)},
+           "witness_ok": bool(dense["witness"]) and verify_witness_cols(p, dense["witness"], dense["value"])}
+    if not args.no_cpu:
+        # the oracle's restatement of the reference's numpy Dense steps on the same sweep, stopped after ~12 s
+        sw = sparse_dp.Sweep(tr, +1, K, K, False, backend=oracle.OracleTables(tr), dense=0.25)
+        cpu = {"entries": 0, "steps": 0, "cut": 0}
+        t0 = time.perf_counter()
+
+        def count(c, tab, changed, extra):
+            cpu["cut"] = c
+            if isinstance(tab, oracle.OracleDense):
+                cpu["entries"] += tab.T.size
+                cpu["steps"] += 1
+            return time.perf_counter() - t0 > 12.0
+
+        sw.run(on_cut=count)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": cpu["entries"] / dt, "unit": "dense entries/s", "cores": 1, "kind": "port",
+                               "sample": "the first %d cuts of the same sweep (%d dense steps) through the oracle's "
+                                         "numpy restatement of the reference's Dense table" % (cpu["cut"],
+                                                                                              cpu["steps"]),
+                               "seconds": dt}
+    return out
+
+
+def verify_witness_cols(p, cols, value):
+    from tropidist_amd.problem import verify_witness
+    return verify_witness(p, sorted(cols), value, 0)
 
 
 def _serve_run(serve, reqs, rate, seed, max_states):
@@ -1037,6 +1145,7 @@ def main():
                     help="leave out the headline section (value null; for profiling the other sections alone)")
     ap.add_argument("--skip-batched", action="store_true")
     ap.add_argument("--skip-sparse", action="store_true")
+    ap.add_argument("--skip-sparse-dense", action="store_true", help="skip the sparse sweeps' dense fallback")
     ap.add_argument("--skip-tree", action="store_true")
     ap.add_argument("--skip-tables", action="store_true")
     ap.add_argument("--skip-serving", action="store_true")
@@ -1046,6 +1155,7 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=512,
                     help="problems of the scan certified one at a time for the plain-GPU rate (default: all)")
     ap.add_argument("--sparse-K", type=int, default=12)
+    ap.add_argument("--dense-fallback-K", type=int, default=10)
     ap.add_argument("--serving-requests", type=int, default=10000)
     ap.add_argument("--serving-poisson-requests", type=int, default=2000,
                     help="requests of the Poisson stream, the first ones of the mix (0: all)")
@@ -1110,6 +1220,10 @@ def main():
         out["sparse"] = bench_sparse(args)
         progress("sparse done")
         _release_cache()
+        if not args.skip_sparse_dense:
+            out["sparse"]["dense_fallback"] = bench_sparse_dense(args)
+            progress("sparse dense fallback done")
+            _release_cache()
     if not args.skip_tree:
         out["tree"] = bench_tree(args)
         progress("tree done")

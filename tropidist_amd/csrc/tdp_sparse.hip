@@ -103,7 +103,7 @@ __global__ __launch_bounds__(kSpBlock) void k_join_pick(const Ent *__restrict__ 
                                                         uint64_t tm_lo, uint64_t tm_hi, int stage,
                                                         unsigned long long *best) {
     uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x;
-    if (i >= nf) return;
+    if (i >= nf || best[0] == ~0ull) return;          // no join total: nothing to pick
     Ent x = f[i];
     uint64_t hi = x.hv & kKeyMask;
     if (stage >= 2 && hi != best[1]) return;

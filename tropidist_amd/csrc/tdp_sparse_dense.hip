@@ -12,8 +12,8 @@
 // min(T[x], T[x ^ cm] + 1) is kept if it is within the cap and the lookahead bound.  Both index maps (new -> mid,
 // mid -> old) are a few runs of consecutive bits, held in registers.  The bound is linear in the bits of j ^ tmask
 // (sum_c w_c |u & m_c| = sum_b u_b W_b, W_b = sum of the weights of the classes holding bit b), so it is read from
-// per-byte tables in LDS.  When bits 0 and 1 of the index pass through both maps, a lane handles four consecutive
-// entries with one dword load per table and one dword store; the partner's two low bits become a byte permutation.
+// per-byte tables in LDS.  When bits 0-3 (or 0-1) of the index pass through both maps, a lane handles sixteen (four)
+// consecutive entries with one load per table and one store; the partner's low bits become a byte permutation.
 // Counts are reduced per block before one atomic: atomics on one cache line serialise at about 12 ns each on MI355X,
 // so the step's blocks are few and large (1024 threads, at most kDnStepGrid of them).
 #include "tdp_sparse.h"
@@ -28,7 +28,8 @@ constexpr int kDnIdxBytes = 5;            // bytes of a 40-bit compact index
 constexpr uint32_t kDnGridCap = 1u << 20;
 constexpr uint32_t kDnLutGrid = 2048;     // blocks of the kernels that first build lookup tables in LDS
 constexpr int kDnBlock = 1024;            // threads of a step block
-constexpr uint32_t kDnStepGrid = 512;     // step blocks at most (one count atomic each)
+constexpr int kDnUnroll = 4;              // groups in flight per thread of a fast step
+constexpr uint32_t kDnStepGrid = 256;     // step blocks at most: one per CU (one count atomic each)
 constexpr uint64_t kDnChunk = 1ull << 16; // table entries per block of the dense-to-sparse gather (one atomic each)
 constexpr int64_t kDnQ = 720720;
 static_assert(kSpBlock == 256, "one LDS table entry per thread and byte value");
@@ -259,11 +260,22 @@ __global__ __launch_bounds__(kDnBlock) void k_dn_step(const uint8_t *__restrict_
     dn_block_count(kept, count);
 }
 
-// Fast step.  DW: a lane handles entries 4g .. 4g + 3 (bits 0 and 1 pass through both maps unchanged).
-template <bool DW>
+// Fast step: a lane handles W consecutive entries (16, 4 or 1: as many as the low bits that pass unchanged through
+// both index maps allow), with one load per table and one store of W bytes; the partner's low bits become a byte
+// permutation inside the loaded bytes.
+template <int W>
+__device__ __forceinline__ uint4 dn_load(const uint8_t *__restrict__ T, uint64_t y) {
+    if (W == 16) return *reinterpret_cast<const uint4 *>(T + y);
+    if (W == 4) return make_uint4(*reinterpret_cast<const uint32_t *>(T + y), 0u, 0u, 0u);
+    return make_uint4(T[y], 0u, 0u, 0u);
+}
+
+template <int W>
 __global__ __launch_bounds__(kDnBlock) void k_dn_step_fast(const uint8_t *__restrict__ T, uint8_t *__restrict__ U,
                                                            uint64_t ngroups, DnFastArgs a,
                                                            unsigned long long *count) {
+    constexpr int R = kDnUnroll;                        // groups in flight per thread
+    constexpr uint64_t low = W - 1;
     __shared__ int64_t lut[kDnIdxBytes][256];
     if (a.use_h) {
         if (threadIdx.x < 256) {
@@ -277,45 +289,54 @@ __global__ __launch_bounds__(kDnBlock) void k_dn_step_fast(const uint8_t *__rest
         }
         __syncthreads();
     }
+    const uint32_t none = W == 1 ? 255u : 0xFFFFFFFFu;
     const uint64_t stride = (uint64_t)gridDim.x * kDnBlock;
     uint32_t kept = 0;
-    for (uint64_t g = (uint64_t)blockIdx.x * kDnBlock + threadIdx.x; g < ngroups; g += stride) {
-        const uint64_t j = DW ? g << 2 : g;
-        const uint64_t x = dn_runs(j, a.run_new, a.nrun_new) | a.E;
-        const uint64_t xp = x ^ (DW ? (a.cm & ~3ull) : a.cm);
-        uint32_t v = DW ? 0xFFFFFFFFu : 255u, b = v;
-        if (!(x & a.S)) {
-            const uint64_t y = dn_runs(x, a.run_old, a.nrun_old);
-            v = DW ? *reinterpret_cast<const uint32_t *>(T + y) : T[y];
-        }
-        if (a.cm && !(xp & a.S)) {
-            const uint64_t y = dn_runs(xp, a.run_old, a.nrun_old);
-            b = DW ? *reinterpret_cast<const uint32_t *>(T + y) : T[y];
-        }
-        int64_t hi = a.extra;
-        const uint64_t u = j ^ a.tmask;
-        if (a.use_h)
-            for (int k = 1; k < a.nlut; k++) hi += lut[k][(u >> (8 * k)) & 255];
-        if (DW) {
-            if (a.cm) v = min4(v, sat_add4(xor_bytes4(b, (unsigned)(a.cm & 3)), 1u));
-            uint32_t out = 0;
+    for (uint64_t g0 = (uint64_t)blockIdx.x * kDnBlock + threadIdx.x; g0 < ngroups; g0 += R * stride) {
+        uint4 v[R], b[R];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
+        for (int r = 0; r < R; r++) {
+            const uint64_t g = g0 + r * stride;
+            v[r] = b[r] = make_uint4(none, none, none, none);
+            if (g >= ngroups) continue;
+            const uint64_t j = g * W;
+            const uint64_t x = dn_runs(j, a.run_new, a.nrun_new) | a.E;
+            const uint64_t xp = x ^ (a.cm & ~low);
+            if (!(x & a.S)) v[r] = dn_load<W>(T, dn_runs(x, a.run_old, a.nrun_old));
+            if (a.cm && !(xp & a.S)) b[r] = dn_load<W>(T, dn_runs(xp, a.run_old, a.nrun_old));
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint64_t g = g0 + r * stride;
+            if (g >= ngroups) continue;
+            const uint64_t j = g * W;
+            int64_t hi = a.extra;
+            const uint64_t u = j ^ a.tmask;
+            if (a.use_h)
+                for (int k = 1; k < a.nlut; k++) hi += lut[k][(u >> (8 * k)) & 255];
+            uint4 w = v[r];
+            if (a.cm) {
+                if (W == 16) {
+                    w = min16(w, sat_add16(xor_bytes16(b[r], (unsigned)(a.cm & 15)), 1u));
+                } else if (W == 4) {
+                    w.x = min4(w.x, sat_add4(xor_bytes4(b[r].x, (unsigned)(a.cm & 3)), 1u));
+                } else {
+                    const uint32_t b1 = b[r].x >= 254u ? 255u : b[r].x + 1;
+                    w.x = b1 < w.x ? b1 : w.x;
+                }
+            }
+            uint32_t o[4] = {0u, 0u, 0u, 0u};
+            const uint32_t in[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int i = 0; i < W; i++) {
                 const int64_t tot = a.use_h && a.nlut ? hi + lut[0][(u ^ i) & 255] : hi;
-                const uint32_t e = dn_prune((v >> (8 * i)) & 255u, a.cap, a.K, a.use_h != 0, tot);
-                out |= e << (8 * i);
+                const uint32_t e = dn_prune((in[i >> 2] >> (8 * (i & 3))) & 255u, a.cap, a.K, a.use_h != 0, tot);
+                o[i >> 2] |= e << (8 * (i & 3));
                 kept += e != 255u;
             }
-            *reinterpret_cast<uint32_t *>(U + j) = out;
-        } else {
-            if (a.cm) {
-                const uint32_t b1 = b >= 254u ? 255u : b + 1;
-                v = b1 < v ? b1 : v;
-            }
-            const int64_t tot = a.use_h && a.nlut ? hi + lut[0][u & 255] : hi;
-            v = dn_prune(v, a.cap, a.K, a.use_h != 0, tot);
-            U[j] = (uint8_t)v;
-            kept += v != 255u;
+            if (W == 16) *reinterpret_cast<uint4 *>(U + j) = make_uint4(o[0], o[1], o[2], o[3]);
+            else if (W == 4) *reinterpret_cast<uint32_t *>(U + j) = o[0];
+            else U[j] = (uint8_t)o[0];
         }
     }
     dn_block_count(kept, count);
@@ -343,6 +364,7 @@ __global__ __launch_bounds__(kSpBlock) void k_dn_join_pick(const Ent *__restrict
                                                            uint64_t tm_hi, int stage, unsigned long long *best) {
     __shared__ uint8_t sl[kDnMaxBits + 8];
     __shared__ uint64_t c[kDnKeyBytes][256];
+    if (best[0] == ~0ull) return;                     // no join total (the whole block returns): nothing to pick
     dn_lut_compact(s, sl, c);
     const uint64_t stride = (uint64_t)gridDim.x * kSpBlock;
     for (uint64_t i = (uint64_t)blockIdx.x * kSpBlock + threadIdx.x; i < nf; i += stride) {
@@ -503,13 +525,17 @@ int tdp_dn_step(const uint8_t *d_T, int w_old, uint8_t *d_U, int w_mid, const ui
                 for (int b = 0; b < w_new; b++)
                     if ((cls_mask[c] >> b) & 1) f.W[b] += cls_w[c];
         }
-        // four entries per lane when bits 0 and 1 of the new index are bits 0 and 1 of the mid and old indexes
-        if (w_new >= 2 && !((S | Emask) & 3)) {
+        // sixteen (four) entries per lane when bits 0-3 (0-1) of the new index are those of the mid and old indexes
+        if (w_new >= 4 && !((S | Emask) & 15)) {
+            const uint64_t groups = 1ull << (w_new - 4);
+            k_dn_step_fast<16><<<dgrid(groups, kDnStepGrid, kDnBlock), kDnBlock, 0, st>>>(d_T, d_U, groups, f,
+                                                                                         d_count);
+        } else if (w_new >= 2 && !((S | Emask) & 3)) {
             const uint64_t groups = 1ull << (w_new - 2);
-            k_dn_step_fast<true><<<dgrid(groups, kDnStepGrid, kDnBlock), kDnBlock, 0, st>>>(d_T, d_U, groups, f,
-                                                                                           d_count);
+            k_dn_step_fast<4><<<dgrid(groups, kDnStepGrid, kDnBlock), kDnBlock, 0, st>>>(d_T, d_U, groups, f,
+                                                                                        d_count);
         } else {
-            k_dn_step_fast<false><<<dgrid(1ull << w_new, kDnStepGrid, kDnBlock), kDnBlock, 0, st>>>(
+            k_dn_step_fast<1><<<dgrid(1ull << w_new, kDnStepGrid, kDnBlock), kDnBlock, 0, st>>>(
                 d_T, d_U, 1ull << w_new, f, d_count);
         }
         TDP_LAUNCH_CHECK();

@@ -431,12 +431,15 @@ class HipTables:
                 masks = np.array(list(by_mask), dtype=U64)
                 weights = np.array(list(by_mask.values()), dtype=np.int64)
         U = _lib.device_empty(max(4, 1 << len(new)))
-        _lib.check(self.L.tdp_dn_step(dt.T.data_ptr(), dt.w, U.data_ptr(), len(mid), start_pos.ctypes.data,
-                                      len(new_rows), end_pos.ctypes.data, end_bit.ctypes.data, len(ends),
-                                      _compact(int(upd), mid), cap, K, ncls, masks.ctypes.data, weights.ctypes.data,
-                                      extra, ct, int(hd is not None), self.counters.data_ptr(), self._sh),
-                   "tdp_dn_step")
+        self._dense_launch(dt.T.data_ptr(), dt.w, U.data_ptr(), len(mid), start_pos.ctypes.data, len(new_rows),
+                           end_pos.ctypes.data, end_bit.ctypes.data, len(ends), _compact(int(upd), mid), cap, K, ncls,
+                           masks.ctypes.data, weights.ctypes.data, extra, ct, int(hd is not None))
         return DenseTab(U, new, self._read(1)[0])
+
+    def _dense_launch(self, *args):
+        """One tdp_dn_step call on this backend's stream (args up to use_h; a method of its own so that a caller
+        can time the launch alone)."""
+        _lib.check(self.L.tdp_dn_step(*args, self.counters.data_ptr(), self._sh), "tdp_dn_step")
 
     def hist(self, tab):
         h = _lib.device_empty(8 * 256)
