@@ -110,6 +110,19 @@ __device__ __forceinline__ void insert_or_defer(const TabView &t, bool wide, uin
     }
 }
 
+// Adds the block's sum of v to *dst with one atomic (every thread of the block calls it): atomics on one cache line
+// serialise at about 12 ns each, so one per thread of a launch would take longer than the build.
+__device__ __forceinline__ void block_count_add(uint32_t v, unsigned long long *dst) {
+    __shared__ uint32_t s_sum;
+    if (threadIdx.x == 0) s_sum = 0;
+    __syncthreads();
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&s_sum, v);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_sum) atomicAdd(dst, (unsigned long long)s_sum);
+}
+
 __global__ void k_tab_clear(TabView t, uint64_t n, int wide) {
     uint64_t i = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
     if (i >= n) return;
@@ -122,19 +135,19 @@ __global__ void k_tab_clear(TabView t, uint64_t n, int wide) {
 __global__ void k_tab_small(TabView t, int wide, int n, const uint64_t *zh, const uint64_t *zl, const uint64_t *lg,
                             Retry *retry, unsigned long long *nretry, uint64_t retry_cap) {
     int j = (int)(blockIdx.x * kTabBlock + threadIdx.x) - 1;
-    if (j >= n) return;
     uint32_t added = 0;
     if (j < 0) insert_or_defer(t, wide, 0, 0, 0, 0, retry, nretry, retry_cap, added);
-    else insert_or_defer(t, wide, zh[j], zl[j], lg[j], 1, retry, nretry, retry_cap, added);
-    if (added) atomicAdd(t.count, (unsigned long long)added);
+    else if (j < n) insert_or_defer(t, wide, zh[j], zl[j], lg[j], 1, retry, nretry, retry_cap, added);
+    block_count_add(added, t.count);
 }
 
 // subsets of size 2..k with leading pair (i, j), i < j, remaining indices ascending (depth first)
-__global__ void k_tab_pairs(TabView t, int wide, int n, int k, const uint64_t *zh, const uint64_t *zl,
-                            const uint64_t *lg, Retry *retry, unsigned long long *nretry, uint64_t retry_cap) {
-    uint64_t idx = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
+// (one thread's share: returns the number of entries it added to the table)
+__device__ uint32_t tab_pairs_one(uint64_t idx, TabView t, int wide, int n, int k, const uint64_t *zh,
+                                  const uint64_t *zl, const uint64_t *lg, Retry *retry, unsigned long long *nretry,
+                                  uint64_t retry_cap) {
     uint64_t npairs = (uint64_t)n * (uint64_t)(n - 1) / 2;
-    if (idx >= npairs) return;
+    if (idx >= npairs) return 0;
     // unrank idx -> (i, j) with i < j: row i holds n - 1 - i pairs
     int i = 0;
     uint64_t rem = idx;
@@ -188,17 +201,25 @@ __global__ void k_tab_pairs(TabView t, int wide, int n, int k, const uint64_t *z
             depth--;
         }
     }
-    if (added) atomicAdd(t.count, (unsigned long long)added);
+    return added;
+}
+
+__global__ __launch_bounds__(kTabBlock) void k_tab_pairs(TabView t, int wide, int n, int k, const uint64_t *zh,
+                                                         const uint64_t *zl, const uint64_t *lg, Retry *retry,
+                                                         unsigned long long *nretry, uint64_t retry_cap) {
+    const uint64_t idx = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
+    block_count_add(tab_pairs_one(idx, t, wide, n, k, zh, zl, lg, retry, nretry, retry_cap), t.count);
 }
 
 __global__ void k_tab_retry(TabView t, const Retry *in, uint64_t m, Retry *out, unsigned long long *nout,
                             uint64_t cap) {
     uint64_t i = (uint64_t)blockIdx.x * kTabBlock + threadIdx.x;
-    if (i >= m) return;
-    Retry r = in[i];
     uint32_t added = 0;
-    insert_or_defer(t, true, r.zh, r.zl, r.lg, (uint8_t)r.cost, out, nout, cap, added);
-    if (added) atomicAdd(t.count, (unsigned long long)added);
+    if (i < m) {
+        Retry r = in[i];
+        insert_or_defer(t, true, r.zh, r.zl, r.lg, (uint8_t)r.cost, out, nout, cap, added);
+    }
+    block_count_add(added, t.count);
 }
 
 // Least cost <= R of an entry with the query's fingerprint and a different label; -1 if none (gmitm.c:136-163).
