@@ -805,9 +805,11 @@ def compile_native(plan, dmax=DMAX, min_bits=12, max_updates=None, witness=False
     L = _lib.lib()
     enc = np.ascontiguousarray(plan.encode(), dtype=np.uint64)
     nops = len(enc)
-    segs = np.zeros((nops + 2, 3), dtype=np.int64)
-    arr = np.zeros(max(1, nops + 1), dtype=PASS_DTYPE)
-    steps = np.zeros(nops + 2, dtype=WSTEP_DTYPE) if witness else None
+    # capacities by the op count (a pass descriptor is 880 bytes): left uninitialised, since the compiler writes every
+    # field of every segment, descriptor and step it reports
+    segs = np.empty((nops + 2, 3), dtype=np.int64)
+    arr = np.empty(max(1, nops + 1), dtype=PASS_DTYPE)
+    steps = np.empty(nops + 2, dtype=WSTEP_DTYPE) if witness else None
     ns, npass, nst = ctypes.c_int64(0), ctypes.c_int64(0), ctypes.c_int64(0)
     rc = L.tdp_fused_compile(enc.ctypes.data_as(ctypes.c_void_p), nops, dmax, min_bits, max_updates or 0,
                              segs.ctypes.data_as(ctypes.c_void_p), len(segs), ctypes.byref(ns),

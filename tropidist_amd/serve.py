@@ -5,11 +5,13 @@
     ticket = srv.submit_sparse(problem, K, order=None)      # least weight <= K by the sparse MITM DP, or None
     ticket.wait() -> record;  srv.close()
 
-Dense requests wait in a FIFO queue.  Each dense lane is a thread with its own HIP stream: when free, it takes the
-waiting requests (a long queue is shared with the other idle lanes, see claim_count; up to max_batch), plans them,
-admits in arrival order those whose plans fit in the free device budget, runs them as one batched launch
-(batch.certify_planned: one persistent kernel for the values, one for the witnesses) and completes them; requests
-that arrive meanwhile are admitted by the next free lane, so batches of different lanes overlap on the device.
+Dense requests are planned on the host by `planners` threads (the C++ planner and pass compiler release the
+interpreter lock), a few requests at a time in arrival order, and then wait in a FIFO queue.  Each dense lane is a
+thread with its own HIP stream: when free, it takes the waiting planned requests (a long queue is shared with the
+other idle lanes, see claim_count; up to max_batch), admits in arrival order those whose plans fit in the free
+device budget, runs them as one batched launch (batch.certify_planned: one persistent kernel for the values, one for
+the witnesses) and completes them; requests planned meanwhile are admitted by the next free lane, so batches of
+different lanes overlap on the device, and no lane waits for planning.
 
 Sparse requests go to one engine thread with its own stream and a resident sparse_batch.Engine, whose arena
 (sparse_arena_bytes, held for the server's life) is set apart from the dense budget.  Between any two engine steps
@@ -83,7 +85,7 @@ def _dense_footprint(plans):
 class Server:
     def __init__(self, memory_bytes=64 << 30, dense_lanes=2, max_batch=1024, sparse_arena_bytes=4 << 30,
                  sparse_max_states=30_000_000, sparse_admit=0.5, sparse_max_jobs=4096, prewarm=True,
-                 sparse_deepen=False):
+                 sparse_deepen=False, planners=4):
         if sparse_arena_bytes >= memory_bytes:
             raise ValueError("the sparse arena (%d bytes) must leave room for dense requests in %d bytes"
                              % (sparse_arena_bytes, memory_bytes))
@@ -94,7 +96,9 @@ class Server:
         self.max_batch = max_batch
         self.idle_lanes = 0                     # dense lanes waiting for requests
         self.share_above = 64                   # queue length from which a lane leaves a share to idle lanes
-        self.dense_q = collections.deque()
+        self.plan_q = collections.deque()       # dense requests waiting for host planning
+        self.planning = 0                       # dense requests being planned
+        self.dense_q = collections.deque()      # planned dense requests waiting for a lane
         self.sparse_q = collections.deque()
         self.sparse_failed = None               # the error that stopped the engine thread, if any
         self.cv = threading.Condition()
@@ -110,6 +114,13 @@ class Server:
         # run (a traced serving run showed single hipMalloc calls of 2 s, during which the other lanes' launches
         # waited).
         self.threads = []
+        # host planning (the C++ planner and pass compiler, which release the interpreter lock) runs on threads of
+        # its own, so that a lane's device work never waits for the planning of the requests it claims
+        self.planner_threads = []
+        for _ in range(max(1, planners)):
+            th = threading.Thread(target=self._planner, daemon=True)
+            th.start()
+            self.planner_threads.append(th)
         share = self.budget // max(1, dense_lanes)
         for k in range(dense_lanes):
             st_main, st_wide = t.cuda.Stream(), t.cuda.Stream()
@@ -138,7 +149,7 @@ class Server:
     def submit_dense(self, problem, order=None):
         tk = Ticket("dense", problem, order, None)
         with self.cv:
-            self.dense_q.append(tk)
+            self.plan_q.append(tk)
             self.cv.notify_all()
         return tk
 
@@ -157,7 +168,7 @@ class Server:
         with self.cv:
             self.closed = True
             self.cv.notify_all()
-        for th in self.threads:
+        for th in self.planner_threads + self.threads:
             th.join()
         self.engine_thread.join()               # may still hand requests to the single-query lane
         with self.cv:
@@ -195,26 +206,42 @@ class Server:
             tk.footprint = _dense_footprint([p for p in flat if p.feasible])
         return tk
 
-    def _dense_lane(self, stream, wide_stream):
-        t = _lib.torch()
+    PLAN_CHUNK = 8                              # requests a planner takes at once
+
+    def _planner(self):
+        """Plans waiting dense requests a few at a time and hands them to the lanes (dense_q)."""
         while True:
             with self.cv:
-                self.idle_lanes += 1
-                while not self.dense_q and not self.closed:
+                while not self.plan_q and not self.closed:
                     self.cv.wait()
-                self.idle_lanes -= 1
-                if not self.dense_q and self.closed:
+                if not self.plan_q:
                     return
-                n = claim_count(len(self.dense_q), self.idle_lanes, self.max_batch, self.share_above)
-                claimed = [self.dense_q.popleft() for _ in range(n)]
+                take = [self.plan_q.popleft() for _ in range(min(self.PLAN_CHUNK, len(self.plan_q)))]
+                self.planning += len(take)
             planned = []
-            for tk in claimed:                    # host planning outside the lock, one request at a time
+            for tk in take:
                 try:
                     planned.append(self._plan(tk))
                 except Exception as exc:          # this request alone is answered with its error
                     tk.plans, tk.footprint = None, 0
                     self._finish([tk], error=exc)
-            claimed = planned
+            with self.cv:
+                self.planning -= len(take)
+                self.dense_q.extend(planned)
+                self.cv.notify_all()
+
+    def _dense_lane(self, stream, wide_stream):
+        t = _lib.torch()
+        while True:
+            with self.cv:
+                self.idle_lanes += 1
+                while not self.dense_q and not (self.closed and not self.plan_q and not self.planning):
+                    self.cv.wait()
+                self.idle_lanes -= 1
+                if not self.dense_q:
+                    return                        # closed, and nothing left to plan
+                n = claim_count(len(self.dense_q), self.idle_lanes, self.max_batch, self.share_above)
+                claimed = [self.dense_q.popleft() for _ in range(n)]
             while claimed:
                 with self.cv:
                     # admit the longest prefix that fits the free budget (at least one request when idle)
