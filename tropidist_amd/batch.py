@@ -178,9 +178,8 @@ def _columns(plan, order, used):
 
 def plan_problems(problems, orders=None, compile_wide=True):
     """Host planning of a batch with the C++ planner: (orders, flat list of plans, owner (problem, observable) of
-    each plan).  compile_wide: also compile the fused value programs of plans with 2^WIDE_BITS entries or more,
-    which the wide path runs (cached on the plans); a witness program is compiled when its observable turns out to
-    be the minimising one."""
+    each plan).  compile_wide: also compile the fused programs (value and witness) of plans with 2^WIDE_BITS
+    entries or more, which the wide path runs (cached on the plans), so that the device phase does no compiling."""
     from .trellis import build_plans_native
     orders = orders or [auto.planning_order(p) for p in problems]
     flat, owner = [], []
@@ -195,6 +194,7 @@ def plan_problems(problems, orders=None, compile_wide=True):
             if plan.feasible and plan.peak_bits >= WIDE_BITS:
                 try:
                     fused.program_for(plan, min_bits=2)
+                    fused.witness_program_for(plan)
                 except ValueError:          # runs on the per-op executor
                     pass
     return orders, flat, owner
