@@ -912,16 +912,11 @@ static int build_schedule(const tdp_run_t *runs, int nruns, Schedule &S) {
         to[r] = runs[r].steps ? nullptr : runs[r].d_T1;
     }
     S.finals.assign(nruns, nullptr);
-    // Programs take turns in the launch in alternating order: blocks are dispatched about in index order, so the
-    // program whose table the last round wrote last (still in the memory-side cache) is read first.
-    uint64_t nround = 0;
     while (true) {
         const size_t j0 = S.jobs.size();
         uint64_t nblocks = 0;
         uint64_t round_bytes = 0;
-        const bool backward = (nround++ & 1) != 0;
-        for (int ri = 0; ri < nruns; ri++) {
-            const int r = backward ? nruns - 1 - ri : ri;
+        for (int r = 0; r < nruns; r++) {
             const tdp_run_t &R = runs[r];
             // per-op work on a tiny table is this round's job of the program (its next pass waits a round)
             auto foldable = [&](const Ev &e) {
