@@ -76,6 +76,18 @@ int tdp_hist(const uint8_t *d_T, uint64_t n, uint64_t *d_hist, void *stream);
 int tdp_run_plan(const uint64_t *ops, int64_t nops, int peak_bits, uint8_t *d_T0, uint8_t *d_T1, int *value,
                  void *stream);
 
+/* The same plan in one launch of one workgroup, its tables in LDS, for plans of at most 2^tdp_plan_small_bits()
+ * entries.  The host checks the op stream as tdp_run_plan does and groups it into steps: an optional expand, a run
+ * of updates with linearly independent masks, an optional select.  Each step runs with one barrier; one thread
+ * takes each coset of the run's masks.  The steps are uploaded into d_work (tdp_plan_small_bytes(nops) bytes of
+ * device memory) and the result is read back.  The result equals tdp_run_plan's.  On the per-op executor a small
+ * plan costs a launch per op, and on the fused one a launch per pass; here it costs one launch.  Counterpart: the
+ * loop of run_plan (cdist/kernel.py:94-138) over the ops of cdist/viterbi.c. */
+int tdp_plan_small_bits(void);
+int tdp_plan_small_bytes(int64_t nops, uint64_t *bytes);
+int tdp_run_plan_small(const uint64_t *ops, int64_t nops, int peak_bits, void *d_work, uint64_t work_bytes,
+                       int *value, void *stream);
+
 /* The same plan by fused passes: one kernel applies up to five update columns (with the expands and selects of
  * their rows) to register-resident cubes of the table, so a pass reads 2^b_in and writes 2^b_out bytes for all
  * of them.  segments: nseg triples of int64 (0, first, last) = ops [first, last) by the per-op kernels, or (1, k, 0)

@@ -262,3 +262,62 @@ def test_program_graph_replays_equal_run_many():
         assert fused.run_many(plans[:2]) == want[:2]          # eager work of the same program between replays
         t.cuda.synchronize()
     g.close()
+
+
+
+def _run_small(plan, peak_bits=None, work_bytes=None):
+    """tdp_run_plan_small through the C-ABI: (status, raw value)."""
+    L = _lib.lib()
+    ops = plan.encode()
+    need = ctypes.c_uint64(0)
+    _lib.check(L.tdp_plan_small_bytes(len(ops), ctypes.byref(need)), "tdp_plan_small_bytes")
+    work = _lib.device_empty(need.value)
+    value = ctypes.c_int(-1)
+    rc = L.tdp_run_plan_small(ops.ctypes.data_as(ctypes.c_void_p), len(ops),
+                              plan.peak_bits if peak_bits is None else peak_bits, work.data_ptr(),
+                              need.value if work_bytes is None else work_bytes, ctypes.byref(value),
+                              _lib.stream_handle())
+    return rc, value.value
+
+
+@pytest.mark.parametrize("W,weights", [(6, None), (8, None), (10, (1, 3)), (12, (1, 2)), (13, None)])
+def test_small_plan_in_one_workgroup(W, weights):
+    """Plans of at most 2^14 entries run whole in one workgroup's LDS: the value equals the oracle's and the
+    per-op executor's, and run_plan's default takes that path for them."""
+    print("case small plan W", W, flush=True)
+    p = problem.banded(R=48, N=300, kmin=1, kmax=4, W=W, nobs=2, rho=0.1, weights=weights, seed=1000 + W)
+    L = _lib.lib()
+    for obs in range(p.nobs):
+        plan = trellis.build_plan(p.h, p.l, 1 << obs, list(range(p.ncols)), p.nrows, weights=p.w)
+        if not plan.feasible:
+            continue
+        assert plan.peak_bits <= L.tdp_plan_small_bits()
+        want = oracle.run_plan_c(plan, "oracle")
+        rc, v = _run_small(plan)
+        assert rc == 0 and kernel.final_value(v) == want
+        got, stats = kernel.run_plan(plan)
+        assert got == want == kernel.run_plan(plan, executor="per-op")[0] and stats["executor"] == "small"
+
+
+def test_small_plan_refusals():
+    """The one-workgroup path refuses a plan wider than its LDS tables, a peak below the plan's, a workspace
+    smaller than it needs, and a malformed op stream, before any launch; a wide plan asked for as "small" raises."""
+    L = _lib.lib()
+    p = problem.banded(R=48, N=300, kmin=1, kmax=4, W=8, nobs=1, rho=0.1, seed=1008)
+    plan = trellis.build_plan(p.h, p.l, 1, list(range(p.ncols)), p.nrows)
+    assert plan.feasible and plan.peak_bits >= 3
+    assert _run_small(plan, peak_bits=L.tdp_plan_small_bits() + 1)[0] == _lib.TDP_EINVAL
+    assert _run_small(plan, peak_bits=plan.peak_bits - 1)[0] == _lib.TDP_EINVAL
+    assert _run_small(plan, work_bytes=16)[0] == _lib.TDP_EINVAL
+    bad = plan.encode()[:-1].copy()                   # drops the final select: the stream ends with 2 entries
+    need = ctypes.c_uint64(0)
+    _lib.check(L.tdp_plan_small_bytes(len(bad), ctypes.byref(need)), "tdp_plan_small_bytes")
+    work = _lib.device_empty(need.value)
+    value = ctypes.c_int(-1)
+    assert L.tdp_run_plan_small(bad.ctypes.data_as(ctypes.c_void_p), len(bad), plan.peak_bits, work.data_ptr(),
+                                need.value, ctypes.byref(value), _lib.stream_handle()) == _lib.TDP_EINVAL
+    wide = problem.banded(R=80, N=500, kmin=2, kmax=5, W=16, nobs=1, rho=0.1, seed=77)
+    wplan = trellis.build_plan(wide.h, wide.l, 1, list(range(wide.ncols)), wide.nrows)
+    if wplan.peak_bits > L.tdp_plan_small_bits():
+        with pytest.raises(ValueError):
+            kernel.run_plan(wplan, executor="small")

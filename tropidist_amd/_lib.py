@@ -27,6 +27,9 @@ SIGNATURES = {
     "tdp_fill": (ci, [vp, u64, u8, vp]),
     "tdp_hist": (ci, [vp, u64, vp, vp]),
     "tdp_run_plan": (ci, [vp, i64, ci, vp, vp, ctypes.POINTER(ci), vp]),
+    "tdp_plan_small_bits": (ci, []),
+    "tdp_plan_small_bytes": (ci, [i64, ctypes.POINTER(u64)]),
+    "tdp_run_plan_small": (ci, [vp, i64, ci, vp, u64, ctypes.POINTER(ci), vp]),
     "tdp_pass_args_bytes": (ci, []),
     "tdp_run_pass": (ci, [vp, vp, vp, vp]),
     "tdp_witness_step_bytes": (ci, []),

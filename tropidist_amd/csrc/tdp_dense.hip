@@ -9,6 +9,8 @@
 // Semantics follow cdist/viterbi.c:72-219 (saturating min-plus butterflies on old values).
 #include "tdp_common.h"
 
+#include <vector>
+
 namespace tdp {
 
 static __thread char g_err[512];
@@ -137,6 +139,114 @@ __global__ __launch_bounds__(kBlock) void k_hist(const uint8_t *__restrict__ T, 
 __global__ void k_zero_u64(unsigned long long *p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------------------------------------- small plans
+// A whole plan of at most 2^kSmallBits entries in one workgroup, its two tables in LDS, with a barrier between
+// steps.  The host groups the op stream into steps: an optional expand, a run of up to kSmallRun updates whose
+// masks are linearly independent over GF(2), and an optional select.  A step's updates pair an entry with its
+// translates by the masks, so the table falls into cosets of their span, 2^r entries each.  One thread loads a
+// coset, applies the run's updates to it in op order, and stores it.
+//   - The coset of representative s is {s ^ X_c}, with X_c the XOR of the masks picked by the bits of c.
+//   - Update u pairs c with c ^ 2^u.
+//   - The representatives are the indices with zeros at the pivot bits of the span's echelon basis (each nonzero
+//     element of the span has its top bit at a pivot, so the cosets partition the table).
+//   - An expand is folded in by reading the new half as 255.
+//   - A select is folded in by storing only the kept entries, compacted, into the other table.
+// One launch and about a third as many barriers as ops replace a launch per op (a few hundred for the
+// reference's CPU case).  A step is 16 bytes, staged in LDS a chunk at a time, so no step waits on a global load:
+//   word 0: expand | select << 1 | position << 2 | target << 6 | r << 7 | pivots (4 bits each, ascending) << 10
+//   words 1-2: the run's masks, 16 bits each; word 3: the run's weights, 8 bits each (saturated at 255).
+constexpr int kSmallBits = 14;
+constexpr int kSmallRun = 4;
+constexpr int kSmallBlock = 1024;
+constexpr int kSmallSteps = 1024;       // steps per LDS chunk (16 KB)
+
+template <int R, int BLOCK>
+__device__ __forceinline__ void small_step(const uint8_t *cur, uint8_t *dst, uint32_t n, uint32_t nold, uint4 st) {
+    const bool E = st.x & 1, S = (st.x >> 1) & 1;
+    const uint32_t pos = (st.x >> 2) & 15, tb = (st.x >> 6) & 1;
+    const uint32_t low = (1u << pos) - 1;
+    uint32_t X[1 << R], w[R > 0 ? R : 1];
+    X[0] = 0;
+#pragma unroll
+    for (int c = 1; c < (1 << R); c++) {
+        const int q = __builtin_ctz(c);
+        const uint32_t a = ((q < 2 ? st.y : st.z) >> (16 * (q & 1))) & 0xFFFF;
+        X[c] = X[c & (c - 1)] ^ a;
+    }
+#pragma unroll
+    for (int u = 0; u < R; u++) w[u] = (st.w >> (8 * u)) & 255;
+    for (uint32_t g = threadIdx.x; g < (n >> R); g += BLOCK) {
+        uint32_t s = g;
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            const uint32_t p = (st.x >> (10 + 4 * q)) & 15;
+            s = ((s >> p) << (p + 1)) | (s & ((1u << p) - 1));
+        }
+        uint32_t v[1 << R];
+#pragma unroll
+        for (int c = 0; c < (1 << R); c++) {
+            const uint32_t idx = s ^ X[c];
+            v[c] = (E && idx >= nold) ? 255u : cur[idx];
+        }
+#pragma unroll
+        for (int u = 0; u < R; u++) {
+#pragma unroll
+            for (int c = 0; c < (1 << R); c++) {
+                if (c & (1 << u)) continue;
+                const uint32_t x = v[c], y = v[c | (1 << u)];
+                const uint32_t xw = x + w[u] > 255u ? 255u : x + w[u], yw = y + w[u] > 255u ? 255u : y + w[u];
+                v[c] = yw < x ? yw : x;
+                v[c | (1 << u)] = xw < y ? xw : y;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < (1 << R); c++) {
+            const uint32_t idx = s ^ X[c];
+            if (!S) dst[idx] = (uint8_t)v[c];
+            else if (((idx >> pos) & 1) == tb) dst[(idx & low) | ((idx >> (pos + 1)) << pos)] = (uint8_t)v[c];
+        }
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_plan_small(const uint4 *__restrict__ steps, int64_t nsteps,
+                                                      uint8_t *value) {
+    __shared__ uint8_t tab[2][1 << kSmallBits];
+    __shared__ uint4 lsteps[kSmallSteps];
+    uint8_t *cur = tab[0], *oth = tab[1];
+    uint32_t n = 1;
+    if (threadIdx.x == 0) cur[0] = 0;
+    for (int64_t base = 0; base < nsteps; base += kSmallSteps) {
+        const int m = (int)(nsteps - base < kSmallSteps ? nsteps - base : kSmallSteps);
+        __syncthreads();                                    // the previous chunk's steps are all read
+        for (int j = threadIdx.x; j < m; j += BLOCK) lsteps[j] = steps[base + j];
+        __syncthreads();
+        for (int j = 0; j < m; j++) {
+            const uint4 raw = lsteps[j];
+            const uint4 st = make_uint4(__builtin_amdgcn_readfirstlane(raw.x), __builtin_amdgcn_readfirstlane(raw.y),
+                                        __builtin_amdgcn_readfirstlane(raw.z), __builtin_amdgcn_readfirstlane(raw.w));
+            const uint32_t nold = n;
+            if (st.x & 1) n <<= 1;
+            uint8_t *dst = ((st.x >> 1) & 1) ? oth : cur;
+            switch ((st.x >> 7) & 7) {
+            case 0: small_step<0, BLOCK>(cur, dst, n, nold, st); break;
+            case 1: small_step<1, BLOCK>(cur, dst, n, nold, st); break;
+            case 2: small_step<2, BLOCK>(cur, dst, n, nold, st); break;
+            case 3: small_step<3, BLOCK>(cur, dst, n, nold, st); break;
+            default: small_step<4, BLOCK>(cur, dst, n, nold, st); break;
+            }
+            if ((st.x >> 1) & 1) {
+                oth = cur;
+                cur = dst;
+                n >>= 1;
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *value = cur[0];
 }
 
 // ------------------------------------------------------------------------------------------------- launchers
@@ -310,6 +420,109 @@ int tdp_hist(const uint8_t *d_T, uint64_t n, uint64_t *d_hist, void *stream) {
     if (blocks > 4096) blocks = 4096;
     k_hist<<<(unsigned)blocks, kBlock, 0, st>>>(d_T, n, reinterpret_cast<unsigned long long *>(d_hist));
     TDP_LAUNCH_CHECK();
+    return TDP_OK;
+}
+
+int tdp_plan_small_bits(void) { return kSmallBits; }
+
+int tdp_plan_small_bytes(int64_t nops, uint64_t *bytes) {
+    TDP_REQUIRE(nops >= 0 && bytes, "bad op count");
+    *bytes = ((uint64_t)nops * 16 + 255) / 256 * 256 + 256;       // at most one step per op, then the result
+    return TDP_OK;
+}
+
+int tdp_run_plan_small(const uint64_t *ops, int64_t nops, int peak_bits, void *d_work, uint64_t work_bytes,
+                       int *value, void *stream) {
+    hipStream_t st = as_stream(stream);
+    TDP_REQUIRE(peak_bits >= 0 && peak_bits <= kSmallBits, "a small plan holds at most 2^%d entries (2^%d asked)",
+                kSmallBits, peak_bits);
+    TDP_REQUIRE(nops >= 0 && (nops == 0 || ops), "bad op stream");
+    uint64_t need = 0;
+    int rc = tdp_plan_small_bytes(nops, &need);
+    if (rc) return rc;
+    TDP_REQUIRE(d_work && work_bytes >= need, "small-plan workspace of %llu bytes, %llu needed",
+                (unsigned long long)work_bytes, (unsigned long long)need);
+    // The checks of the per-op executor, on the host, before anything is launched, while the ops are grouped
+    // into steps.  A run of updates ends at the first mask in the span of the run's earlier masks (a mask of 0
+    // updates nothing and is dropped).
+    const uint64_t cap = 1ull << peak_bits;
+    std::vector<uint32_t> steps;
+    steps.reserve((size_t)nops * 4);
+    uint64_t n = 1;
+    int64_t i = 0;
+    auto bad_op = [&](int64_t k) -> int {
+        const uint64_t code = ops[3 * k], a = ops[3 * k + 1], b = ops[3 * k + 2];
+        if (code == 0) {
+            TDP_REQUIRE(2 * n <= cap, "op %lld: expand beyond 2^%d entries", (long long)k, peak_bits);
+        } else if (code == 1) {
+            TDP_REQUIRE(a < n && b >= 1, "op %lld: bad update", (long long)k);
+        } else if (code == 2) {
+            TDP_REQUIRE(n >= 2 && a < (uint64_t)top_bit(n) && b <= 1, "op %lld: bad select", (long long)k);
+        } else {
+            TDP_REQUIRE(false, "op %lld: unknown code %llu", (long long)k, (unsigned long long)code);
+        }
+        return TDP_OK;
+    };
+    while (i < nops) {
+        uint32_t w0 = 0, masks[kSmallRun] = {0, 0, 0, 0}, wts = 0;
+        uint32_t basis[kSmallRun];
+        int r = 0;
+        if (ops[3 * i] == 0) {
+            if ((rc = bad_op(i))) return rc;
+            w0 |= 1;
+            n <<= 1;
+            i++;
+        }
+        // at least 256 cosets a step: a longer run on a small table leaves a few waves with a long serial chain
+        const int run = top_bit(n) - 8 < 1 ? 1 : top_bit(n) - 8 > kSmallRun ? kSmallRun : top_bit(n) - 8;
+        while (i < nops && ops[3 * i] == 1 && r < run) {
+            if ((rc = bad_op(i))) return rc;
+            const uint32_t a = (uint32_t)ops[3 * i + 1];
+            uint32_t x = a;
+            for (int q = 0; q < r; q++)                 // basis kept with distinct top bits, highest first
+                if ((x >> top_bit(basis[q])) & 1) x ^= basis[q];
+            if (a != 0 && x == 0) break;                // in the span: the next step's
+            if (a != 0) {
+                int q = r;
+                while (q > 0 && top_bit(basis[q - 1]) < top_bit(x)) {
+                    basis[q] = basis[q - 1];
+                    q--;
+                }
+                basis[q] = x;
+                masks[r] = a;
+                wts |= (uint32_t)(ops[3 * i + 2] > 255 ? 255 : ops[3 * i + 2]) << (8 * r);
+                r++;
+            }
+            i++;
+        }
+        if (i < nops && ops[3 * i] == 2) {
+            if ((rc = bad_op(i))) return rc;
+            w0 |= 2 | (uint32_t)ops[3 * i + 1] << 2 | (uint32_t)ops[3 * i + 2] << 6;
+            i++;
+        } else if (i < nops && ops[3 * i] != 0 && ops[3 * i] != 1) {
+            return bad_op(i);
+        }
+        for (int q = 0; q < r; q++) w0 |= (uint32_t)top_bit(basis[r - 1 - q]) << (10 + 4 * q);    // ascending
+        w0 |= (uint32_t)r << 7;
+        if (w0 & 2) n >>= 1;
+        if (w0 == 0) continue;                          // updates of mask 0 only
+        steps.push_back(w0);
+        steps.push_back(masks[0] | masks[1] << 16);
+        steps.push_back(masks[2] | masks[3] << 16);
+        steps.push_back(wts);
+    }
+    TDP_REQUIRE(n == 1, "op stream ends with %llu entries, not 1", (unsigned long long)n);
+    const int64_t nsteps = (int64_t)steps.size() / 4;
+    uint8_t *w = static_cast<uint8_t *>(d_work);
+    const uint64_t off_val = need - 256;
+    if (nsteps) TDP_HIP_CHECK(hipMemcpyAsync(w, steps.data(), (size_t)nsteps * 16, hipMemcpyHostToDevice, st));
+    const uint4 *d_steps = reinterpret_cast<const uint4 *>(w);
+    k_plan_small<kSmallBlock><<<1, kSmallBlock, 0, st>>>(d_steps, nsteps, w + off_val);
+    TDP_LAUNCH_CHECK();
+    uint8_t v = 0;
+    TDP_HIP_CHECK(hipMemcpyAsync(&v, w + off_val, 1, hipMemcpyDeviceToHost, st));
+    TDP_HIP_CHECK(hipStreamSynchronize(st));
+    *value = v;
     return TDP_OK;
 }
 

@@ -34,18 +34,36 @@ EXECUTORS = ("fused", "per-op")
 def run_plan(plan, log=None, progress_every=0.0, stream=None, executor="fused"):
     """Run the plan on the GPU.  Returns (min cost or None if infeasible, stats); OverflowError if the cost is
     >= 255 (cdist/kernel.py:94-138).  `plan` is a Plan or any object with the reference's plan attributes.
-    stats: seconds, peak_bits, cus (= threads), executor."""
-    if executor not in EXECUTORS:
-        raise ValueError("executor must be one of %s" % (EXECUTORS,))
+    executor: "fused" (passes; a plan of at most 2^tdp_plan_small_bits() entries runs whole in one workgroup's LDS
+    instead, "small"), "per-op" (one launch per op) or "small".  stats: seconds, peak_bits, cus (= threads),
+    executor."""
+    if executor not in EXECUTORS + ("small",):
+        raise ValueError("executor must be one of %s" % (EXECUTORS + ("small",),))
     from .trellis import as_plan
     plan = as_plan(plan)
     if not plan.feasible:
         return None, {"feasible": False}
     L = _lib.lib()
+    small = plan.peak_bits <= L.tdp_plan_small_bits()
+    if executor == "small" and not small:
+        raise ValueError("a plan of 2^%d entries is not small (at most 2^%d)" % (plan.peak_bits,
+                                                                                L.tdp_plan_small_bits()))
+    if executor == "fused" and small:
+        executor = "small"
+    t0 = time.time()
+    if executor == "small":
+        ops = plan.encode()
+        need = ctypes.c_uint64(0)
+        _lib.check(L.tdp_plan_small_bytes(len(ops), ctypes.byref(need)), "tdp_plan_small_bytes")
+        work = _lib.device_empty(need.value)
+        value = ctypes.c_int(-1)
+        _lib.check(L.tdp_run_plan_small(ops.ctypes.data_as(ctypes.c_void_p), len(ops), plan.peak_bits,
+                                        work.data_ptr(), need.value, ctypes.byref(value),
+                                        _lib.stream_handle(stream)), "tdp_run_plan_small")
+        return _finish(L, plan, value.value, executor, t0, log)
     size = 1 << plan.peak_bits
     T0 = _lib.device_empty(size)
     T1 = _lib.device_empty(size)
-    t0 = time.time()
     if executor == "fused":
         from . import fused
         try:
@@ -61,6 +79,10 @@ def run_plan(plan, log=None, progress_every=0.0, stream=None, executor="fused"):
                             _lib.dptr(T1), ctypes.byref(value), _lib.stream_handle(stream))
         _lib.check(rc, "tdp_run_plan")
         raw = value.value
+    return _finish(L, plan, raw, executor, t0, log)
+
+
+def _finish(L, plan, raw, executor, t0, log):
     seconds = time.time() - t0
     if log:
         log("  plan of %d ops on 2^%d entries (%s): %.4f s" % (plan.nops, plan.peak_bits, executor, seconds))

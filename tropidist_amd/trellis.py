@@ -177,6 +177,8 @@ class Plan:
     ncols: int
     feasible: bool = True
     columns: list = field(default_factory=list)     # column position (in the order) of each update op
+    _encoded: object = field(default=None, init=False, repr=False, compare=False)
+    _encoded_key: object = field(default=None, init=False, repr=False, compare=False)
 
     def work(self):
         """Table entries touched by updates: sum over updates of the table size at that update."""
@@ -192,15 +194,25 @@ class Plan:
 
     def encode(self):
         """The op stream as an (nops, 3) uint64 array of (code, a, b): expand (0, 0, 0), update (1, mask, weight),
-        select (2, pos, target) -- the format of the C-ABI plan executors."""
-        out = np.zeros((len(self.ops), 3), dtype=np.uint64)
-        for i, op in enumerate(self.ops):
-            if op[0] == "expand":
-                out[i, 0] = OP_EXPAND
-            elif op[0] == "update":
-                out[i] = (OP_UPDATE, op[1], op[2] if len(op) > 2 else 1)
+        select (2, pos, target) -- the format of the C-ABI plan executors.  The array is read-only and kept on the
+        plan while `ops` is the same list of the same length, so a plan run many times is encoded once (assign a
+        new list to change a plan's ops; edits in place are not seen)."""
+        key = self._encoded_key
+        if key is not None and key[0] is self.ops and key[1] == len(self.ops):
+            return self._encoded
+        flat = []
+        put = flat.extend
+        for op in self.ops:
+            kind = op[0]
+            if kind == "expand":
+                put((OP_EXPAND, 0, 0))
+            elif kind == "update":
+                put((OP_UPDATE, op[1], op[2] if len(op) > 2 else 1))
             else:
-                out[i] = (OP_SELECT, op[1], op[2])
+                put((OP_SELECT, op[1], op[2]))
+        out = np.fromiter(flat, dtype=np.uint64, count=len(flat)).reshape(-1, 3)
+        out.setflags(write=False)
+        self._encoded, self._encoded_key = out, (self.ops, len(self.ops))   # holds the list: its id stays its own
         return out
 
     def max_weight(self):
