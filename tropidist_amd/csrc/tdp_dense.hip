@@ -193,12 +193,11 @@ __device__ __forceinline__ void small_step(const uint8_t *cur, uint8_t *dst, uin
 #pragma unroll
         for (int u = 0; u < R; u++) {
 #pragma unroll
-            for (int c = 0; c < (1 << R); c++) {
+            for (int c = 0; c < (1 << R); c++) {       // no clamp: min(x, y + w) <= x <= 255
                 if (c & (1 << u)) continue;
                 const uint32_t x = v[c], y = v[c | (1 << u)];
-                const uint32_t xw = x + w[u] > 255u ? 255u : x + w[u], yw = y + w[u] > 255u ? 255u : y + w[u];
-                v[c] = yw < x ? yw : x;
-                v[c | (1 << u)] = xw < y ? xw : y;
+                v[c] = min(x, y + w[u]);
+                v[c | (1 << u)] = min(y, x + w[u]);
             }
         }
 #pragma unroll
