@@ -144,8 +144,8 @@ def measured_traffic(name, kernel, alg_bytes):
 
 
 def step_roofline(plans, stream, step_seconds):
-    """Roofline of the step's dominant kernel, k_pass_multi (one launch per round: the next pass of each of the
-    four programs): one more step with HIP events recorded on the step's stream around every round
+    """Roofline of the step's dominant kernel, k_pass_round (one launch per round: the next pass of each of the
+    four programs, the round's jobs passed by value): one more step with HIP events recorded on the step's stream around every round
     (tdp_run_programs_timed), and each round's algorithmic bytes (every input entry read once, every output entry
     written once).  achieved = all rounds' bytes / all rounds' time; the widest rounds alone beside it."""
     import numpy as np
@@ -158,10 +158,10 @@ def step_roofline(plans, stream, step_seconds):
     wide_s = float(us[wide].sum()) * 1e-6
     return {"bound": "hbm", "achieved": alg / kernel_s / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": alg / kernel_s / HBM_PEAK,
-            "traffic": measured_traffic("step_pass_multi_banded_w25", "k_pass_multi", alg / len(us)),
+            "traffic": measured_traffic("step_pass_multi_banded_w25", "k_pass_round", alg / len(us)),
             "traffic_source": "profiles/step_pass_multi_banded_w25.json (rocprofv3 FETCH_SIZE / WRITE_SIZE of every "
-                              "k_pass_multi launch of one step, scaled by a device copy of 2^26 bytes)",
-            "kernel": "tdp::k_pass_multi", "launches": int(len(us)), "alg_bytes_per_launch": alg / len(us),
+                              "k_pass_round launch of one step, scaled by a device copy of 2^26 bytes)",
+            "kernel": "tdp::k_pass_round", "launches": int(len(us)), "alg_bytes_per_launch": alg / len(us),
             "launch_seconds": kernel_s / len(us), "kernel_seconds_per_step": kernel_s,
             "kernel_share_of_step": kernel_s / step_seconds, "step_alg_bytes": alg,
             "step_frac": alg / step_seconds / HBM_PEAK,
@@ -1060,7 +1060,7 @@ def summarise_step_profile(args):
     trace_csv, fetch_csv, write_csv, log = args.summarise_step_profile
     with open(log) as f:
         info = [json.loads(line) for line in f if line.startswith("{")][-1]
-    kernel = "k_pass_multi"
+    kernel = "k_pass_round"         # rounds of at most 8 jobs; k_pass_multi runs larger rounds (none in this step)
     stats = [r for r in _csv_rows(trace_csv) if kernel in r["Name"]]
     if not stats:
         raise SystemExit("no %s row in %s" % (kernel, trace_csv))
@@ -1086,7 +1086,7 @@ def summarise_step_profile(args):
     alg = info["alg_bytes_per_launch"]
     out = {"kernel": st["Name"],
            "workload": "the headline step: the four observables' fused programs of the banded R=600 N=6000 W=25 "
-                       "instance (seed 2000) as multi-program rounds, one k_pass_multi launch per round",
+                       "instance (seed 2000) as multi-program rounds, one k_pass_round launch per round",
            "alg_bytes_per_launch": alg, "launches_per_step": info["launches_per_step"],
            "alg_bytes_per_step": info["alg_bytes_per_step"],
            "copy_kernel": {"name": "__amd_rocclr_copyBuffer", "bytes_read": cb, "bytes_written": cb,

@@ -526,6 +526,35 @@ __device__ void tiny_ops(const OpJob &J) {
 // The same bound spills 16 bytes per lane here; the headline step of four programs is still 2% faster with it than
 // at five waves.
 // NT: nontemporal input loads (rounds larger than the memory-side cache, kNtBytes).
+// One block's share of one job of a round: an op job (one thread), or its cube of a pass.
+template <bool NT>
+__device__ __forceinline__ void pass_job(const PassArgs *__restrict__ descs, const uint8_t *in, uint8_t *out,
+                                         uint64_t nthreads, uint32_t desc, uint32_t first_block) {
+    if (desc == kOpJob) {
+        if (threadIdx.x == 0) tiny_ops(*reinterpret_cast<const OpJob *>(in));
+        return;
+    }
+    const uint64_t local = (uint64_t)(blockIdx.x - first_block);
+    if (local * kPassBlock + threadIdx.x >= nthreads) return;
+    const PassArgs &a = descs[desc];
+    const uint64_t bb = local * kPassBlock;
+    // one compiled shape per (D, store mode); the branch is uniform over the block
+    switch (a.D * 2 + (a.byte_store ? 1 : 0)) {
+        case 10: pass_cube<5, false, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 11: pass_cube<5, true, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 8: pass_cube<4, false, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 9: pass_cube<4, true, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 6: pass_cube<3, false, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 7: pass_cube<3, true, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 4: pass_cube<2, false, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 5: pass_cube<2, true, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 2: pass_cube<1, false, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 3: pass_cube<1, true, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        case 0: pass_cube<0, false, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+        default: pass_cube<0, true, -1, uint32_t, NT>(in, out, bb, threadIdx.x, a); break;
+    }
+}
+
 template <bool NT>
 __global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass_multi(const PassArgs *__restrict__ descs,
                                                            const Job *__restrict__ jobs, uint32_t njobs) {
@@ -539,29 +568,34 @@ __global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass_multi(const Pas
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(&s_count, c);
     __syncthreads();
     const Job &J = jobs[s_count - 1];
-    if (J.desc == kOpJob) {
-        if (threadIdx.x == 0) tiny_ops(*reinterpret_cast<const OpJob *>(J.in));
-        return;
-    }
-    const uint64_t local = (uint64_t)(blockIdx.x - J.first_block);
-    if (local * kPassBlock + threadIdx.x >= J.nthreads) return;
-    const PassArgs &a = descs[J.desc];
-    const uint64_t bb = local * kPassBlock;
-    // one compiled shape per (D, store mode); the branch is uniform over the block
-    switch (a.D * 2 + (a.byte_store ? 1 : 0)) {
-        case 10: pass_cube<5, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 11: pass_cube<5, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 8: pass_cube<4, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 9: pass_cube<4, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 6: pass_cube<3, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 7: pass_cube<3, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 4: pass_cube<2, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 5: pass_cube<2, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 2: pass_cube<1, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 3: pass_cube<1, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        case 0: pass_cube<0, false, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-        default: pass_cube<0, true, -1, uint32_t, NT>(J.in, J.out, bb, threadIdx.x, a); break;
-    }
+    pass_job<NT>(descs, J.in, J.out, J.nthreads, J.desc, J.first_block);
+}
+
+// A round of at most kRoundJobs jobs, its jobs passed by value: a block finds its job by scalar compares of kernel
+// arguments instead of loading the job list, the job and then the descriptor one after another.
+constexpr int kRoundJobs = 8;
+struct RoundJobs {
+    Job job[kRoundJobs];
+    uint32_t njobs;
+};
+
+template <bool NT>
+__global__ __launch_bounds__(kPassBlock, kPassWaves) void k_pass_round(const PassArgs *__restrict__ descs,
+                                                                       const RoundJobs R) {
+    const uint8_t *in = R.job[0].in;
+    uint8_t *out = R.job[0].out;
+    uint64_t nthreads = R.job[0].nthreads;
+    uint32_t desc = R.job[0].desc, first = R.job[0].first_block;
+#pragma unroll
+    for (int k = 1; k < kRoundJobs; k++)            // uniform: constant indices, scalar selects
+        if ((uint32_t)k < R.njobs && blockIdx.x >= R.job[k].first_block) {
+            in = R.job[k].in;
+            out = R.job[k].out;
+            nthreads = R.job[k].nthreads;
+            desc = R.job[k].desc;
+            first = R.job[k].first_block;
+        }
+    pass_job<NT>(descs, in, out, nthreads, desc, first);
 }
 
 struct TraceJob {
@@ -1258,10 +1292,19 @@ static int run_programs(tdp_run_t *runs, int nruns, uint8_t *d_work, uint64_t wo
                 evs.push_back(e1);
                 TDP_HIP_CHECK(hipEventRecord(e0, st));
             }
-if (act.nt)
-                k_pass_multi<true><<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, (uint32_t)act.b);
-            else
-                k_pass_multi<false><<<(unsigned)act.nblocks, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, (uint32_t)act.b);
+const unsigned nbk = (unsigned)act.nblocks;
+            const uint32_t nj = (uint32_t)act.b;
+            if (nj <= (uint32_t)kRoundJobs) {
+                RoundJobs rj{};
+                for (uint32_t k = 0; k < nj; k++) rj.job[k] = S.jobs[act.a + k];
+                rj.njobs = nj;
+                if (act.nt) k_pass_round<true><<<nbk, kPassBlock, 0, st>>>(d_descs, rj);
+                else k_pass_round<false><<<nbk, kPassBlock, 0, st>>>(d_descs, rj);
+            } else if (act.nt) {
+                k_pass_multi<true><<<nbk, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, nj);
+            } else {
+                k_pass_multi<false><<<nbk, kPassBlock, 0, st>>>(d_descs, d_jobs + act.a, nj);
+            }
             TDP_LAUNCH_CHECK();
             if (timed) TDP_HIP_CHECK(hipEventRecord(evs.back(), st));
         }
